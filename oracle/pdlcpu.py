@@ -1,0 +1,310 @@
+"""ctypes binding of the ORACLE (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg.  The product package (pdl_amd) never imports it.
+
+`OPdl` is a minimal host-side ndarray in PDL's layout (dim 0 fastest, a root
+buffer plus composed virtual-affine dims/incs/offs, PDL's bad-value state)
+with the zero-copy views of Slices.pd (xchg, mv, dummy, slice) and
+`run(op, ...)`, which calls the C restatement of pdl_run_<op>.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liboracle.so")
+
+SB, B, S, US, L, UL, IND, ULL, LL, F, D, LD, CF, CD, CLD = range(15)
+TYPE_NAMES = ["sbyte", "byte", "short", "ushort", "long", "ulong", "indx", "ulonglong",
+              "longlong", "float", "double", "ldouble", "cfloat", "cdouble", "cldouble"]
+NP_TYPES = {SB: np.int8, B: np.uint8, S: np.int16, US: np.uint16, L: np.int32, UL: np.uint32,
+            IND: np.int64, ULL: np.uint64, LL: np.int64, F: np.float32, D: np.float64}
+NOMYDIMS, BADVAL, VAFFINE = 1, 2, 4
+MAXDIMS = 16
+
+
+class Value(ctypes.Union):
+    _fields_ = [("A", ctypes.c_int8), ("B", ctypes.c_uint8), ("S", ctypes.c_int16),
+                ("U", ctypes.c_uint16), ("L", ctypes.c_int32), ("K", ctypes.c_uint32),
+                ("N", ctypes.c_int64), ("P", ctypes.c_uint64), ("Q", ctypes.c_int64),
+                ("F", ctypes.c_float), ("D", ctypes.c_double)]
+
+
+class Anyval(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("pad_", ctypes.c_int32), ("value", Value)]
+
+
+class CPdl(ctypes.Structure):
+    _fields_ = [("datatype", ctypes.c_int32), ("state", ctypes.c_int32), ("ndims", ctypes.c_int32),
+                ("has_badvalue", ctypes.c_int32), ("dims", ctypes.c_int64 * MAXDIMS),
+                ("incs", ctypes.c_int64 * MAXDIMS), ("offs", ctypes.c_int64),
+                ("data", ctypes.c_void_p), ("badvalue", Anyval)]
+
+
+class CError(ctypes.Structure):
+    _fields_ = [("error", ctypes.c_int32), ("message", ctypes.c_char * 4096)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        l = ctypes.CDLL(LIBPATH)
+        l.pdlc_run.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(CPdl)), ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_int64), ctypes.c_int, ctypes.c_int,
+                               ctypes.POINTER(CError)]
+        l.pdlc_run.restype = ctypes.c_int
+        l.pdlc_free.argtypes = [ctypes.c_void_p]
+        l.pdlc_find_max_pthread.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_int)]
+        l.pdlc_random_fill.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64]
+        l.pdlc_xoshiro256plus_next.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
+        l.pdlc_xoshiro256plus_next.restype = ctypes.c_uint64
+        l.pdlc_splitmix64_next.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
+        l.pdlc_splitmix64_next.restype = ctypes.c_uint64
+        l.pdlc_truth_matmult_ld.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int64] * 3
+        l.pdlc_convert.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]
+        _lib = l
+    return _lib
+
+
+class OracleError(Exception):
+    pass
+
+
+def default_badval(t):
+    info = {SB: -128, B: 255, S: -32768, US: 65535, L: -2**31, UL: 2**32 - 1, IND: -2**63,
+            ULL: 2**64 - 1, LL: -2**63}
+    if t in info:
+        return info[t]
+    if t == F:
+        return -np.finfo(np.float32).max
+    if t == D:
+        return -np.finfo(np.float64).max
+    raise ValueError(t)
+
+
+class OPdl:
+    """Host ndarray in PDL layout (dim 0 fastest)."""
+
+    def __init__(self, root, dtype, dims, incs=None, offs=0, state=0, badvalue=None):
+        self.root = root            # 1-D numpy array (the root buffer), or None for null
+        self.dtype = dtype
+        self.dims = list(dims)
+        if incs is None:
+            incs, inc = [], 1
+            for d in self.dims:
+                incs.append(inc)
+                inc *= d
+        self.incs = list(incs)
+        self.offs = offs
+        self.state = state
+        self.badvalue = badvalue    # None -> type default
+        self.root_dims = list(self.dims)
+
+    # ---- constructors ------------------------------------------------------
+    @classmethod
+    def from_numpy(cls, arr, dtype=None, badflag=False, badvalue=None):
+        arr = np.asarray(arr)
+        if dtype is None:
+            dtype = {np.dtype(v): k for k, v in NP_TYPES.items() if k != IND}.get(arr.dtype, D)
+        root = np.ascontiguousarray(arr, dtype=NP_TYPES[dtype]).reshape(-1).copy()
+        p = cls(root, dtype, list(arr.shape[::-1]))
+        if badflag:
+            p.state |= BADVAL
+        p.badvalue = badvalue
+        return p
+
+    @classmethod
+    def null(cls):
+        return cls(None, -1, [], state=NOMYDIMS)
+
+    @property
+    def nvals(self):
+        n = 1
+        for d in self.dims:
+            n *= d
+        return n
+
+    def isnull(self):
+        return bool(self.state & NOMYDIMS)
+
+    # ---- views (Slices.pd affine transforms) ---------------------------------
+    def _view(self, dims, incs, offs):
+        v = OPdl(self.root, self.dtype, dims, incs, offs, (self.state & BADVAL) | VAFFINE, self.badvalue)
+        v.root_dims = self.root_dims
+        v.parent = self
+        return v
+
+    def root_pdl(self):
+        p = self
+        while getattr(p, "parent", None) is not None:
+            p = p.parent
+        p.state |= self.state & BADVAL
+        return p
+
+    def xchg(self, i, j):
+        dims, incs = list(self.dims), list(self.incs)
+        dims[i], dims[j] = dims[j], dims[i]
+        incs[i], incs[j] = incs[j], incs[i]
+        return self._view(dims, incs, self.offs)
+
+    def mv(self, i, j):
+        dims, incs = list(self.dims), list(self.incs)
+        d, c = dims.pop(i), incs.pop(i)
+        dims.insert(j, d)
+        incs.insert(j, c)
+        return self._view(dims, incs, self.offs)
+
+    def dummy(self, pos, size=1):
+        dims, incs = list(self.dims), list(self.incs)
+        if pos < 0:
+            pos = len(dims) + 1 + pos
+        while len(dims) < pos:
+            dims.append(1)
+            incs.append(0)
+        dims.insert(pos, size)
+        incs.insert(pos, 0)
+        return self._view(dims, incs, self.offs)
+
+    def slice(self, *specs):
+        """specs per dim: None (all), int k -> '(k)' (drop dim), (start, stop, step) inclusive stop
+        as PDL's 'a:b:s'."""
+        dims, incs, offs = [], [], self.offs
+        for d in range(len(self.dims)):
+            sp = specs[d] if d < len(specs) else None
+            if sp is None:
+                dims.append(self.dims[d])
+                incs.append(self.incs[d])
+            elif isinstance(sp, int):
+                k = sp if sp >= 0 else self.dims[d] + sp
+                offs += k * self.incs[d]
+            else:
+                a, b, s = sp
+                a = a if a >= 0 else self.dims[d] + a
+                b = b if b >= 0 else self.dims[d] + b
+                n = (b - a) // s + 1
+                dims.append(max(n, 0))
+                incs.append(self.incs[d] * s)
+                offs += a * self.incs[d]
+        return self._view(dims, incs, offs)
+
+    # ---- data ------------------------------------------------------------------
+    def gather(self):
+        """dense numpy copy, shape = reversed dims"""
+        return self.root[self.lin_index()].copy()
+
+    def lin_index(self):
+        """root-buffer index of every element, numpy shape = reversed dims"""
+        lin = np.full(tuple(self.dims[::-1]), self.offs, dtype=np.int64)
+        for d in range(len(self.dims)):
+            shape = [1] * len(self.dims)
+            shape[len(self.dims) - 1 - d] = self.dims[d]
+            lin = lin + (np.arange(self.dims[d], dtype=np.int64) * self.incs[d]).reshape(shape)
+        return lin
+
+    def numpy(self):
+        return self.gather()
+
+    def badval(self):
+        return self.badvalue if self.badvalue is not None else default_badval(self.dtype)
+
+    def isbad_mask(self):
+        v = self.gather()
+        bv = self.badval()
+        if isinstance(bv, float) and np.isnan(bv):
+            return np.isnan(v)
+        return v == np.asarray(bv).astype(v.dtype)
+
+    # ---- ctypes --------------------------------------------------------------
+    def to_c(self):
+        c = CPdl()
+        c.datatype = self.dtype
+        c.state = self.state
+        c.ndims = len(self.dims)
+        for i, d in enumerate(self.dims):
+            c.dims[i] = d
+            c.incs[i] = self.incs[i]
+        c.offs = self.offs
+        c.data = self.root.ctypes.data if self.root is not None else None
+        if self.badvalue is not None:
+            c.has_badvalue = 1
+            c.badvalue.type = self.dtype
+            setattr(c.badvalue.value, "ABSUKLNPQFD"[[SB, B, S, US, L, UL, IND, ULL, LL, F, D].index(self.dtype)],
+                    self.badvalue)
+        return c
+
+    def from_c(self, c, created):
+        self.state = c.state
+        if self.state & BADVAL and getattr(self, "parent", None) is not None:
+            self.root_pdl()
+        if created:
+            self.dtype = c.datatype
+            self.dims = [c.dims[i] for i in range(c.ndims)]
+            self.incs = [c.incs[i] for i in range(c.ndims)]
+            self.offs = 0
+            n = max(self.nvals, 1)
+            size = np.dtype(NP_TYPES[self.dtype]).itemsize
+            buf = (ctypes.c_char * (n * size)).from_address(c.data)
+            self.root = np.frombuffer(bytes(buf), dtype=NP_TYPES[self.dtype]).copy()
+            lib().pdlc_free(c.data)
+
+
+def run(op, *pdls, swap=0, nthreads=0, size_mi=1, m_size=None):
+    """pdl_run_<op>(pdls..., swap). Raises OracleError with PDL's message."""
+    cs = [p.to_c() for p in pdls]
+    # the same OPdl passed twice (inplace) must be one C struct
+    uniq = {}
+    ptrs = (ctypes.POINTER(CPdl) * len(pdls))()
+    for i, p in enumerate(pdls):
+        if id(p) not in uniq:
+            uniq[id(p)] = cs[i]
+        ptrs[i] = ctypes.pointer(uniq[id(p)])
+    created = [p.isnull() for p in pdls]
+    other = (ctypes.c_int64 * 1)(swap if m_size is None else m_size)
+    err = CError()
+    rc = lib().pdlc_run(op.encode(), ptrs, len(pdls), other, nthreads, size_mi, ctypes.byref(err))
+    if rc:
+        for i, p in enumerate(pdls):
+            if created[i] and cs[i].data and not (cs[i].state & NOMYDIMS):
+                pass
+        raise OracleError(err.message.decode())
+    done = set()
+    for i, p in enumerate(pdls):
+        if id(p) in done:
+            continue
+        done.add(id(p))
+        p.from_c(uniq[id(p)], created[i])
+    return pdls
+
+
+def find_max_pthread(dims, target, realdims=0):
+    arr = (ctypes.c_int64 * max(len(dims), 1))(*dims)
+    mp, md = ctypes.c_int(0), ctypes.c_int(-1)
+    lib().pdlc_find_max_pthread(arr, len(dims), realdims, target, ctypes.byref(mp), ctypes.byref(md))
+    return mp.value, md.value
+
+
+def random_uniform(n, seed):
+    """n draws of PDL's pdl_drand from one xoshiro256+ stream (autopthread off)."""
+    out = np.empty(n, dtype=np.float64)
+    lib().pdlc_random_fill(out.ctypes.data, n, seed)
+    return out
+
+
+def truth_matmult_ld(a, b):
+    """row-major A(h,t) @ B(t,w) accumulated in long double"""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    h, t = a.shape
+    w = b.shape[1]
+    c = np.empty((h, w), dtype=np.float64)
+    lib().pdlc_truth_matmult_ld(a.ctypes.data, b.ctypes.data, c.ctypes.data, h, t, w)
+    return c
