@@ -1,0 +1,188 @@
+"""Benchmark of PDL's broadcast-loop path on one MI355X.
+
+Workload ("broadcast + reductions at HBM scale"):
+  x float32 (4096,4096,16) normal(0,1); y float32 (4096,16) -> y->dummy(1)
+  z = x->xchg(0,1) * y->dummy(1)                    (transposed view * dummy dim)
+  sumover(z), average(z), minimum(z), maximum(z)    (along dim 0)
+  int32 (4096,4096) uniform[-2^20,2^20]: sumover, cumusumover   (bit-exact)
+  int16 (8192,8192), 1% BAD (SHRT_MIN) + one all-bad row: sumover -> int32
+One step = that whole chain; inputs resident in HBM, outputs preallocated.
+value = algorithmic bytes of the step / step time (GB/s).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-matmult]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "binop/sumover HBM GB/s (%peak); matmult TFLOP/s split-FP16 vs FP32-SIMT @err"
+
+
+def algorithmic_bytes(nx, ny, nk, ni, nj, mi, mj):
+    """distinct elements of every operand and output x size (dummy reuse counted once)"""
+    f4, i4, i2 = 4, 4, 2
+    binop = nx * ny * nk * f4 * 2 + nx * nk * f4
+    red = nx * ny * nk * f4 + ny * nk * f4
+    isum = ni * nj * i4 + nj * i4
+    iscan = ni * nj * i4 * 2
+    bsum = mi * mj * i2 + mj * i4
+    return {"binop": binop, "sumover": red, "average": red, "minimum": red, "maximum": red,
+            "i32_sumover": isum, "i32_cumusumover": iscan, "i16bad_sumover": bsum}
+
+
+def make_int_inputs(ni, nj, mi, mj, seed=0):
+    rng = np.random.default_rng(seed)
+    i32 = rng.integers(-2**20, 2**20, size=(nj, ni), endpoint=True, dtype=np.int32)
+    i16 = rng.integers(-32767, 32767, size=(mj, mi), endpoint=True, dtype=np.int16)
+    flat = i16.reshape(-1)
+    flat[rng.random(flat.size) < 0.01] = -32768
+    i16[17, :] = -32768                       # an all-bad row
+    return i32, i16
+
+
+def run_gpu(args):
+    from pdl_amd import core as C
+
+    C.init(0)
+    NX, NY, NK = 4096, 4096, 16
+    NI, NJ = 4096, 4096
+    MI, MJ = 8192, 8192
+    x = C.random_normal(NX, NY, NK, dtype=C.L.F, seed=1)          # PDL dims (4096,4096,16)
+    y = C.random_normal(NY, NK, dtype=C.L.F, seed=2)              # (4096,16)
+    xv = x.xchg(0, 1)
+    yv = y.dummy(1)
+    z = C.ndarray.new_dense(C.L.F, [NY, NX, NK])
+    outs = {k: C.ndarray.new_dense(C.L.F, [NX, NK]) for k in ("sumover", "average", "minimum", "maximum")}
+    i32h, i16h = make_int_inputs(NI, NJ, MI, MJ)
+    i32 = C.ndarray.from_numpy(i32h, C.L.L)
+    i16 = C.ndarray.from_numpy(i16h, C.L.S, badflag=True)
+    is_out = C.ndarray.new_dense(C.L.L, [NJ])
+    ic_out = C.ndarray.new_dense(C.L.L, [NI, NJ])
+    ib_out = C.ndarray.new_dense(C.L.L, [MJ])
+    ib_out.badflag(True)
+
+    ops = [
+        ("binop", lambda: C._run("mult", [xv, yv, z])),
+        ("sumover", lambda: C._run("sumover", [z, outs["sumover"]])),
+        ("average", lambda: C._run("average", [z, outs["average"]])),
+        ("minimum", lambda: C._run("minimum", [z, outs["minimum"]])),
+        ("maximum", lambda: C._run("maximum", [z, outs["maximum"]])),
+        ("i32_sumover", lambda: C._run("sumover", [i32, is_out])),
+        ("i32_cumusumover", lambda: C._run("cumusumover", [i32, ic_out])),
+        ("i16bad_sumover", lambda: C._run("sumover", [i16, ib_out])),
+    ]
+    nbytes = algorithmic_bytes(NX, NY, NK, NI, NJ, MI, MJ)
+    total_bytes = sum(nbytes.values())
+    for _ in range(args.warmup):
+        for _, f in ops:
+            f()
+    C.synchronize()
+    evs = [[(C.Event(), C.Event()) for _ in ops] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    C.synchronize()
+    w0 = C.Event().record()
+    for s in range(args.steps):
+        for k, (_, f) in enumerate(ops):
+            evs[s][k][0].record()
+            f()
+            evs[s][k][1].record()
+    w1 = C.Event().record()
+    C.synchronize()
+    t1 = time.perf_counter()
+    step_ms_events = w0.elapsed_ms(w1) / args.steps
+    step_ms_wall = (t1 - t0) * 1000 / args.steps
+    per_op = {}
+    for k, (name, _) in enumerate(ops):
+        ms = float(np.mean([evs[s][k][0].elapsed_ms(evs[s][k][1]) for s in range(args.steps)]))
+        gbs = nbytes[name] / (ms * 1e-3) / 1e9
+        per_op[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 3)}
+    ms_per_step = step_ms_wall
+    value = total_bytes / (ms_per_step * 1e-3) / 1e9
+    dom = max(per_op, key=lambda k: per_op[k]["ms"])
+    # parity spot check of this run's results (bit-exact parts)
+    check = {}
+    got = is_out.numpy()
+    check["i32_sumover_exact"] = bool(np.array_equal(got, i32h.astype(np.int64).sum(axis=1).astype(np.int32)))
+    got = ic_out.numpy()
+    check["i32_cumusumover_exact"] = bool(np.array_equal(got, np.cumsum(i32h.astype(np.int64), axis=1).astype(np.int32)))
+    good = i16h != -32768
+    want = np.where(good, i16h.astype(np.int64), 0).sum(axis=1)
+    want = np.where(good.any(axis=1), want, -2**31).astype(np.int32)
+    check["i16bad_sumover_exact"] = bool(np.array_equal(ib_out.numpy(), want))
+    return {
+        "value": value, "ms_per_step": ms_per_step, "ms_per_step_events": step_ms_events,
+        "per_op": per_op, "dominant": dom, "dominant_bytes": nbytes[dom], "total_bytes": total_bytes,
+        "check": check,
+    }
+
+
+def run_cpu_baseline(frac_k=2):
+    """The oracle (the C restatement of PDL's CPU path) on a bounded sample of
+    the same workload: x (4096,4096,frac_k) instead of 16 k-planes, the int
+    arrays at 1/4 of their rows; PDL's autopthread rule with every host core."""
+    from oracle import pdlcpu as O
+
+    nthr = os.cpu_count() or 1
+    rng = np.random.default_rng(3)
+    NX, NY, NK = 4096, 4096, frac_k
+    x = rng.standard_normal((NK, NY, NX), dtype=np.float32)
+    y = rng.standard_normal((NK, NY), dtype=np.float32)
+    i32h, i16h = make_int_inputs(4096, 1024, 8192, 2048)
+    px = O.OPdl.from_numpy(x, O.F).xchg(0, 1)
+    py = O.OPdl.from_numpy(y, O.F).dummy(1)
+    p32 = O.OPdl.from_numpy(i32h, O.L)
+    p16 = O.OPdl.from_numpy(i16h, O.S, badflag=True)
+    nb = algorithmic_bytes(NX, NY, NK, 4096, 1024, 8192, 2048)
+    t0 = time.perf_counter()
+    z = O.OPdl.null()
+    O.run("mult", px, py, z, nthreads=nthr)
+    for op in ("sumover", "average", "minimum", "maximum"):
+        O.run(op, z, O.OPdl.null(), nthreads=nthr)
+    O.run("sumover", p32, O.OPdl.null(), nthreads=nthr)
+    O.run("cumusumover", p32, O.OPdl.null(), nthreads=nthr)
+    O.run("sumover", p16, O.OPdl.null(), nthreads=nthr)
+    dt = time.perf_counter() - t0
+    return {"value": round(sum(nb.values()) / dt / 1e9, 2), "unit": "GB/s", "cores": nthr, "kind": "port",
+            "sample": "same chain at x (4096,4096,%d), int32 (4096,1024), int16 (8192,2048); %.2f s" % (NK, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus %d refused)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    r = run_gpu(args)
+    per = r["per_op"][r["dominant"]]
+    out = {
+        "metric": METRIC, "value": round(r["value"], 1), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(r["ms_per_step"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+i32+i16", "data": "synthetic",
+        "config": {"workload": "broadcast+reductions at HBM scale: z=x->xchg(0,1)*y->dummy(1) f32 (4096,4096,16); "
+                               "sumover/average/minimum/maximum dim 0; i32 (4096,4096) sumover+cumusumover; "
+                               "i16 (8192,8192) 1% BAD sumover",
+                   "bytes_per_step": r["total_bytes"], "frac_of_hbm_peak": round(r["value"] / HBM_PEAK_GBS, 3)},
+        "roofline": {"bound": "hbm", "achieved": per["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": per["frac"], "traffic": None, "kernel": r["dominant"]},
+        "per_op": r["per_op"], "checks": r["check"],
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = run_cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

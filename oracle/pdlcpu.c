@@ -721,6 +721,8 @@ static void rd_append(trans *tr, int thr) {
   {nm, 3, 2, {"a", "b", "c"}, {0, 0, 0}, {{0}}, {0}, {0, 0, P_ISOUT | P_ISCREAT}, {-1, -1, -1}, gt, rd, opc, mode}
 #define RED_VT(nm, opc, oflags, otype) \
   {nm, 2, 1, {"a", "b"}, {1, 0}, {{0}, {0}}, {"n"}, {0, P_ISOUT | P_ISCREAT | (oflags)}, {-1, otype}, NULL, rd_ufunc, opc, 0}
+#define RED_VT_C(nm, opc, oflags, otype) \
+  {nm, 2, 1, {"a", "c"}, {1, 0}, {{0}, {0}}, {"n"}, {0, P_ISOUT | P_ISCREAT | (oflags)}, {-1, otype}, NULL, rd_ufunc, opc, 0}
 #define SCAN_VT(nm, opc, oflags, otype) \
   {nm, 2, 1, {"a", "b"}, {1, 1}, {{0}, {0}}, {"n"}, {0, P_ISOUT | P_ISCREAT | (oflags)}, {-1, otype}, NULL, rd_ufunc, opc, 0}
 
@@ -755,10 +757,11 @@ static vtable vtables[] = {
   SCAN_VT("dcumuprodover", R_DCUMUPRODOVER, P_ISTYPED, PDLC_D),
   RED_VT("average", R_AVERAGE, P_ISTYPED | P_ISTPLUS, PDLC_L),
   RED_VT("daverage", R_DAVERAGE, P_ISTYPED, PDLC_D),
-  RED_VT("minimum", R_MINIMUM, 0, -1),
-  RED_VT("maximum", R_MAXIMUM, 0, -1),
-  RED_VT("minimum_ind", R_MINIMUM_IND, P_ISTYPED, PDLC_IND),
-  RED_VT("maximum_ind", R_MAXIMUM_IND, P_ISTYPED, PDLC_IND),
+  /* min/max name their output c (Ufunc.pd:451,468) */
+  RED_VT_C("minimum", R_MINIMUM, 0, -1),
+  RED_VT_C("maximum", R_MAXIMUM, 0, -1),
+  RED_VT_C("minimum_ind", R_MINIMUM_IND, P_ISTYPED, PDLC_IND),
+  RED_VT_C("maximum_ind", R_MAXIMUM_IND, P_ISTYPED, PDLC_IND),
   {"minmaximum", 5, 1, {"a", "cmin", "cmax", "cmin_ind", "cmax_ind"}, {1, 0, 0, 0, 0}, {{0}}, {"n"},
    {0, P_ISOUT | P_ISCREAT, P_ISOUT | P_ISCREAT, P_ISOUT | P_ISCREAT | P_ISTYPED, P_ISOUT | P_ISCREAT | P_ISTYPED},
    {-1, -1, -1, PDLC_IND, PDLC_IND}, NULL, rd_ufunc, R_MINMAXIMUM, 0},

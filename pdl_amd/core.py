@@ -1,0 +1,670 @@
+"""Host-side mirror of PDL's ndarray API for the broadcast-loop path.
+
+An `ndarray` is PDL's `struct pdl` as the broadcast engine sees it: a root
+buffer in HBM, dims (dim 0 fastest), the composed virtual-affine increments
+and offset of a view, the bad-value state and value.  Views (xchg, mv, dummy,
+slice, transpose) are zero-copy like Slices.pd's affine transforms.  Every op
+goes through the C ABI (include/pdlhip.h) to the HIP kernels; data reaches
+the host only at PDL's physical boundaries (`numpy()`, `at()`, `sclr()`,
+printing), as Core.xs's make_physical / get_dataref do.
+
+Perl-level wrapper logic restated here (cited where defined):
+  matmult PMCode        lib/PDL/Primitive.pd:197-209
+  scalar literal typing lib/PDL/Core/pdlperl.h:185-205
+  whole-array reducers  lib/PDL/Ufunc.pd:618-663 (`$x->flat->op`)
+  operator overloads    lib/PDL/PP.pm:1420-1484 (swap for reversed operands)
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import PDLError
+
+TYPE_NAMES = ["sbyte", "byte", "short", "ushort", "long", "ulong", "indx", "ulonglong", "longlong",
+              "float", "double", "ldouble", "cfloat", "cdouble", "cldouble"]
+NP_TYPES = {L.SB: np.int8, L.B: np.uint8, L.S: np.int16, L.US: np.uint16, L.L: np.int32, L.UL: np.uint32,
+            L.IND: np.int64, L.ULL: np.uint64, L.LL: np.int64, L.F: np.float32, L.D: np.float64}
+_FROM_NP = {np.dtype(np.int8): L.SB, np.dtype(np.uint8): L.B, np.dtype(np.int16): L.S,
+            np.dtype(np.uint16): L.US, np.dtype(np.int32): L.L, np.dtype(np.uint32): L.UL,
+            np.dtype(np.int64): L.LL, np.dtype(np.uint64): L.ULL, np.dtype(np.float32): L.F,
+            np.dtype(np.float64): L.D, np.dtype(np.bool_): L.B}
+_VALFIELD = {L.SB: "A", L.B: "B", L.S: "S", L.US: "U", L.L: "L", L.UL: "K", L.IND: "N", L.ULL: "P",
+             L.LL: "Q", L.F: "F", L.D: "D"}
+
+_inited = {"device": None, "plan_only": False}
+
+
+def init(device=0):
+    """Bind the library to one GPU (one process per GPU)."""
+    if _inited["device"] is None:
+        L.check(L.lib().pdlhip_init(device))
+        _inited["device"] = device
+
+
+def _need():
+    if _inited["device"] is None and not _inited["plan_only"]:
+        init(0)
+
+
+def set_plan_only(flag):
+    """Dry-run mode of the C ABI: ops plan, type-coerce and create output
+    dims/types but allocate and launch nothing (no device needed)."""
+    _inited["plan_only"] = bool(flag)
+    L.check(L.lib().pdlhip_set_plan_only(1 if flag else 0))
+
+
+def shell(dtype, dims, state=0, incs=None, offs=0):
+    """an ndarray header without data (for plan-only use)"""
+    p = ndarray()
+    p.dtype = typecode(dtype)
+    p.dims = list(dims)
+    p.incs = list(incs) if incs is not None else _dense_incs(p.dims)
+    p.offs = offs
+    p.state = state
+    return p
+
+
+def typecode(t):
+    if isinstance(t, int):
+        return t
+    if isinstance(t, str):
+        return TYPE_NAMES.index(t)
+    return _FROM_NP[np.dtype(t)]
+
+
+class _Buf:
+    """An HBM allocation owned by this process (released to the cache on GC)."""
+    __slots__ = ("ptr", "nbytes")
+
+    def __init__(self, ptr, nbytes):
+        self.ptr = ptr
+        self.nbytes = nbytes
+
+    def __del__(self):
+        if self.ptr and L._lib is not None and _inited["device"] is not None:
+            try:
+                L._lib.pdlhip_free(self.ptr)
+            except Exception:
+                pass
+            self.ptr = None
+
+
+def _alloc(nbytes):
+    _need()
+    p = ctypes.c_void_p()
+    L.check(L.lib().pdlhip_alloc(max(int(nbytes), 1), ctypes.byref(p)))
+    return _Buf(p.value, nbytes)
+
+
+def _dense_incs(dims):
+    incs, inc = [], 1
+    for d in dims:
+        incs.append(inc)
+        inc *= d
+    return incs
+
+
+class ndarray:
+    """PDL ndarray with data in HBM."""
+    __array_priority__ = 1000
+
+    def __init__(self):
+        self._buf = None
+        self.dtype = -1
+        self.dims = []
+        self.incs = []
+        self.offs = 0
+        self.state = L.NOMYDIMS
+        self.badvalue_ = None
+        self._parent = None
+
+    __hash__ = object.__hash__
+
+    # ------------------------------------------------------------ construction
+    @classmethod
+    def from_numpy(cls, arr, dtype=None, badflag=False, badvalue=None):
+        """Upload a numpy array (C order: numpy shape = reversed PDL dims)."""
+        arr = np.asarray(arr)
+        t = typecode(dtype) if dtype is not None else _FROM_NP.get(arr.dtype, L.D)
+        host = np.ascontiguousarray(arr, dtype=NP_TYPES[t])
+        p = cls()
+        p.dtype = t
+        p.dims = list(host.shape[::-1])
+        p.incs = _dense_incs(p.dims)
+        p.state = L.BADVAL if badflag else 0
+        p.badvalue_ = badvalue
+        p._buf = _alloc(host.nbytes)
+        if host.nbytes:
+            L.check(L.lib().pdlhip_upload(p._buf.ptr, host.ctypes.data, host.nbytes))
+        return p
+
+    @classmethod
+    def null(cls):
+        return cls()
+
+    @classmethod
+    def new_dense(cls, dtype, dims):
+        p = cls()
+        p.dtype = typecode(dtype)
+        p.dims = list(dims)
+        p.incs = _dense_incs(p.dims)
+        p.state = 0
+        p._buf = _alloc(p.nelem * np.dtype(NP_TYPES[p.dtype]).itemsize)
+        return p
+
+    # ------------------------------------------------------------ info
+    @property
+    def ndims(self):
+        return len(self.dims)
+
+    @property
+    def nelem(self):
+        n = 1
+        for d in self.dims:
+            n *= d
+        return n
+
+    def isnull(self):
+        return bool(self.state & L.NOMYDIMS)
+
+    def isempty(self):
+        return self.nelem == 0
+
+    def type(self):
+        return TYPE_NAMES[self.dtype]
+
+    def dim(self, i):
+        return self.dims[i] if i < len(self.dims) else 1
+
+    def getndims(self):
+        return len(self.dims)
+
+    def badflag(self, flag=None):
+        if flag is not None:
+            if flag:
+                self.state |= L.BADVAL
+            else:
+                self.state &= ~L.BADVAL
+        return bool(self.state & L.BADVAL)
+
+    def badvalue(self, v=None):
+        if v is not None:
+            self.badvalue_ = v
+        if self.badvalue_ is not None:
+            return self.badvalue_
+        return default_badvalue(self.dtype)
+
+    # ------------------------------------------------------------ views
+    def _view(self, dims, incs, offs):
+        v = ndarray()
+        v._buf = self._buf
+        v.dtype = self.dtype
+        v.dims, v.incs, v.offs = list(dims), list(incs), offs
+        v.state = (self.state & L.BADVAL) | L.VAFFINE
+        v.badvalue_ = self.badvalue_
+        v._parent = self
+        return v
+
+    def xchg(self, i, j):
+        dims, incs = list(self.dims), list(self.incs)
+        dims[i], dims[j] = dims[j], dims[i]
+        incs[i], incs[j] = incs[j], incs[i]
+        return self._view(dims, incs, self.offs)
+
+    def mv(self, i, j):
+        dims, incs = list(self.dims), list(self.incs)
+        d, c = dims.pop(i), incs.pop(i)
+        dims.insert(j, d)
+        incs.insert(j, c)
+        return self._view(dims, incs, self.offs)
+
+    def dummy(self, pos, size=1):
+        dims, incs = list(self.dims), list(self.incs)
+        if pos < 0:
+            pos = len(dims) + 1 + pos
+        while len(dims) < pos:
+            dims.append(1)
+            incs.append(0)
+        dims.insert(pos, size)
+        incs.insert(pos, 0)
+        return self._view(dims, incs, self.offs)
+
+    def transpose(self):
+        return self.dummy(0) if self.ndims < 2 else self.xchg(0, 1)
+
+    def slice(self, *specs):
+        """per dim: None (all), int k ('(k)', drops the dim) or (start, stop, step) with an
+        inclusive stop as in PDL's 'a:b:s'"""
+        dims, incs, offs = [], [], self.offs
+        for d in range(len(self.dims)):
+            sp = specs[d] if d < len(specs) else None
+            if sp is None:
+                dims.append(self.dims[d])
+                incs.append(self.incs[d])
+            elif isinstance(sp, (int, np.integer)):
+                k = int(sp) if sp >= 0 else self.dims[d] + int(sp)
+                if not 0 <= k < self.dims[d]:
+                    raise PDLError(L.EUSERERROR, "slice: index %d out of range for dim %d" % (sp, d))
+                offs += k * self.incs[d]
+            else:
+                a, b, s = sp
+                a = a if a >= 0 else self.dims[d] + a
+                b = b if b >= 0 else self.dims[d] + b
+                n = (b - a) // s + 1
+                dims.append(max(n, 0))
+                incs.append(self.incs[d] * s)
+                offs += a * self.incs[d]
+        return self._view(dims, incs, offs)
+
+    def copy(self):
+        out = ndarray()
+        _run("assgn", [self, out])
+        return out
+
+    def make_physical(self):
+        return self
+
+    def flat(self):
+        """clump(-1): a dense 1-D copy (Slices.pd:1373-1400)"""
+        c = self.copy()
+        c.dims = [c.nelem]
+        c.incs = [1]
+        return c
+
+    def clump_all(self):
+        return self.flat()
+
+    def convert(self, t):
+        t = typecode(t)
+        if t == self.dtype:
+            return self
+        out = ndarray.new_dense(t, self.dims)
+        out.state |= self.state & L.BADVAL
+        _run("assgn_convert", [self, out])
+        return out
+
+    # ------------------------------------------------------------ host mirror
+    def _is_dense(self):
+        return self.offs == 0 and self.incs == _dense_incs(self.dims) and not (self.state & L.VAFFINE)
+
+    def numpy(self):
+        """make_physical + copy to host; numpy shape = reversed dims"""
+        if self.isnull():
+            raise PDLError(L.EUSERERROR, "numpy(): ndarray is null")
+        src = self if self._is_dense() else self.copy()
+        out = np.empty(src.dims[::-1], dtype=NP_TYPES[src.dtype])
+        if out.nbytes:
+            L.check(L.lib().pdlhip_download(out.ctypes.data, src._buf.ptr, out.nbytes))
+        return out
+
+    def at(self, *idx):
+        if len(idx) != self.ndims or any(not 0 <= i < d for i, d in zip(idx, self.dims)):
+            raise PDLError(L.EUSERERROR, "Position %s out of range" % (list(idx),))
+        off = self.offs + sum(i * c for i, c in zip(idx, self.incs))
+        out = np.empty(1, dtype=NP_TYPES[self.dtype])
+        size = out.itemsize
+        L.check(L.lib().pdlhip_download(out.ctypes.data, self._buf.ptr + off * size, size))
+        return out[0]
+
+    def sclr(self):
+        if self.nelem < 1:
+            raise PDLError(L.EUSERERROR, "sclr: empty ndarray")
+        return self.numpy().reshape(-1)[0]
+
+    def __repr__(self):
+        if self.isnull():
+            return "ndarray(null)"
+        return "ndarray(%s %s: %s)" % (self.type(), self.dims, self.numpy())
+
+    # ------------------------------------------------------------ ctypes
+    def _to_c(self):
+        c = L.Pdl()
+        c.datatype = self.dtype
+        c.state = self.state
+        c.ndims = len(self.dims)
+        if c.ndims > L.MAXDIMS:
+            raise PDLError(L.EUSERERROR, "too many dims")
+        for i, d in enumerate(self.dims):
+            c.dims[i] = d
+            c.incs[i] = self.incs[i]
+        c.offs = self.offs
+        c.data = self._buf.ptr if self._buf is not None else None
+        if self.badvalue_ is not None:
+            c.has_badvalue = 1
+            c.badvalue.type = self.dtype
+            setattr(c.badvalue.value, _VALFIELD[self.dtype], self.badvalue_)
+        return c
+
+    def _from_c(self, c, created):
+        self.state = c.state
+        if created:
+            self.dtype = c.datatype
+            self.dims = [c.dims[i] for i in range(c.ndims)]
+            self.incs = [c.incs[i] for i in range(c.ndims)]
+            self.offs = 0
+            self._buf = _Buf(c.data, self.nelem * np.dtype(NP_TYPES[self.dtype]).itemsize) if c.data else None
+        p = self._parent
+        while p is not None and (self.state & L.BADVAL):
+            p.state |= L.BADVAL
+            p = p._parent
+
+    # ------------------------------------------------------------ operators
+    def _bin(self, op, other, swap):
+        out = ndarray()
+        _run(op, [self, topdl(other), out], swap=swap)
+        return out
+
+    def _mut(self, op, other):
+        _run(op, [self, topdl(other), self], swap=0)
+        return self
+
+    def __add__(self, o): return self._bin("plus", o, 0)
+    def __radd__(self, o): return self._bin("plus", o, 1)
+    def __sub__(self, o): return self._bin("minus", o, 0)
+    def __rsub__(self, o): return self._bin("minus", o, 1)
+    def __mul__(self, o): return self._bin("mult", o, 0)
+    def __rmul__(self, o): return self._bin("mult", o, 1)
+    def __truediv__(self, o): return self._bin("divide", o, 0)
+    def __rtruediv__(self, o): return self._bin("divide", o, 1)
+    def __gt__(self, o): return self._bin("gt", o, 0)
+    def __lt__(self, o): return self._bin("lt", o, 0)
+    def __ge__(self, o): return self._bin("ge", o, 0)
+    def __le__(self, o): return self._bin("le", o, 0)
+    def __eq__(self, o): return self._bin("eq", o, 0)
+    def __ne__(self, o): return self._bin("ne", o, 0)
+    def __lshift__(self, o): return self._bin("shiftleft", o, 0)
+    def __rlshift__(self, o): return self._bin("shiftleft", o, 1)
+    def __rshift__(self, o): return self._bin("shiftright", o, 0)
+    def __rrshift__(self, o): return self._bin("shiftright", o, 1)
+    def __or__(self, o): return self._bin("or2", o, 0)
+    def __ror__(self, o): return self._bin("or2", o, 1)
+    def __and__(self, o): return self._bin("and2", o, 0)
+    def __rand__(self, o): return self._bin("and2", o, 1)
+    def __xor__(self, o): return self._bin("xor", o, 0)
+    def __rxor__(self, o): return self._bin("xor", o, 1)
+    def __pow__(self, o): return self._bin("power", o, 0)
+    def __rpow__(self, o): return self._bin("power", o, 1)
+    def __mod__(self, o): return self._bin("modulo", o, 0)
+    def __rmod__(self, o): return self._bin("modulo", o, 1)
+    def __matmul__(self, o): return matmult(self, o)
+    def __rmatmul__(self, o): return matmult(topdl(o), self)
+    def __iadd__(self, o): return self._mut("plus", o)
+    def __isub__(self, o): return self._mut("minus", o)
+    def __imul__(self, o): return self._mut("mult", o)
+    def __itruediv__(self, o): return self._mut("divide", o)
+    def __ipow__(self, o): return self._mut("power", o)
+    def __imod__(self, o): return self._mut("modulo", o)
+    def __ior__(self, o): return self._mut("or2", o)
+    def __iand__(self, o): return self._mut("and2", o)
+    def __ixor__(self, o): return self._mut("xor", o)
+    def __ilshift__(self, o): return self._mut("shiftleft", o)
+    def __irshift__(self, o): return self._mut("shiftright", o)
+
+    def spaceship(self, o, swap=0): return self._bin("spaceship", o, swap)
+    def atan2(self, o, swap=0): return self._bin("atan2", o, swap)
+
+    # ------------------------------------------------------------ Ufunc
+    def _red(self, op, out=None):
+        out = ndarray() if out is None else out
+        _run(op, [self, out])
+        return out
+
+    def sumover(self, out=None): return self._red("sumover", out)
+    def prodover(self, out=None): return self._red("prodover", out)
+    def dsumover(self, out=None): return self._red("dsumover", out)
+    def dprodover(self, out=None): return self._red("dprodover", out)
+    def cumusumover(self, out=None): return self._red("cumusumover", out)
+    def cumuprodover(self, out=None): return self._red("cumuprodover", out)
+    def dcumusumover(self, out=None): return self._red("dcumusumover", out)
+    def dcumuprodover(self, out=None): return self._red("dcumuprodover", out)
+    def average(self, out=None): return self._red("average", out)
+    avgover = average
+    def daverage(self, out=None): return self._red("daverage", out)
+    def minimum(self, out=None): return self._red("minimum", out)
+    minover = minimum
+    def maximum(self, out=None): return self._red("maximum", out)
+    maxover = maximum
+    def minimum_ind(self, out=None): return self._red("minimum_ind", out)
+    def maximum_ind(self, out=None): return self._red("maximum_ind", out)
+
+    def minimum_n_ind(self, m, out=None):
+        out = ndarray() if out is None else out
+        _run("minimum_n_ind", [self, out], m_size=m)
+        return out
+
+    def maximum_n_ind(self, m, out=None):
+        out = ndarray() if out is None else out
+        _run("maximum_n_ind", [self, out], m_size=m)
+        return out
+
+    def minmaximum(self):
+        outs = [ndarray() for _ in range(4)]
+        _run("minmaximum", [self] + outs)
+        return tuple(outs)
+
+    # whole-array reducers: $x->flat->op (Ufunc.pd:618-663)
+    def sum(self): return self.flat().sumover()
+    def prod(self): return self.flat().prodover()
+    def dsum(self): return self.flat().dsumover()
+    def avg(self): return self.flat().average()
+    def min(self): return self.flat().minimum()
+    def max(self): return self.flat().maximum()
+
+    def minmax(self):
+        mn, mx, _, _ = self.flat().minmaximum()
+        return mn, mx
+
+    def matmult(self, b, out=None):
+        return matmult(self, b, out)
+
+
+def default_badvalue(t):
+    info = {L.SB: -128, L.B: 255, L.S: -32768, L.US: 65535, L.L: -2**31, L.UL: 2**32 - 1, L.IND: -2**63,
+            L.ULL: 2**64 - 1, L.LL: -2**63, L.F: float(-np.finfo(np.float32).max),
+            L.D: float(-np.finfo(np.float64).max)}
+    return info[t]
+
+
+def _scalar_type(v):
+    """PDL's typing of a Perl scalar (pdlperl.h:185-205): integers take the
+    smallest signed type that holds them, floats are double."""
+    if isinstance(v, (bool, np.bool_)):
+        v = int(v)
+    if isinstance(v, np.generic):
+        return _FROM_NP[v.dtype]
+    if isinstance(v, int):
+        for t, lo, hi in ((L.SB, -2**7, 2**7 - 1), (L.S, -2**15, 2**15 - 1), (L.L, -2**31, 2**31 - 1),
+                          (L.IND, -2**63, 2**63 - 1)):
+            if lo <= v <= hi:
+                return t
+        if 0 <= v < 2**64:
+            return L.ULL
+        raise PDLError(L.EUSERERROR, "integer %d out of range" % v)
+    return L.D
+
+
+def topdl(x, dtype=None):
+    if isinstance(x, ndarray):
+        return x
+    if isinstance(x, (int, float, bool, np.generic)):
+        t = typecode(dtype) if dtype is not None else _scalar_type(x)
+        return ndarray.from_numpy(np.array(x, dtype=NP_TYPES[t]), t)
+    return ndarray.from_numpy(np.asarray(x), dtype)
+
+
+def pdl(data, dtype=None, badflag=False):
+    """PDL->new: nested lists / numpy in C order (numpy shape = reversed dims)"""
+    a = np.asarray(data)
+    if dtype is None and a.dtype.kind in "iub":
+        dtype = L.D   # pdl(...) of Perl numbers is double
+    return ndarray.from_numpy(a, dtype if dtype is not None else L.D, badflag=badflag)
+
+
+def null():
+    return ndarray()
+
+
+def _fill_const(p, v):
+    c = p._to_c()
+    L.check(L.lib().pdlhip_fill_uniform(ctypes.byref(c), 0, float(v), float(v)))
+    return p
+
+
+def zeroes(*dims, dtype=L.D):
+    return _fill_const(ndarray.new_dense(dtype, dims), 0)
+
+
+def ones(*dims, dtype=L.D):
+    return _fill_const(ndarray.new_dense(dtype, dims), 1)
+
+
+def empty(dtype=L.SB):
+    return ndarray.new_dense(dtype, [0])
+
+
+def sequence(*dims, dtype=L.D):
+    n = 1
+    for d in dims:
+        n *= d
+    return ndarray.from_numpy(np.arange(n).reshape(dims[::-1]), dtype)
+
+
+def random_uniform(*dims, dtype=L.D, seed=0, lo=0.0, hi=1.0):
+    """counter-based uniform[lo,hi) generated on the device"""
+    p = ndarray.new_dense(dtype, dims)
+    c = p._to_c()
+    L.check(L.lib().pdlhip_fill_uniform(ctypes.byref(c), seed, lo, hi))
+    return p
+
+
+def random_normal(*dims, dtype=L.F, seed=0, mean=0.0, sd=1.0):
+    p = ndarray.new_dense(dtype, dims)
+    c = p._to_c()
+    L.check(L.lib().pdlhip_fill_normal(ctypes.byref(c), seed, mean, sd))
+    return p
+
+
+def _run(op, pdls, swap=0, m_size=None):
+    """One pdl_run_<op> call through the C ABI."""
+    _need()
+    if op == "assgn_convert":
+        op = "assgn"
+    uniq, cs = {}, []
+    for p in pdls:
+        if id(p) not in uniq:
+            uniq[id(p)] = p._to_c()
+        cs.append(uniq[id(p)])
+    created = {id(p): p.isnull() for p in pdls}
+    arr = (L.PP * len(pdls))(*[ctypes.pointer(c) for c in cs])
+    other = (ctypes.c_int64 * 1)(swap if m_size is None else m_size)
+    L.check(L.lib().pdlhip_run(op.encode(), arr, len(pdls), other))
+    done = set()
+    for p in pdls:
+        if id(p) in done:
+            continue
+        done.add(id(p))
+        p._from_c(uniq[id(p)], created[id(p)])
+    return pdls
+
+
+def run(op, *pdls, swap=0, m_size=None):
+    """PDL::<op>(pdls...) with explicit outputs (null() to create)"""
+    return _run(op, [topdl(p) if not isinstance(p, ndarray) else p for p in pdls], swap=swap, m_size=m_size)
+
+
+def matmult(x, y, c=None):
+    """PDL::matmult PMCode (Primitive.pd:197-209): promote <2-D operands with
+    dummy dims, 1x1 operands multiply elementwise, mismatched inner dims barf."""
+    x = topdl(x)
+    y = topdl(y)
+    c = ndarray() if c is None else c
+    while x.ndims < 2:
+        x = x.dummy(-1)
+    while y.ndims < 2:
+        y = y.dummy(-1)
+    if (x.dim(0) == 1 and x.dim(1) == 1) or (y.dim(0) == 1 and y.dim(1) == 1):
+        t = x * y
+        _run("assgn", [t, c])
+        return c
+    if y.dim(1) != x.dim(0):
+        raise PDLError(L.EUSERERROR, "Dim mismatch in matmult of [%dx%d] x [%dx%d]: %d != %d"
+                       % (x.dim(0), x.dim(1), y.dim(0), y.dim(1), x.dim(0), y.dim(1)))
+    _run("matmult", [x, y, c])
+    return c
+
+
+def set_matmult_precision(mode):
+    """'split16' (default), 'fp32', 'pdlorder' or 'fp8' for float matmult"""
+    m = {"split16": L.MM_SPLIT16, "fp32": L.MM_FP32, "pdlorder": L.MM_PDLORDER, "fp8": L.MM_FP8}[mode]
+    L.check(L.lib().pdlhip_set_matmult_precision(m))
+
+
+def synchronize():
+    _need()
+    L.check(L.lib().pdlhip_synchronize())
+
+
+class Timer:
+    """elapsed ms of work enqueued on the library's stream (HIP events)"""
+
+    def __enter__(self):
+        _need()
+        L.check(L.lib().pdlhip_timer_start())
+        return self
+
+    def __exit__(self, *a):
+        ms = ctypes.c_float()
+        L.check(L.lib().pdlhip_timer_stop(ctypes.byref(ms)))
+        self.ms = ms.value
+
+
+class Event:
+    """a HIP event recorded on the library's stream"""
+
+    def __init__(self):
+        _need()
+        p = ctypes.c_void_p()
+        L.check(L.lib().pdlhip_event_create(ctypes.byref(p)))
+        self.ev = p.value
+
+    def record(self):
+        L.check(L.lib().pdlhip_event_record(self.ev))
+        return self
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float()
+        L.check(L.lib().pdlhip_event_elapsed(self.ev, end.ev, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "ev", None) and L._lib is not None:
+            try:
+                L._lib.pdlhip_event_destroy(self.ev)
+            except Exception:
+                pass
+
+
+# functional forms (PDL::plus($a, $b, $c, $swap) ...)
+def _make_biop(name):
+    def f(a, b, c=None, swap=0):
+        c = ndarray() if c is None else c
+        _run(name, [topdl(a), topdl(b), c], swap=swap)
+        return c
+    f.__name__ = name
+    return f
+
+
+def _make_unary(name):
+    def f(a, b=None):
+        b = ndarray() if b is None else b
+        _run(name, [topdl(a), b])
+        return b
+    f.__name__ = name
+    return f
+
+
+for _n in L.BIOPS:
+    globals()[_n] = _make_biop(_n)
+for _n in L.UNARY:
+    globals()[_n] = _make_unary(_n)

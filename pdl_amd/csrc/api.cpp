@@ -1,0 +1,1093 @@
+// pdlhip C ABI: host planner + dispatch.
+//
+// The planner restates what PDL does before a readdata runs:
+//   type selection      pdl__transtype_select, lib/PDL/Core/pdlapi.c:1182-1236
+//   output types        pdl__set_output_type_badvalue, pdlapi.c:1240-1260
+//   input conversion    pdl__type_convert, pdlapi.c:1262-1311 (device kernel)
+//   null checks         pdl_trans_flow_null_checks, pdlapi.c:711-738
+//   bad-value flag      pdl_make_trans_mutual, pdlapi.c:760-770, 806-808
+//   broadcast plan      pdl_initbroadcaststruct / pdl_broadcast_dim_checks /
+//                       pdl_dim_checks / pdl_broadcast_create_parameter,
+//                       lib/PDL/Core/pdlbroadcast.c:211-523
+//   core inc sizes      pdl_redodims_default, pdlapi.c:829-869
+//   error texts         pdl_param_error, lib/PDL/Core/pdlutil.c:65-145
+// then turns PDL's broadcast plan into a canonical launch plan (unit dims
+// dropped, dims ordered by the output's strides, contiguous dims merged) and
+// launches the HIP kernels of k_*.hip.  There is no CPU compute path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pdlhip.h"
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace pdlhip {
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_msg;
+
+static pdlhip_error ok() { return pdlhip_error{PDLHIP_ENONE, nullptr}; }
+
+static pdlhip_error make_err(int kind, const char *fmt, ...) {
+  char buf[8192];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_msg = buf;
+  return pdlhip_error{kind, g_msg.c_str()};
+}
+
+static pdlhip_error hip_err(hipError_t e, const char *what) {
+  return make_err(PDLHIP_EFATAL, "pdlhip: %s: %s", what, hipGetErrorString(e));
+}
+
+#define RETURN_IF(e) do { pdlhip_error e_ = (e); if (e_.error) return e_; } while (0)
+
+static const char *type_name(int t) {
+  static const char *n[] = {"sbyte", "byte", "short", "ushort", "long", "ulong", "indx", "ulonglong",
+                            "longlong", "float", "double", "ldouble", "cfloat", "cdouble", "cldouble"};
+  return (t >= 0 && t < PDLHIP_NTYPES) ? n[t] : "INVALID";
+}
+
+// ------------------------------------------------------------------ state
+struct Global {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int mm_mode = PDLHIP_MM_SPLIT16;
+  bool plan_only = false;
+  std::mutex mu;
+};
+static Global g;
+
+static pdlhip_error ensure_init() {
+  if (g.plan_only) return ok();
+  if (g.device >= 0) return ok();
+  return make_err(PDLHIP_EFATAL, "pdlhip: not initialised (call pdlhip_init)");
+}
+
+// ------------------------------------------------------------------ allocator
+// A size-bucketed cache of HBM blocks.  Every launch of the library is
+// ordered on one stream, so a block released here can be handed to the next
+// op without a device sync.
+struct Cache {
+  std::mutex mu;
+  std::multimap<size_t, void *> freelist;
+  std::map<void *, size_t> owned;
+  size_t cached = 0;
+};
+static Cache g_cache;
+
+static size_t round_size(size_t n) {
+  if (n == 0) n = 1;
+  if (n <= ((size_t)1 << 20)) return (n + 511) & ~(size_t)511;
+  return (n + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
+}
+
+static void release_cached() {
+  for (auto &kv : g_cache.freelist) {
+    g_cache.owned.erase(kv.second);
+    (void)hipFree(kv.second);
+  }
+  g_cache.freelist.clear();
+  g_cache.cached = 0;
+}
+
+void *dev_alloc(size_t n) {
+  size_t sz = round_size(n);
+  std::lock_guard<std::mutex> lk(g_cache.mu);
+  auto it = g_cache.freelist.lower_bound(sz);
+  if (it != g_cache.freelist.end() && it->first <= sz + sz / 4) {
+    void *p = it->second;
+    g_cache.cached -= it->first;
+    g_cache.freelist.erase(it);
+    return p;
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, sz) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(g.stream);
+    release_cached();
+    p = nullptr;
+    if (hipMalloc(&p, sz) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+  }
+  g_cache.owned[p] = sz;
+  return p;
+}
+
+bool dev_free(void *p) {
+  if (!p) return true;
+  std::lock_guard<std::mutex> lk(g_cache.mu);
+  auto it = g_cache.owned.find(p);
+  if (it == g_cache.owned.end()) return false;
+  g_cache.freelist.insert({it->second, p});
+  g_cache.cached += it->second;
+  if (g_cache.cached > ((size_t)64 << 30)) {  // keep the cache bounded
+    (void)hipStreamSynchronize(g.stream);
+    release_cached();
+  }
+  return true;
+}
+
+void *scratch_alloc(size_t bytes) { return dev_alloc(bytes); }
+void scratch_free(void *p) { dev_free(p); }
+
+// ------------------------------------------------------------------ signatures
+enum Family { FAM_BIOP, FAM_BIFUNC, FAM_ASSGN, FAM_RED, FAM_NIND, FAM_MM };
+enum ParFlag { P_OUT = 1, P_CREAT = 2, P_TYPED = 4, P_TPLUS = 8 };
+
+struct Sig {
+  std::string name;
+  int npdls, nparents;
+  std::vector<std::string> par_names;
+  std::vector<int> realdims;
+  std::vector<std::vector<int>> ind;   // per par, ind id of each core dim
+  std::vector<std::string> ind_names;
+  std::vector<int> flags;
+  std::vector<int> par_types;
+  std::vector<int> gentypes;           // GenericTypes in PP order
+  int family, op;
+};
+
+// GenericTypes lists of Ops.pd:1-14 / Ufunc.pd / Primitive.pd
+static const std::vector<int> GT_ALL = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14};
+static const std::vector<int> GT_REAL = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
+static const std::vector<int> GT_INT = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+static const std::vector<int> GT_POWER = {12, 13, 14, 9, 11, 10};  // [@$C, @$F], $F = F,LD,D
+static const std::vector<int> GT_ATAN2 = {9, 11, 10};
+
+static std::vector<Sig> build_sigs() {
+  std::vector<Sig> v;
+  auto bi = [&](const char *n, int op, const std::vector<int> &gt, int fam) {
+    v.push_back(Sig{n, 3, 2, {"a", "b", "c"}, {0, 0, 0}, {{}, {}, {}}, {}, {0, 0, P_OUT | P_CREAT}, {-1, -1, -1}, gt,
+                    fam, op});
+  };
+  bi("plus", OP_PLUS, GT_ALL, FAM_BIOP);
+  bi("mult", OP_MULT, GT_ALL, FAM_BIOP);
+  bi("minus", OP_MINUS, GT_ALL, FAM_BIOP);
+  bi("divide", OP_DIVIDE, GT_ALL, FAM_BIOP);
+  bi("gt", OP_GT, GT_REAL, FAM_BIOP);
+  bi("lt", OP_LT, GT_REAL, FAM_BIOP);
+  bi("le", OP_LE, GT_REAL, FAM_BIOP);
+  bi("ge", OP_GE, GT_REAL, FAM_BIOP);
+  bi("eq", OP_EQ, GT_ALL, FAM_BIOP);
+  bi("ne", OP_NE, GT_ALL, FAM_BIOP);
+  bi("shiftleft", OP_SHL, GT_INT, FAM_BIOP);
+  bi("shiftright", OP_SHR, GT_INT, FAM_BIOP);
+  bi("or2", OP_OR, GT_INT, FAM_BIOP);
+  bi("and2", OP_AND, GT_INT, FAM_BIOP);
+  bi("xor", OP_XOR, GT_INT, FAM_BIOP);
+  bi("power", OP_POWER, GT_POWER, FAM_BIFUNC);
+  bi("atan2", OP_ATAN2, GT_ATAN2, FAM_BIFUNC);
+  bi("modulo", OP_MODULO, GT_REAL, FAM_BIFUNC);
+  bi("spaceship", OP_SPACESHIP, GT_REAL, FAM_BIFUNC);
+  v.push_back(Sig{"assgn", 2, 1, {"a", "b"}, {0, 0}, {{}, {}}, {}, {0, P_OUT | P_CREAT}, {-1, -1}, GT_ALL, FAM_ASSGN, 0});
+  auto red = [&](const char *n, int op, int oflags, int otype, const std::vector<int> &gt, bool scan) {
+    v.push_back(Sig{n, 2, 1, {"a", "b"}, {1, scan ? 1 : 0}, {{0}, scan ? std::vector<int>{0} : std::vector<int>{}},
+                    {"n"}, {0, P_OUT | P_CREAT | oflags}, {-1, otype}, gt, FAM_RED, op});
+  };
+  red("sumover", R_SUM, P_TYPED | P_TPLUS, PDLHIP_L, GT_ALL, false);
+  red("prodover", R_PROD, P_TYPED | P_TPLUS, PDLHIP_L, GT_ALL, false);
+  red("dsumover", R_DSUM, P_TYPED, PDLHIP_D, GT_REAL, false);
+  red("dprodover", R_DPROD, P_TYPED, PDLHIP_D, GT_REAL, false);
+  red("cumusumover", R_CUMUSUM, P_TYPED | P_TPLUS, PDLHIP_L, GT_ALL, true);
+  red("cumuprodover", R_CUMUPROD, P_TYPED | P_TPLUS, PDLHIP_L, GT_ALL, true);
+  red("dcumusumover", R_DCUMUSUM, P_TYPED, PDLHIP_D, GT_REAL, true);
+  red("dcumuprodover", R_DCUMUPROD, P_TYPED, PDLHIP_D, GT_REAL, true);
+  red("average", R_AVERAGE, P_TYPED | P_TPLUS, PDLHIP_L, GT_REAL, false);
+  red("daverage", R_DAVERAGE, P_TYPED, PDLHIP_D, GT_REAL, false);
+  red("minimum", R_MIN, 0, -1, GT_REAL, false);
+  red("maximum", R_MAX, 0, -1, GT_REAL, false);
+  red("minimum_ind", R_MIN_IND, P_TYPED, PDLHIP_IND, GT_REAL, false);
+  red("maximum_ind", R_MAX_IND, P_TYPED, PDLHIP_IND, GT_REAL, false);
+  v.back().par_names[1] = "c";
+  for (auto &s : v)
+    if (s.name == "minimum" || s.name == "maximum" || s.name == "minimum_ind") s.par_names[1] = "c";
+  v.push_back(Sig{"minmaximum", 5, 1, {"a", "cmin", "cmax", "cmin_ind", "cmax_ind"}, {1, 0, 0, 0, 0},
+                  {{0}, {}, {}, {}, {}}, {"n"},
+                  {0, P_OUT | P_CREAT, P_OUT | P_CREAT, P_OUT | P_CREAT | P_TYPED, P_OUT | P_CREAT | P_TYPED},
+                  {-1, -1, -1, PDLHIP_IND, PDLHIP_IND}, GT_REAL, FAM_RED, R_MINMAXIMUM});
+  auto nind = [&](const char *n, int op) {
+    v.push_back(Sig{n, 2, 1, {"a", "c"}, {1, 1}, {{0}, {1}}, {"n", "m"}, {0, P_OUT | P_CREAT | P_TYPED},
+                    {-1, PDLHIP_IND}, GT_REAL, FAM_NIND, op});
+  };
+  nind("minimum_n_ind", R_MIN_N_IND);
+  nind("maximum_n_ind", R_MAX_N_IND);
+  v.push_back(Sig{"matmult", 3, 2, {"a", "b", "c"}, {2, 2, 2}, {{0, 1}, {2, 0}, {2, 1}}, {"t", "h", "w"},
+                  {0, 0, P_OUT | P_CREAT}, {-1, -1, -1}, GT_ALL, FAM_MM, 0});
+  return v;
+}
+
+static const Sig *find_sig(const char *name) {
+  static const std::vector<Sig> sigs = build_sigs();
+  for (auto &s : sigs)
+    if (s.name == name) return &s;
+  return nullptr;
+}
+
+// ------------------------------------------------------------------ bad values
+static pdlhip_value default_badval(int t) {
+  pdlhip_value v;
+  std::memset(&v, 0, sizeof v);
+  switch (t) {
+  case PDLHIP_SB: v.A = INT8_MIN; break;
+  case PDLHIP_B: v.B = UINT8_MAX; break;
+  case PDLHIP_S: v.S = INT16_MIN; break;
+  case PDLHIP_US: v.U = UINT16_MAX; break;
+  case PDLHIP_L: v.L = INT32_MIN; break;
+  case PDLHIP_UL: v.K = UINT32_MAX; break;
+  case PDLHIP_IND: v.N = INT64_MIN; break;
+  case PDLHIP_ULL: v.P = UINT64_MAX; break;
+  case PDLHIP_LL: v.Q = INT64_MIN; break;
+  case PDLHIP_F: v.F = -3.40282346638528859812e+38F; break;
+  case PDLHIP_D: v.D = -1.79769313486231570815e+308; break;
+  default: break;
+  }
+  return v;
+}
+
+// per-ndarray bad value if it has one of its own type, else the type default
+// (PDL_DECLARE_PARAMETER_BADVAL, lib/PDL/Core/pdlcore.h:202-205)
+static BadInfo badinfo(const pdlhip_pdl *p, int check) {
+  BadInfo b;
+  std::memset(&b, 0, sizeof b);
+  b.check = check;
+  b.bv = (p->has_badvalue && p->badvalue.type == p->datatype) ? p->badvalue.value : default_badval(p->datatype);
+  b.isnan = (p->datatype == PDLHIP_F) ? std::isnan(b.bv.F) : (p->datatype == PDLHIP_D) ? std::isnan(b.bv.D) : 0;
+  return b;
+}
+
+// ------------------------------------------------------------------ type selection
+static int adjust_from_trans(int ttype, int flags, int par_type) {
+  if (flags & P_TYPED) return !(flags & P_TPLUS) ? par_type : std::max(par_type, ttype);
+  return ttype;
+}
+
+static int select_transtype(const Sig &s, pdlhip_pdl **pdls) {
+  bool avail[PDLHIP_NTYPES] = {false};
+  int last = -1;
+  for (int t : s.gentypes) avail[last = t] = true;
+  if (s.gentypes.front() == last) return last;
+  bool use_last = false;
+  int ret = -1;
+  for (int i = s.npdls - 1; i >= 0; i--) {
+    const pdlhip_pdl *p = pdls[i];
+    if (!(p->state & PDLHIP_NOMYDIMS) && !(s.flags[i] & P_TYPED)) {
+      int t = p->datatype;
+      bool is_avail = (t >= 0 && t < PDLHIP_NTYPES) && avail[t];
+      if (!is_avail && t > last) use_last = true;
+      if (t >= 0 && is_avail && ret < t) ret = t;
+    }
+    if (i == s.nparents && ret >= 0) return ret;
+  }
+  if (use_last || ret < 0 || !avail[ret]) ret = last;
+  return ret;
+}
+
+// ------------------------------------------------------------------ broadcast plan
+struct Brc {
+  int npdls = 0, ndims = 0, nimpl = 0;
+  int64_t dims[PDLHIP_MAXDIMS + 2];
+  int64_t incs[(PDLHIP_MAXDIMS + 2) * MAXP];
+  int64_t offs[MAXP];
+  int64_t ind_sizes[4];
+  int64_t inc_sizes[MAXP][2];
+};
+
+static pdlhip_error param_error(const Sig &s, int pidx, pdlhip_pdl **pdls, int nimpl, const bool *creating,
+                                const char *fmt, ...) {
+  std::string m;
+  char tmp[1024];
+  if (pidx < 0 || pidx >= s.npdls) {
+    m += "ERROR: UNKNOWN PARAMETER";
+  } else {
+    m += s.name + "(";
+    for (int i = 0; i < s.npdls; i++) { m += s.par_names[i]; if (i < s.npdls - 1) m += ","; }
+    m += "): parameter '" + s.par_names[pidx] + "': ";
+  }
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(tmp, sizeof tmp, fmt, ap);
+  va_end(ap);
+  m += tmp;
+  snprintf(tmp, sizeof tmp, "\nThere are %d PDLs in the expression; %d broadcast dim%s.\n", s.npdls, nimpl,
+           nimpl == 1 ? "" : "s");
+  m += tmp;
+  int maxrd = 0;
+  for (int i = 0; i < s.npdls; i++) maxrd = std::max(maxrd, s.realdims[i]);
+  m += "   PDL IN EXPR.    ";
+  if (maxrd > 0) {
+    char f[32];
+    snprintf(f, sizeof f, "%%%ds", 8 * maxrd + 3);
+    snprintf(tmp, sizeof tmp, f, "ACTIVE DIMS | ");
+    m += tmp;
+  }
+  m += "BROADCAST DIMS\n";
+  for (int i = 0; i < s.npdls; i++) {
+    snprintf(tmp, sizeof tmp, "   #%3d (%s", i, creating[i] ? "null)\n" : "normal): ");
+    m += tmp;
+    if (creating[i]) continue;
+    if (maxrd == 1) m += "    ";
+    for (int j = 0; j < maxrd - s.realdims[i]; j++) { snprintf(tmp, sizeof tmp, "%8s", " "); m += tmp; }
+    for (int j = 0; j < s.realdims[i]; j++) { snprintf(tmp, sizeof tmp, "%8lld", (long long)pdls[i]->dims[j]); m += tmp; }
+    if (maxrd) m += " | ";
+    for (int j = 0; j < nimpl && j + s.realdims[i] < pdls[i]->ndims; j++) {
+      snprintf(tmp, sizeof tmp, "%8lld", (long long)pdls[i]->dims[j + s.realdims[i]]);
+      m += tmp;
+    }
+    m += "\n";
+  }
+  return make_err(PDLHIP_EUSERERROR, "%s", m.c_str());
+}
+
+// pdl_dim_checks (pdlbroadcast.c:211-273)
+static pdlhip_error dim_checks(const Sig &s, pdlhip_pdl **pdls, const bool *creating, bool load_only, int nimpl,
+                               int64_t *ind_sizes) {
+  for (int i = 0; i < s.npdls; i++) {
+    const pdlhip_pdl *p = pdls[i];
+    int nd = p->ndims;
+    if (!load_only && creating[i]) continue;
+    bool isout = i >= s.nparents;
+    for (int j = 0; j < s.realdims[i]; j++) {
+      int id = s.ind[i][j];
+      int64_t sz = ind_sizes[id];
+      if (j >= nd && sz == -1) sz = ind_sizes[id] = 1;
+      if (load_only && creating[i]) continue;
+      if (sz == -1 || (j < nd && sz == 1)) { ind_sizes[id] = p->dims[j]; continue; }
+      if (j >= nd && isout && sz != 1)
+        return param_error(s, i, pdls, nimpl, creating,
+                           "index '%s' size %lld, can't broadcast over output ndarray with size > 1",
+                           s.ind_names[id].c_str(), (long long)sz);
+      if (isout && sz != 1 && (p->state & PDLHIP_VAFFINE) && j < nd && p->incs[j] == 0)
+        return param_error(s, i, pdls, nimpl, creating,
+                           "index '%s' size %lld, can't broadcast over dummy dim with size > 1",
+                           s.ind_names[id].c_str(), (long long)sz);
+      if (j < nd && sz != p->dims[j] && (isout || p->dims[j] != 1))
+        return param_error(s, i, pdls, nimpl, creating, "index '%s' size %lld, but ndarray dim has size %lld",
+                           s.ind_names[id].c_str(), (long long)sz, (long long)p->dims[j]);
+    }
+  }
+  return ok();
+}
+
+// pdl_initbroadcaststruct (implicit broadcast dims, nobl = 2) + checks +
+// output creation + core inc sizes.  The CPU pthread split is not planned:
+// the whole broadcast is one device launch.
+static pdlhip_error redodims(const Sig &s, pdlhip_pdl **pdls, const bool *creating, Brc &b) {
+  const int np = s.npdls;
+  b.npdls = np;
+  int nimpl = 0;
+  for (int j = 0; j < np; j++)
+    if (!creating[j]) nimpl = std::max(nimpl, pdls[j]->ndims - s.realdims[j]);
+  if (nimpl > PDLHIP_MAXDIMS) return make_err(PDLHIP_EFATAL, "pdlhip: too many broadcast dims");
+  b.nimpl = nimpl;
+  b.ndims = nimpl + std::max(0, 2 - nimpl);
+  for (int d = 0; d < b.ndims; d++) b.dims[d] = 1;
+  for (int k = 0; k < b.ndims * np; k++) b.incs[k] = 0;
+  // pdl_broadcast_dim_checks (pdlbroadcast.c:275-327)
+  for (int nth = 0; nth < nimpl; nth++) {
+    for (int j = 0; j < np; j++) {
+      if (creating[j]) continue;
+      const pdlhip_pdl *p = pdls[j];
+      bool isout = j >= s.nparents;
+      int rd = s.realdims[j];
+      if (nth >= p->ndims - rd) {
+        if (isout && b.dims[nth] != 1)
+          return param_error(s, nth, pdls, nimpl, creating,
+                             "implicit dim %d size %lld, can't broadcast over output ndarray with size > 1", nth,
+                             (long long)b.dims[nth]);
+        continue;
+      }
+      int64_t cur = p->dims[nth + rd];
+      if (isout && cur == 1 && cur != b.dims[nth])
+        return param_error(s, nth, pdls, nimpl, creating, "implicit dim %d size %lld, but dim has size %lld", nth,
+                           (long long)b.dims[nth], (long long)cur);
+      if (isout && cur != 1 && (p->state & PDLHIP_VAFFINE) && p->incs[nth + rd] == 0)
+        return param_error(s, nth, pdls, nimpl, creating, "implicit dim %d size %lld, but dim is dummy", nth,
+                           (long long)b.dims[nth]);
+      if (cur != 1) {
+        if (b.dims[nth] != 1) {
+          if (b.dims[nth] != cur)
+            return param_error(s, j, pdls, nimpl, creating,
+                               "Mismatched implicit broadcast dimension %d: size %lld vs. %lld", nth,
+                               (long long)b.dims[nth], (long long)cur);
+        } else {
+          b.dims[nth] = cur;
+        }
+        b.incs[nth * np + j] = p->incs[nth + rd];
+      }
+    }
+  }
+  RETURN_IF(dim_checks(s, pdls, creating, false, nimpl, b.ind_sizes));
+  for (int i = 0; i < np; i++) {
+    if (!creating[i]) continue;
+    for (int j = 0; j < s.realdims[i]; j++)
+      if (b.ind_sizes[s.ind[i][j]] == -1) b.ind_sizes[s.ind[i][j]] = 1;
+  }
+  // pdl_broadcast_create_parameter (pdlbroadcast.c:490-523)
+  for (int i = 0; i < np; i++) {
+    if (!creating[i]) continue;
+    pdlhip_pdl *p = pdls[i];
+    int rd = s.realdims[i];
+    if (rd + nimpl > PDLHIP_MAXDIMS) return make_err(PDLHIP_EFATAL, "pdlhip: too many dims");
+    p->ndims = rd + nimpl;
+    for (int j = 0; j < rd; j++) p->dims[j] = b.ind_sizes[s.ind[i][j]];
+    for (int j = 0; j < nimpl; j++) p->dims[rd + j] = b.dims[j];
+    int64_t inc = 1;
+    for (int j = 0; j < p->ndims; j++) { p->incs[j] = inc; inc *= p->dims[j]; }
+    p->offs = 0;
+    p->state &= ~PDLHIP_VAFFINE;
+    for (int j = 0; j < nimpl; j++) b.incs[j * np + i] = p->incs[j + rd];
+  }
+  for (int i = 0; i < np; i++) b.offs[i] = pdls[i]->offs;
+  // core inc_sizes (pdlapi.c:858-864)
+  for (int i = 0; i < np; i++)
+    for (int j = 0; j < s.realdims[i]; j++)
+      b.inc_sizes[i][j] = (pdls[i]->ndims <= j || pdls[i]->dims[j] <= 1) ? 0 : pdls[i]->incs[j];
+  return ok();
+}
+
+// ------------------------------------------------------------------ canonical plans
+struct Dimset {
+  int n = 0;
+  int np = 0;
+  int64_t dims[PDLHIP_MAXDIMS + 2];
+  int64_t inc[MAXP][PDLHIP_MAXDIMS + 2];
+};
+
+// drop unit dims, order by the key operand's |stride|, merge dims that are
+// contiguous for every operand.  Returns false if the result is empty.
+static bool canonicalise(Dimset &d, int key) {
+  Dimset o;
+  o.np = d.np;
+  bool empty = false;
+  for (int k = 0; k < d.n; k++) {
+    if (d.dims[k] == 0) empty = true;
+    if (d.dims[k] == 1) continue;
+    o.dims[o.n] = d.dims[k];
+    for (int p = 0; p < d.np; p++) o.inc[p][o.n] = d.inc[p][k];
+    o.n++;
+  }
+  if (empty) { d.n = 0; return false; }
+  // stable insertion sort on |inc[key]|
+  for (int i = 1; i < o.n; i++) {
+    for (int j = i; j > 0; j--) {
+      int64_t x = std::llabs(o.inc[key][j]), y = std::llabs(o.inc[key][j - 1]);
+      if (x >= y) break;
+      std::swap(o.dims[j], o.dims[j - 1]);
+      for (int p = 0; p < o.np; p++) std::swap(o.inc[p][j], o.inc[p][j - 1]);
+    }
+  }
+  Dimset m;
+  m.np = o.np;
+  for (int k = 0; k < o.n; k++) {
+    if (m.n > 0) {
+      bool merge = true;
+      for (int p = 0; p < o.np; p++)
+        if (o.inc[p][k] != m.inc[p][m.n - 1] * m.dims[m.n - 1]) { merge = false; break; }
+      if (merge) { m.dims[m.n - 1] *= o.dims[k]; continue; }
+    }
+    m.dims[m.n] = o.dims[k];
+    for (int p = 0; p < o.np; p++) m.inc[p][m.n] = o.inc[p][k];
+    m.n++;
+  }
+  if (m.n == 0) {
+    m.n = 1;
+    m.dims[0] = 1;
+    for (int p = 0; p < m.np; p++) m.inc[p][0] = 0;
+  }
+  d = m;
+  return true;
+}
+
+static Dimset brc_dimset(const Brc &b, const int *which, int np) {
+  Dimset d;
+  d.np = np;
+  d.n = b.ndims;
+  for (int k = 0; k < b.ndims; k++) {
+    d.dims[k] = b.dims[k];
+    for (int p = 0; p < np; p++) d.inc[p][k] = b.incs[k * b.npdls + which[p]];
+  }
+  return d;
+}
+
+// ------------------------------------------------------------------ dispatch
+static hipError_t binop_dispatch(int T, int op, const EwPlan &p, const void *a, const void *b, void *c, bool bad,
+                                 const BadInfo *bi, hipStream_t s) {
+  switch (T) {
+#define X(TC) case TC: return launch_binop_t##TC(op, p, a, b, c, bad, bi, s);
+    PDLHIP_FOR_EACH_TYPE(X)
+#undef X
+  default: return hipErrorInvalidValue;
+  }
+}
+static hipError_t reduce_dispatch(int T, int op, int otype, const RedPlan &p, const void *a, void *const *outs,
+                                  bool bad, const BadInfo &ba, const pdlhip_value *obv, int *flag, hipStream_t s) {
+  switch (T) {
+#define X(TC) case TC: return launch_reduce_t##TC(op, otype, p, a, outs, bad, ba, obv, flag, s);
+    PDLHIP_FOR_EACH_TYPE(X)
+#undef X
+  default: return hipErrorInvalidValue;
+  }
+}
+static hipError_t matmult_dispatch(int T, const MmPlan &p, const void *a, const void *b, void *c, bool bad,
+                                   const BadInfo *bi, int mode, hipStream_t s) {
+  switch (T) {
+#define X(TC) case TC: return launch_matmult_t##TC(p, a, b, c, bad, bi, mode, s);
+    PDLHIP_FOR_EACH_TYPE(X)
+#undef X
+  default: return hipErrorInvalidValue;
+  }
+}
+
+static bool fill_ew(EwPlan &e, Dimset &d) {
+  std::memset(&e, 0, sizeof e);
+  if (d.n > MAXK) return false;
+  e.nd = d.n;
+  e.n = 1;
+  for (int k = 0; k < d.n; k++) {
+    e.dims[k] = d.dims[k];
+    e.n *= d.dims[k];
+    for (int p = 0; p < 3; p++) e.inc[p][k] = p < d.np ? d.inc[p][k] : 0;
+  }
+  return true;
+}
+
+// the readdata of one op over a planned broadcast (the pdlhip_readdata body)
+static pdlhip_error readdata(const Sig &s, int T, bool bad, const Brc &b, pdlhip_pdl **pdls, const int64_t *other) {
+  if (g.plan_only) return ok();
+  if (!type_on_device(T)) return make_err(PDLHIP_EUSERERROR, "pdlhip: %s: type %s is not supported on the device",
+                                          s.name.c_str(), type_name(T));
+  hipStream_t st = g.stream;
+  hipError_t e = hipSuccess;
+  if (s.family == FAM_BIOP || s.family == FAM_BIFUNC) {
+    int which[3] = {0, 1, 2};
+    Dimset d = brc_dimset(b, which, 3);
+    if (!canonicalise(d, 2)) return ok();
+    EwPlan p;
+    if (!fill_ew(p, d)) return make_err(PDLHIP_EFATAL, "pdlhip: %s: too many distinct dims", s.name.c_str());
+    for (int k = 0; k < 3; k++) p.off[k] = b.offs[k];
+    BadInfo bi[3];
+    for (int k = 0; k < 3; k++)
+      bi[k] = badinfo(pdls[k], s.family == FAM_BIFUNC ? 1 : (pdls[k]->state & PDLHIP_BADVAL) != 0);
+    e = binop_dispatch(T, s.op, p, pdls[0]->data, pdls[1]->data, pdls[2]->data, bad, bi, st);
+  } else if (s.family == FAM_ASSGN) {
+    int which[3] = {0, 1, 1};
+    Dimset d = brc_dimset(b, which, 3);
+    if (!canonicalise(d, 2)) return ok();
+    EwPlan p;
+    if (!fill_ew(p, d)) return make_err(PDLHIP_EFATAL, "pdlhip: assgn: too many distinct dims");
+    p.off[0] = b.offs[0];
+    p.off[2] = b.offs[1];
+    for (int k = 0; k < p.nd; k++) p.inc[1][k] = 0;
+    BadInfo sb = badinfo(pdls[0], 1);
+    BadInfo db = badinfo(pdls[1], 1);
+    e = launch_convert(T, T, p, pdls[0]->data, pdls[1]->data, bad, sb, db.bv, st);
+  } else if (s.family == FAM_RED || s.family == FAM_NIND) {
+    int np = s.npdls;
+    int which[MAXP] = {0, 1, 2, 3, 4, 5};
+    Dimset d = brc_dimset(b, which, np);
+    RedPlan p;
+    std::memset(&p, 0, sizeof p);
+    p.npdl = np;
+    p.n = b.ind_sizes[0];
+    p.inc_n[0] = b.inc_sizes[0][0];
+    p.inc_n[1] = (s.realdims[1] == 1) ? b.inc_sizes[1][0] : 0;
+    if (s.family == FAM_NIND) { p.m = b.ind_sizes[1]; p.inc_m = b.inc_sizes[1][0]; p.inc_n[1] = 0; }
+    bool nonempty = canonicalise(d, 1);
+    if (d.n > MAXK) return make_err(PDLHIP_EFATAL, "pdlhip: %s: too many distinct dims", s.name.c_str());
+    p.nd = nonempty ? d.n : 0;
+    p.nout = nonempty ? 1 : 0;
+    for (int k = 0; k < p.nd; k++) {
+      p.dims[k] = d.dims[k];
+      p.nout *= d.dims[k];
+      for (int q = 0; q < np; q++) p.inc[q][k] = d.inc[q][k];
+    }
+    for (int q = 0; q < np; q++) p.off[q] = b.offs[q];
+    void *outs[4] = {nullptr, nullptr, nullptr, nullptr};
+    pdlhip_value obv[4];
+    std::memset(obv, 0, sizeof obv);
+    for (int q = 1; q < np && q <= 4; q++) {
+      outs[q - 1] = pdls[q]->data;
+      obv[q - 1] = badinfo(pdls[q], 1).bv;
+    }
+    BadInfo ba = badinfo(pdls[0], 1);
+    int *flag = nullptr;
+    if (s.family == FAM_NIND && bad && p.nout > 0) {
+      flag = (int *)dev_alloc(sizeof(int));
+      if (!flag) return make_err(PDLHIP_EFATAL, "pdlhip: out of device memory");
+      (void)hipMemsetAsync(flag, 0, sizeof(int), st);
+    }
+    e = reduce_dispatch(T, s.op, pdls[1]->datatype, p, pdls[0]->data, outs, bad, ba, obv, flag, st);
+    // state flags the reference sets from inside readdata
+    bool minmax = (s.op == R_MIN || s.op == R_MAX || s.op == R_MIN_IND || s.op == R_MAX_IND || s.op == R_MINMAXIMUM);
+    if (minmax && p.n == 0 && p.nout > 0)
+      for (int q = 1; q < np; q++) pdls[q]->state |= PDLHIP_BADVAL;  // Ufunc.pd:463-464
+    if (s.family == FAM_NIND && p.nout > 0) {
+      pdls[1]->state &= ~PDLHIP_BADVAL;  // $PDLSTATESETGOOD(c)
+      int h = 0;
+      if (flag) {
+        if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        dev_free(flag);
+      }
+      if (h) pdls[1]->state |= PDLHIP_BADVAL;
+    }
+  } else if (s.family == FAM_MM) {
+    int which[3] = {0, 1, 2};
+    Dimset d = brc_dimset(b, which, 3);
+    MmPlan p;
+    std::memset(&p, 0, sizeof p);
+    p.T = b.ind_sizes[0]; p.H = b.ind_sizes[1]; p.W = b.ind_sizes[2];
+    p.a_t = b.inc_sizes[0][0]; p.a_h = b.inc_sizes[0][1];
+    p.b_w = b.inc_sizes[1][0]; p.b_t = b.inc_sizes[1][1];
+    p.c_w = b.inc_sizes[2][0]; p.c_h = b.inc_sizes[2][1];
+    p.atdi = pdls[0]->ndims > 0 ? pdls[0]->incs[0] : 0;  // PDL_REPRINCS(a)[0]
+    p.btdi = pdls[1]->ndims > 1 ? pdls[1]->incs[1] : 0;  // PDL_REPRINCS(b)[1]
+    bool nonempty = canonicalise(d, 2);
+    if (d.n > MAXK) return make_err(PDLHIP_EFATAL, "pdlhip: matmult: too many distinct dims");
+    p.nd = nonempty ? d.n : 0;
+    p.nbatch = nonempty ? 1 : 0;
+    for (int k = 0; k < p.nd; k++) {
+      p.dims[k] = d.dims[k];
+      p.nbatch *= d.dims[k];
+      for (int q = 0; q < 3; q++) p.inc[q][k] = d.inc[q][k];
+    }
+    for (int q = 0; q < 3; q++) p.off[q] = b.offs[q];
+    BadInfo bi[3] = {badinfo(pdls[0], 1), badinfo(pdls[1], 1), badinfo(pdls[2], 1)};
+    e = matmult_dispatch(T, p, pdls[0]->data, pdls[1]->data, pdls[2]->data, bad, bi, g.mm_mode, st);
+  }
+  if (e != hipSuccess) return hip_err(e, s.name.c_str());
+  (void)other;
+  return ok();
+}
+
+// dense temp of type `t` shaped like `p` (the converted pdl of
+// pdl__get_convertedpdl_recprotect)
+static pdlhip_error make_converted(const pdlhip_pdl *p, int t, pdlhip_pdl *out, bool copy_in) {
+  *out = *p;
+  out->datatype = t;
+  out->state = p->state & PDLHIP_BADVAL;
+  out->has_badvalue = 0;
+  out->offs = 0;
+  int64_t n = 1;
+  for (int j = 0; j < p->ndims; j++) { out->incs[j] = n; n *= p->dims[j]; }
+  out->data = nullptr;
+  if (g.plan_only) return ok();
+  out->data = dev_alloc((size_t)std::max<int64_t>(n, 1) * type_size(t));
+  if (!out->data) return make_err(PDLHIP_EFATAL, "pdlhip: out of device memory");
+  if (copy_in && n > 0) {
+    Dimset d;
+    d.np = 3;
+    d.n = p->ndims;
+    for (int j = 0; j < p->ndims; j++) {
+      d.dims[j] = p->dims[j];
+      d.inc[0][j] = p->incs[j];
+      d.inc[1][j] = 0;
+      d.inc[2][j] = out->incs[j];
+    }
+    if (!canonicalise(d, 2)) return ok();
+    EwPlan e;
+    if (!fill_ew(e, d)) return make_err(PDLHIP_EFATAL, "pdlhip: too many distinct dims");
+    e.off[0] = p->offs;
+    e.off[2] = 0;
+    BadInfo sb = badinfo(p, 1);
+    hipError_t he = launch_convert(p->datatype, t, e, p->data, out->data, (p->state & PDLHIP_BADVAL) != 0, sb,
+                                   default_badval(t), g.stream);
+    if (he != hipSuccess) return hip_err(he, "convert");
+  }
+  return ok();
+}
+
+static pdlhip_error write_back(const pdlhip_pdl *tmp, pdlhip_pdl *p) {
+  if (g.plan_only) return ok();
+  Dimset d;
+  d.np = 3;
+  d.n = p->ndims;
+  for (int j = 0; j < p->ndims; j++) {
+    d.dims[j] = p->dims[j];
+    d.inc[0][j] = tmp->incs[j];
+    d.inc[1][j] = 0;
+    d.inc[2][j] = p->incs[j];
+  }
+  if (!canonicalise(d, 2)) return ok();
+  EwPlan e;
+  if (!fill_ew(e, d)) return make_err(PDLHIP_EFATAL, "pdlhip: too many distinct dims");
+  e.off[0] = 0;
+  e.off[2] = p->offs;
+  BadInfo sb = badinfo(tmp, 1);
+  hipError_t he = launch_convert(tmp->datatype, p->datatype, e, tmp->data, p->data, (tmp->state & PDLHIP_BADVAL) != 0,
+                                 sb, badinfo(p, 1).bv, g.stream);
+  if (he != hipSuccess) return hip_err(he, "convert back");
+  if (tmp->state & PDLHIP_BADVAL) p->state |= PDLHIP_BADVAL;
+  return ok();
+}
+
+// pdl_run_<op>: everything PDL does between the XS call and readdata
+static pdlhip_error run(const Sig &s, pdlhip_pdl **user, int npdls, const int64_t *other) {
+  if (npdls != s.npdls)
+    return make_err(PDLHIP_EUSERERROR, "pdlhip: %s takes %d ndarrays, got %d", s.name.c_str(), s.npdls, npdls);
+  RETURN_IF(ensure_init());
+  pdlhip_pdl *pdls[MAXP];
+  for (int i = 0; i < npdls; i++) {
+    if (!user[i]) return make_err(PDLHIP_EUSERERROR, "pdlhip: %s: ndarray %d is NULL", s.name.c_str(), i);
+    if (!(user[i]->state & PDLHIP_NOMYDIMS) && (user[i]->ndims < 0 || user[i]->ndims > PDLHIP_MAXDIMS))
+      return make_err(PDLHIP_EUSERERROR, "pdlhip: %s: ndarray %d has %d dims", s.name.c_str(), i, user[i]->ndims);
+    pdls[i] = user[i];
+  }
+  if ((s.family == FAM_BIOP || s.family == FAM_BIFUNC) && other && other[0]) std::swap(pdls[0], pdls[1]);
+  bool inputs_bad = false;
+  for (int i = 0; i < s.nparents; i++)
+    if (pdls[i]->state & PDLHIP_BADVAL) inputs_bad = true;
+  for (int i = 0; i < s.nparents; i++)
+    if (pdls[i]->state & PDLHIP_NOMYDIMS)
+      return make_err(PDLHIP_EUSERERROR, "Error in %s: input parameter '%s' is null", s.name.c_str(),
+                      s.par_names[i].c_str());
+  for (int i = s.nparents; i < npdls; i++)
+    if ((pdls[i]->state & PDLHIP_NOMYDIMS) && !(s.flags[i] & P_CREAT))
+      return make_err(PDLHIP_EUSERERROR, "Error in %s: io parameter '%s' is null", s.name.c_str(),
+                      s.par_names[i].c_str());
+  const int T = select_transtype(s, pdls);
+  if (!type_on_device(T))
+    return make_err(PDLHIP_EUSERERROR, "pdlhip: %s: type %s is not supported on the device", s.name.c_str(),
+                    type_name(T));
+  bool creating[MAXP] = {false};
+  int req[MAXP];
+  for (int i = 0; i < npdls; i++) {
+    req[i] = adjust_from_trans(T, s.flags[i], s.par_types[i]);
+    if (pdls[i]->state & PDLHIP_NOMYDIMS) {
+      creating[i] = true;
+      pdls[i]->datatype = req[i];
+      pdls[i]->has_badvalue = 0;
+    }
+  }
+  // converted copies (pdl__type_convert); the same ndarray used as input and
+  // output (inplace) converts once
+  pdlhip_pdl conv[MAXP];
+  bool converted[MAXP] = {false};
+  pdlhip_pdl *orig[MAXP];
+  for (int i = 0; i < npdls; i++) orig[i] = pdls[i];
+  pdlhip_error err = ok();
+  for (int i = npdls - 1; i >= 0 && !err.error; i--) {
+    if (creating[i] || pdls[i]->datatype == req[i]) continue;
+    if (!type_on_device(pdls[i]->datatype)) {
+      err = make_err(PDLHIP_EUSERERROR, "pdlhip: %s: type %s is not supported on the device", s.name.c_str(),
+                     type_name(pdls[i]->datatype));
+      break;
+    }
+    int dup = -1;
+    for (int j = i + 1; j < npdls; j++)
+      if (orig[j] == orig[i] && converted[j]) dup = j;
+    if (dup >= 0) { pdls[i] = &conv[dup]; continue; }
+    err = make_converted(orig[i], req[i], &conv[i], true);
+    if (!err.error) { pdls[i] = &conv[i]; converted[i] = true; }
+  }
+  Brc b;
+  for (int k = 0; k < 4; k++) b.ind_sizes[k] = -1;
+  // dims setters run first with a load-only check (REDODIMS, pdlapi.c:41-75)
+  if (!err.error && s.family == FAM_NIND) {
+    err = dim_checks(s, pdls, creating, true, 0, b.ind_sizes);
+    if (!err.error) {
+      b.ind_sizes[1] = other ? other[0] : 0;
+      if (b.ind_sizes[1] > b.ind_sizes[0])
+        err = make_err(PDLHIP_EUSERERROR, "Error in %s:m_size > n_size", s.name.c_str());
+    }
+  }
+  if (!err.error) err = redodims(s, pdls, creating, b);
+  if (!err.error) {
+    if (inputs_bad)
+      for (int i = s.nparents; i < npdls; i++) pdls[i]->state |= PDLHIP_BADVAL;
+    for (int i = 0; i < npdls && !err.error; i++) {
+      if (!creating[i]) continue;
+      int64_t n = 1;
+      for (int j = 0; j < pdls[i]->ndims; j++) n *= pdls[i]->dims[j];
+      pdls[i]->data = nullptr;
+      if (!g.plan_only) {
+        pdls[i]->data = dev_alloc((size_t)std::max<int64_t>(n, 1) * type_size(pdls[i]->datatype));
+        if (!pdls[i]->data) err = make_err(PDLHIP_EFATAL, "pdlhip: out of device memory");
+      }
+      pdls[i]->state &= ~PDLHIP_NOMYDIMS;
+    }
+  }
+  if (!err.error) err = readdata(s, T, inputs_bad, b, pdls, other);
+  for (int i = s.nparents; i < npdls && !err.error; i++) {
+    if (!converted[i]) continue;
+    err = write_back(&conv[i], orig[i]);
+    orig[i]->state |= conv[i].state & PDLHIP_BADVAL;
+  }
+  for (int i = 0; i < npdls; i++)
+    if (converted[i]) dev_free(conv[i].data);
+  return err;
+}
+
+}  // namespace pdlhip
+
+// ==================================================================== C ABI
+using namespace pdlhip;
+
+extern "C" {
+
+int pdlhip_abi_version(void) { return PDLHIP_ABI_VERSION; }
+
+pdlhip_error pdlhip_init(int device) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  if (g.device == device) return ok();
+  if (g.device >= 0) return make_err(PDLHIP_EUSERERROR, "pdlhip: already initialised on device %d", g.device);
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_err(e, "hipSetDevice");
+  e = hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_err(e, "hipStreamCreate");
+  if ((e = hipEventCreate(&g.ev0)) != hipSuccess) return hip_err(e, "hipEventCreate");
+  if ((e = hipEventCreate(&g.ev1)) != hipSuccess) return hip_err(e, "hipEventCreate");
+  g.device = device;
+  return ok();
+}
+
+pdlhip_error pdlhip_shutdown(void) {
+  std::lock_guard<std::mutex> lk(g.mu);
+  if (g.device < 0) return ok();
+  (void)hipStreamSynchronize(g.stream);
+  {
+    std::lock_guard<std::mutex> lc(g_cache.mu);
+    release_cached();
+  }
+  (void)hipEventDestroy(g.ev0);
+  (void)hipEventDestroy(g.ev1);
+  (void)hipStreamDestroy(g.stream);
+  g.device = -1;
+  g.stream = nullptr;
+  return ok();
+}
+
+pdlhip_error pdlhip_set_plan_only(int enable) {
+  g.plan_only = enable != 0;
+  return ok();
+}
+
+pdlhip_error pdlhip_alloc(size_t nbytes, void **data) {
+  RETURN_IF(ensure_init());
+  if (g.plan_only) { *data = nullptr; return ok(); }
+  *data = dev_alloc(nbytes);
+  if (!*data) return make_err(PDLHIP_EFATAL, "pdlhip: out of device memory (%zu bytes)", nbytes);
+  return ok();
+}
+
+pdlhip_error pdlhip_free(void *data) {
+  if (!data) return ok();
+  if (!dev_free(data)) return make_err(PDLHIP_EUSERERROR, "pdlhip_free: %p was not allocated by pdlhip_alloc", data);
+  return ok();
+}
+
+pdlhip_error pdlhip_upload(void *dst, const void *src, size_t nbytes) {
+  RETURN_IF(ensure_init());
+  if (!nbytes) return ok();
+  hipError_t e = hipMemcpyAsync(dst, src, nbytes, hipMemcpyHostToDevice, g.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+  return e == hipSuccess ? ok() : hip_err(e, "upload");
+}
+
+pdlhip_error pdlhip_download(void *dst, const void *src, size_t nbytes) {
+  RETURN_IF(ensure_init());
+  if (!nbytes) return ok();
+  hipError_t e = hipMemcpyAsync(dst, src, nbytes, hipMemcpyDeviceToHost, g.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+  return e == hipSuccess ? ok() : hip_err(e, "download");
+}
+
+pdlhip_error pdlhip_synchronize(void) {
+  RETURN_IF(ensure_init());
+  hipError_t e = hipStreamSynchronize(g.stream);
+  return e == hipSuccess ? ok() : hip_err(e, "synchronize");
+}
+
+void *pdlhip_stream(void) { return (void *)g.stream; }
+
+pdlhip_error pdlhip_run(const char *op, pdlhip_pdl **pdls, int npdls, const int64_t *otherpars) {
+  const Sig *s = op ? find_sig(op) : nullptr;
+  if (!s) return make_err(PDLHIP_EUSERERROR, "pdlhip: unknown op '%s'", op ? op : "(null)");
+  return run(*s, pdls, npdls, otherpars);
+}
+
+#define BIOP_ENTRY(NAME) \
+  pdlhip_error pdlhip_run_##NAME(pdlhip_pdl *a, pdlhip_pdl *b, pdlhip_pdl *c, int swap) { \
+    pdlhip_pdl *v[3] = {a, b, c}; \
+    int64_t o[1] = {swap}; \
+    return pdlhip_run(#NAME, v, 3, o); \
+  }
+BIOP_ENTRY(plus)
+BIOP_ENTRY(mult)
+BIOP_ENTRY(minus)
+BIOP_ENTRY(divide)
+BIOP_ENTRY(gt)
+BIOP_ENTRY(lt)
+BIOP_ENTRY(le)
+BIOP_ENTRY(ge)
+BIOP_ENTRY(eq)
+BIOP_ENTRY(ne)
+BIOP_ENTRY(shiftleft)
+BIOP_ENTRY(shiftright)
+BIOP_ENTRY(or2)
+BIOP_ENTRY(and2)
+BIOP_ENTRY(xor)
+BIOP_ENTRY(power)
+BIOP_ENTRY(atan2)
+BIOP_ENTRY(modulo)
+BIOP_ENTRY(spaceship)
+
+#define UNARY_ENTRY(NAME) \
+  pdlhip_error pdlhip_run_##NAME(pdlhip_pdl *a, pdlhip_pdl *b) { \
+    pdlhip_pdl *v[2] = {a, b}; \
+    return pdlhip_run(#NAME, v, 2, nullptr); \
+  }
+UNARY_ENTRY(assgn)
+UNARY_ENTRY(sumover)
+UNARY_ENTRY(prodover)
+UNARY_ENTRY(dsumover)
+UNARY_ENTRY(dprodover)
+UNARY_ENTRY(cumusumover)
+UNARY_ENTRY(cumuprodover)
+UNARY_ENTRY(dcumusumover)
+UNARY_ENTRY(dcumuprodover)
+UNARY_ENTRY(average)
+UNARY_ENTRY(daverage)
+UNARY_ENTRY(minimum)
+UNARY_ENTRY(maximum)
+UNARY_ENTRY(minimum_ind)
+UNARY_ENTRY(maximum_ind)
+
+pdlhip_error pdlhip_run_minmaximum(pdlhip_pdl *a, pdlhip_pdl *cmin, pdlhip_pdl *cmax, pdlhip_pdl *cmin_ind,
+                                   pdlhip_pdl *cmax_ind) {
+  pdlhip_pdl *v[5] = {a, cmin, cmax, cmin_ind, cmax_ind};
+  return pdlhip_run("minmaximum", v, 5, nullptr);
+}
+
+pdlhip_error pdlhip_run_minimum_n_ind(pdlhip_pdl *a, pdlhip_pdl *c, int64_t m_size) {
+  pdlhip_pdl *v[2] = {a, c};
+  int64_t o[1] = {m_size};
+  return pdlhip_run("minimum_n_ind", v, 2, o);
+}
+
+pdlhip_error pdlhip_run_maximum_n_ind(pdlhip_pdl *a, pdlhip_pdl *c, int64_t m_size) {
+  pdlhip_pdl *v[2] = {a, c};
+  int64_t o[1] = {m_size};
+  return pdlhip_run("maximum_n_ind", v, 2, o);
+}
+
+pdlhip_error pdlhip_run_matmult(pdlhip_pdl *a, pdlhip_pdl *b, pdlhip_pdl *c) {
+  pdlhip_pdl *v[3] = {a, b, c};
+  return pdlhip_run("matmult", v, 3, nullptr);
+}
+
+pdlhip_error pdlhip_set_matmult_precision(int mode) {
+  if (mode < PDLHIP_MM_SPLIT16 || mode > PDLHIP_MM_FP8)
+    return make_err(PDLHIP_EUSERERROR, "pdlhip: unknown matmult precision %d", mode);
+  g.mm_mode = mode;
+  return ok();
+}
+
+int pdlhip_get_matmult_precision(void) { return g.mm_mode; }
+
+pdlhip_error pdlhip_readdata(const char *op, int datatype, int bvalflag, const pdlhip_broadcast *brc,
+                             pdlhip_pdl **pdls, const int64_t *otherpars) {
+  const Sig *s = op ? find_sig(op) : nullptr;
+  if (!s) return make_err(PDLHIP_EUSERERROR, "pdlhip: unknown op '%s'", op ? op : "(null)");
+  RETURN_IF(ensure_init());
+  if (!brc || brc->npdls != s->npdls || brc->ndims < 0 || brc->ndims > PDLHIP_MAXDIMS + 2)
+    return make_err(PDLHIP_EUSERERROR, "pdlhip_readdata: %s: broadcast struct does not match the signature",
+                    s->name.c_str());
+  Brc b;
+  b.npdls = brc->npdls;
+  b.ndims = brc->ndims;
+  b.nimpl = brc->ndims;
+  for (int d = 0; d < b.ndims; d++) b.dims[d] = brc->dims[d];
+  for (int k = 0; k < b.ndims * b.npdls; k++) b.incs[k] = brc->incs[k];
+  for (int p = 0; p < b.npdls; p++) b.offs[p] = brc->offs[p];
+  for (int k = 0; k < 4; k++) b.ind_sizes[k] = brc->ind_sizes[k];
+  for (int p = 0; p < b.npdls; p++)
+    for (int j = 0; j < 2; j++) b.inc_sizes[p][j] = brc->inc_sizes[p * 2 + j];
+  pdlhip_pdl *v[MAXP];
+  for (int i = 0; i < s->npdls; i++) v[i] = pdls[i];
+  if ((s->family == FAM_BIOP || s->family == FAM_BIFUNC) && otherpars && otherpars[0]) {
+    // PDL's HdrCode swaps the pdls before the trans is built, so a readdata
+    // hook receives them already swapped
+  }
+  return readdata(*s, datatype, bvalflag != 0, b, v, otherpars);
+}
+
+pdlhip_error pdlhip_timer_start(void) {
+  RETURN_IF(ensure_init());
+  hipError_t e = hipEventRecord(g.ev0, g.stream);
+  return e == hipSuccess ? ok() : hip_err(e, "timer");
+}
+
+pdlhip_error pdlhip_timer_stop(float *ms) {
+  RETURN_IF(ensure_init());
+  hipError_t e = hipEventRecord(g.ev1, g.stream);
+  if (e == hipSuccess) e = hipEventSynchronize(g.ev1);
+  if (e == hipSuccess) e = hipEventElapsedTime(ms, g.ev0, g.ev1);
+  return e == hipSuccess ? ok() : hip_err(e, "timer");
+}
+
+pdlhip_error pdlhip_event_create(void **ev) {
+  RETURN_IF(ensure_init());
+  hipEvent_t e;
+  hipError_t he = hipEventCreate(&e);
+  if (he != hipSuccess) return hip_err(he, "hipEventCreate");
+  *ev = (void *)e;
+  return ok();
+}
+
+pdlhip_error pdlhip_event_record(void *ev) {
+  RETURN_IF(ensure_init());
+  hipError_t he = hipEventRecord((hipEvent_t)ev, g.stream);
+  return he == hipSuccess ? ok() : hip_err(he, "hipEventRecord");
+}
+
+pdlhip_error pdlhip_event_elapsed(void *a, void *b, float *ms) {
+  RETURN_IF(ensure_init());
+  hipError_t he = hipEventSynchronize((hipEvent_t)b);
+  if (he == hipSuccess) he = hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b);
+  return he == hipSuccess ? ok() : hip_err(he, "hipEventElapsedTime");
+}
+
+pdlhip_error pdlhip_event_destroy(void *ev) {
+  hipError_t he = hipEventDestroy((hipEvent_t)ev);
+  return he == hipSuccess ? ok() : hip_err(he, "hipEventDestroy");
+}
+
+static pdlhip_error fill(pdlhip_pdl *p, uint64_t seed, double a, double b, int normal) {
+  RETURN_IF(ensure_init());
+  if (!p || (p->state & PDLHIP_NOMYDIMS)) return make_err(PDLHIP_EUSERERROR, "pdlhip_fill: null ndarray");
+  if (!type_on_device(p->datatype)) return make_err(PDLHIP_EUSERERROR, "pdlhip_fill: type not supported");
+  Dimset d;
+  d.np = 3;
+  d.n = p->ndims;
+  for (int j = 0; j < p->ndims; j++) {
+    d.dims[j] = p->dims[j];
+    d.inc[0][j] = d.inc[1][j] = 0;
+    d.inc[2][j] = p->incs[j];
+  }
+  if (!canonicalise(d, 2)) return ok();
+  EwPlan e;
+  if (!fill_ew(e, d)) return make_err(PDLHIP_EFATAL, "pdlhip_fill: too many distinct dims");
+  e.off[2] = p->offs;
+  hipError_t he = launch_fill(p->datatype, e, p->data, seed, a, b, normal, g.stream);
+  return he == hipSuccess ? ok() : hip_err(he, "fill");
+}
+
+pdlhip_error pdlhip_fill_uniform(pdlhip_pdl *p, uint64_t seed, double lo, double hi) { return fill(p, seed, lo, hi, 0); }
+pdlhip_error pdlhip_fill_normal(pdlhip_pdl *p, uint64_t seed, double mean, double sd) {
+  return fill(p, seed, mean, sd, 1);
+}
+
+}  // extern "C"

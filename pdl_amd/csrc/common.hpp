@@ -1,0 +1,96 @@
+// Shared host/device definitions of the pdlhip library: op codes, the
+// canonical launch plans the host planner hands to kernels, and type traits.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/pdlhip.h"
+
+namespace pdlhip {
+
+constexpr int MAXK = 8;      // loop dims of a canonical launch plan
+constexpr int MAXP = 6;      // pdls per trans on this path
+
+// Ops.pd biops (lib/PDL/Ops.pd:288-324) and assgn (382-397)
+enum BinOp : int {
+  OP_PLUS, OP_MULT, OP_MINUS, OP_DIVIDE, OP_GT, OP_LT, OP_LE, OP_GE, OP_EQ, OP_NE,
+  OP_SHL, OP_SHR, OP_OR, OP_AND, OP_XOR, OP_POWER, OP_ATAN2, OP_MODULO, OP_SPACESHIP,
+  OP_NBINOPS
+};
+
+// Ufunc.pd reductions / scans (lib/PDL/Ufunc.pd:88-140, 413-613)
+enum RedOp : int {
+  R_SUM, R_PROD, R_DSUM, R_DPROD, R_CUMUSUM, R_CUMUPROD, R_DCUMUSUM, R_DCUMUPROD,
+  R_AVERAGE, R_DAVERAGE, R_MIN, R_MAX, R_MIN_IND, R_MAX_IND, R_MINMAXIMUM,
+  R_MIN_N_IND, R_MAX_N_IND, R_NREDOPS
+};
+
+// Bad-value description of one operand as a kernel sees it
+// ($ISBAD / $SETBAD of PDLCode.pm:652-693; PDL_ISBAD2 pdl.h.PL:259-262)
+struct BadInfo {
+  int32_t check;       // this operand's values are tested against its bad value
+  int32_t isnan;       // the bad value is NaN: test with isnan
+  pdlhip_value bv;     // the bad value in the operand's own type
+};
+
+// Elementwise launch plan: `nd` loop dims ordered by the output's strides,
+// unit dims dropped, contiguous neighbours merged.
+struct EwPlan {
+  int32_t nd;
+  int32_t pad_;
+  int64_t n;                  // total elements
+  int64_t dims[MAXK];
+  int64_t inc[3][MAXK];       // operand 0,1 inputs; 2 output
+  int64_t off[3];
+};
+
+// Reduction / scan launch plan: core dim n (reduced or scanned) plus the
+// broadcast dims of the output(s).
+struct RedPlan {
+  int64_t n;                  // core dim size
+  int64_t inc_n[2];           // a's and (scans) b's stride along n
+  int32_t nd;                 // broadcast dims after canonicalisation
+  int32_t npdl;
+  int64_t nout;               // product of broadcast dims
+  int64_t dims[MAXK];
+  int64_t inc[MAXP][MAXK];    // per pdl broadcast strides
+  int64_t off[MAXP];
+  int64_t m;                  // *_n_ind: m_size; scans: b's n extent
+  int64_t inc_m;              // *_n_ind: c's stride along m
+};
+
+// Matmult launch plan (Primitive.pd:211-245): a(t,h) b(w,t) c(w,h) with
+// broadcast dims as a batch.
+struct MmPlan {
+  int64_t T, H, W;
+  int64_t a_t, a_h, b_w, b_t, c_w, c_h;  // core strides (0 where size <= 1)
+  int64_t atdi, btdi;                     // PDL_REPRINCS(a)[0], (b)[1]
+  int32_t nd;
+  int32_t pad_;
+  int64_t nbatch;
+  int64_t dims[MAXK];
+  int64_t inc[3][MAXK];
+  int64_t off[3];
+};
+
+// type-code -> C type
+template <int TC> struct CType;
+template <> struct CType<PDLHIP_SB> { typedef int8_t T; };
+template <> struct CType<PDLHIP_B> { typedef uint8_t T; };
+template <> struct CType<PDLHIP_S> { typedef int16_t T; };
+template <> struct CType<PDLHIP_US> { typedef uint16_t T; };
+template <> struct CType<PDLHIP_L> { typedef int32_t T; };
+template <> struct CType<PDLHIP_UL> { typedef uint32_t T; };
+template <> struct CType<PDLHIP_IND> { typedef int64_t T; };
+template <> struct CType<PDLHIP_ULL> { typedef uint64_t T; };
+template <> struct CType<PDLHIP_LL> { typedef int64_t T; };
+template <> struct CType<PDLHIP_F> { typedef float T; };
+template <> struct CType<PDLHIP_D> { typedef double T; };
+
+inline int type_size(int t) {
+  static const int s[] = {1, 1, 2, 2, 4, 4, 8, 8, 8, 4, 8, 16, 8, 16, 32};
+  return (t >= 0 && t < PDLHIP_NTYPES) ? s[t] : 0;
+}
+inline bool type_on_device(int t) { return t >= PDLHIP_SB && t <= PDLHIP_D; }
+
+}  // namespace pdlhip

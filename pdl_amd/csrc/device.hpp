@@ -1,0 +1,141 @@
+// Device helpers: bad-value tests, the Ops.pd operators with C's integer
+// promotion and wrap-around (PDL is built with -fwrapv), and offset math.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace pdlhip {
+
+template <class T> struct Unsigned { typedef T U; };
+template <> struct Unsigned<int8_t> { typedef uint8_t U; };
+template <> struct Unsigned<int16_t> { typedef uint16_t U; };
+template <> struct Unsigned<int32_t> { typedef uint32_t U; };
+template <> struct Unsigned<int64_t> { typedef uint64_t U; };
+
+template <class T> struct IsFloat { static constexpr bool v = false; };
+template <> struct IsFloat<float> { static constexpr bool v = true; };
+template <> struct IsFloat<double> { static constexpr bool v = true; };
+
+// C integer promotion: types narrower than int compute as int
+template <class T> struct Promo { typedef T P; };
+template <> struct Promo<int8_t> { typedef int32_t P; };
+template <> struct Promo<uint8_t> { typedef int32_t P; };
+template <> struct Promo<int16_t> { typedef int32_t P; };
+template <> struct Promo<uint16_t> { typedef int32_t P; };
+
+template <class T> __device__ __host__ inline T bv_as(const pdlhip_value &v) {
+  T r;
+  __builtin_memcpy(&r, &v, sizeof(T));
+  return r;
+}
+
+template <class T> __device__ inline bool is_nan(T v) {
+  if constexpr (IsFloat<T>::v) return v != v;
+  else return false;
+}
+
+// PDL_ISBAD2 (pdl.h.PL:259-262)
+template <class T> __device__ inline bool isbad(T v, T bv, int bvnan) {
+  return bvnan ? is_nan(v) : (v == bv);
+}
+
+// wrapping ring arithmetic in T's width: what PDL's C gives under -fwrapv
+template <class T> __device__ inline T wadd(T a, T b) {
+  if constexpr (IsFloat<T>::v) return a + b;
+  else { typedef typename Unsigned<T>::U U; return (T)(U)((uint64_t)(U)a + (uint64_t)(U)b); }
+}
+template <class T> __device__ inline T wsub(T a, T b) {
+  if constexpr (IsFloat<T>::v) return a - b;
+  else { typedef typename Unsigned<T>::U U; return (T)(U)((uint64_t)(U)a - (uint64_t)(U)b); }
+}
+template <class T> __device__ inline T wmul(T a, T b) {
+  if constexpr (IsFloat<T>::v) return a * b;
+  else if constexpr (sizeof(T) <= 4) {
+    typedef typename Unsigned<T>::U U;
+    return (T)(U)((uint32_t)(U)a * (uint32_t)(U)b);
+  } else {
+    return (T)((uint64_t)a * (uint64_t)b);
+  }
+}
+
+// Ops.pd:68-70 MOD / BU_MOD / SPACE, with the same C conversions
+#define PDLHIP_ABS(A) ((A) >= 0 ? (A) : -(A))
+template <class T> __device__ inline T op_mod(T X, T N) {
+  if constexpr (!IsFloat<T>::v && (T)-1 > (T)0) {   // unsigned: BU_MOD
+    return (T)(((N) == 0) ? 0 : ((X) - (N) * ((uint64_t)((X) / (N)))));
+  } else {
+    return (T)(((N) == 0) ? 0
+             : ((X) - (PDLHIP_ABS(N)) * ((long long)((X) / (PDLHIP_ABS(N))) +
+                 ((((N) * ((long long)((X) / (N)))) != (X)) ? ((((N) < 0) ? 1 : 0) + ((((X) < 0) ? -1 : 0))) : 0))));
+  }
+}
+
+template <class T> __device__ inline T shift_count_mask(T b) {
+  typedef typename Promo<T>::P P;
+  return (T)((uint64_t)b & (sizeof(P) * 8 - 1));
+}
+
+template <int OP, class T> __device__ inline T binop(T a, T b) {
+  typedef typename Promo<T>::P P;
+  if constexpr (OP == OP_PLUS) return wadd(a, b);
+  else if constexpr (OP == OP_MULT) return wmul(a, b);
+  else if constexpr (OP == OP_MINUS) return wsub(a, b);
+  else if constexpr (OP == OP_DIVIDE) {
+    if constexpr (IsFloat<T>::v) return a / b;
+    else return (T)((P)a / (P)b);
+  }
+  else if constexpr (OP == OP_GT) return (T)(a > b);
+  else if constexpr (OP == OP_LT) return (T)(a < b);
+  else if constexpr (OP == OP_LE) return (T)(a <= b);
+  else if constexpr (OP == OP_GE) return (T)(a >= b);
+  else if constexpr (OP == OP_EQ) return (T)(a == b);
+  else if constexpr (OP == OP_NE) return (T)(a != b);
+  else if constexpr (OP == OP_SHL) {
+    if constexpr (IsFloat<T>::v) return a;
+    else { typedef typename Unsigned<P>::U UP; return (T)(P)((UP)(P)a << (int)shift_count_mask(b)); }
+  }
+  else if constexpr (OP == OP_SHR) {
+    if constexpr (IsFloat<T>::v) return a;
+    else return (T)((P)a >> (int)shift_count_mask(b));
+  }
+  else if constexpr (OP == OP_OR) { if constexpr (IsFloat<T>::v) return a; else return (T)(a | b); }
+  else if constexpr (OP == OP_AND) { if constexpr (IsFloat<T>::v) return a; else return (T)(a & b); }
+  else if constexpr (OP == OP_XOR) { if constexpr (IsFloat<T>::v) return a; else return (T)(a ^ b); }
+  else if constexpr (OP == OP_POWER) {
+    if constexpr (sizeof(T) == 4 && IsFloat<T>::v) return powf(a, b);
+    else if constexpr (IsFloat<T>::v) return pow(a, b);
+    else return a;
+  }
+  else if constexpr (OP == OP_ATAN2) {
+    if constexpr (sizeof(T) == 4 && IsFloat<T>::v) return atan2f(a, b);
+    else if constexpr (IsFloat<T>::v) return atan2(a, b);
+    else return a;
+  }
+  else if constexpr (OP == OP_MODULO) return op_mod<T>(a, b);
+  else if constexpr (OP == OP_SPACESHIP) return (T)(((a) < (b)) ? -1 : ((a) != (b)));
+  else return a;
+}
+
+// VEC elements loaded/stored as one 16-byte (or narrower) access
+template <class T, int VEC> struct alignas(sizeof(T) * VEC) Vec { T v[VEC]; };
+
+// offsets of element (i0, row) where `row` enumerates dims 1..nd-1
+struct RowOffsets {
+  int64_t o[3];
+};
+
+__device__ inline void row_offsets(const EwPlan &p, int64_t row, int64_t *o) {
+  o[0] = p.off[0]; o[1] = p.off[1]; o[2] = p.off[2];
+  for (int d = 1; d < p.nd; d++) {
+    int64_t dd = p.dims[d];
+    int64_t i = row % dd;
+    row /= dd;
+    o[0] += i * p.inc[0][d]; o[1] += i * p.inc[1][d]; o[2] += i * p.inc[2][d];
+  }
+}
+
+}  // namespace pdlhip

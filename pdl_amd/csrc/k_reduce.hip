@@ -1,0 +1,488 @@
+// Reductions and scans of lib/PDL/Ufunc.pd over the core dim n (dim 0 of a),
+// one output element per broadcast position:
+//   sumover/prodover (+d)       Ufunc.pd:88-118
+//   cumusumover/cumuprodover    Ufunc.pd:120-140
+//   average/daverage            Ufunc.pd:413-444
+//   minimum/maximum(+_ind)      Ufunc.pd:446-482
+//   minimum_n_ind/maximum_n_ind Ufunc.pd:484-540
+//   minmaximum                  Ufunc.pd:563-613
+//
+// Kernels:
+//   wave   one 64-lane wavefront per output (rows with unit stride), 16-byte
+//          vector loads, cross-lane combine with shuffles.
+//   split  long rows and few outputs: blocks reduce row chunks into partials
+//          in scratch, a second kernel combines them.
+//   thread one thread per output walking n in index order (strided rows,
+//          scans, float products, *_n_ind): adjacent threads read adjacent
+//          addresses when the broadcast dim 0 is the contiguous one.
+//
+// Semantics kept exactly: integer sums/products wrap in the output type
+// (any order gives PDL's bits), min/max follow PDL's rules (a NaN never
+// displaces a non-NaN, a non-NaN displaces a NaN, ties keep the first,
+// all-NaN gives the last), bad values are skipped, empty/all-bad rows give
+// BAD (or 0 / NaN for average in good mode).  Float sums are compensated
+// (Neumaier per lane, double-double across lanes and blocks), so they are
+// at least as accurate as PDL's sequential float loop; float products and
+// float scans run in PDL's exact sequential order.
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "launch.hpp"
+
+namespace pdlhip {
+
+void *scratch_alloc(size_t bytes);
+void scratch_free(void *p);
+
+template <class T> struct IPlus { typedef T O; };
+template <> struct IPlus<int8_t> { typedef int32_t O; };
+template <> struct IPlus<uint8_t> { typedef int32_t O; };
+template <> struct IPlus<int16_t> { typedef int32_t O; };
+template <> struct IPlus<uint16_t> { typedef int32_t O; };
+
+constexpr bool is_sum(int op) { return op == R_SUM || op == R_DSUM || op == R_AVERAGE || op == R_DAVERAGE; }
+constexpr bool is_prod(int op) { return op == R_PROD || op == R_DPROD; }
+constexpr bool is_minmax(int op) { return op == R_MIN || op == R_MAX || op == R_MIN_IND || op == R_MAX_IND; }
+constexpr bool is_max(int op) { return op == R_MAX || op == R_MAX_IND; }
+
+// ---------------------------------------------------------------- partials
+// Everything one lane / block knows about its share of a row.
+template <class Tin, class Tacc> struct Part {
+  Tacc s, c;          // sum + compensation (floats), wrapped sum/product (ints)
+  int64_t cnt;        // good elements seen
+  Tin v0; int64_t i0; // extreme (min for minmaximum); i0 < 0: none
+  Tin v1; int64_t i1; // max for minmaximum
+};
+
+template <int OP, class Tin, class Tacc> __device__ inline void part_init(Part<Tin, Tacc> &p) {
+  p.s = is_prod(OP) ? Tacc(1) : Tacc(0);
+  p.c = Tacc(0);
+  p.cnt = 0;
+  p.v0 = Tin(0); p.i0 = -1;
+  p.v1 = Tin(0); p.i1 = -1;
+}
+
+// Neumaier step (floats) or wrapping add (ints)
+template <class Tacc> __device__ inline void acc_add(Tacc &s, Tacc &c, Tacc x) {
+  if constexpr (IsFloat<Tacc>::v) {
+    Tacc t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+  } else {
+    s = wadd(s, x);
+  }
+}
+
+// PDL's min/max preference as a total order on (value, index): non-NaN beats
+// NaN; smaller (larger for max) value wins; equal values: earlier index; two
+// NaNs: later index.
+template <bool MAX, class T> __device__ inline bool pref(T v, int64_t i, T w, int64_t j) {
+  bool vn = is_nan(v), wn = is_nan(w);
+  if (vn != wn) return !vn;
+  if (vn) return i > j;
+  if (MAX ? (v > w) : (v < w)) return true;
+  if (MAX ? (w > v) : (w < v)) return false;
+  return i < j;
+}
+
+template <int OP, class Tin, class Tacc> __device__ inline void part_push(Part<Tin, Tacc> &p, Tin x, int64_t i) {
+  if constexpr (is_sum(OP)) {
+    acc_add<Tacc>(p.s, p.c, (Tacc)x);
+    p.cnt++;
+  } else if constexpr (is_prod(OP)) {
+    p.s = wmul(p.s, (Tacc)x);
+    p.cnt++;
+  } else if constexpr (is_minmax(OP)) {
+    if (p.i0 < 0 || pref<is_max(OP)>(x, i, p.v0, p.i0)) { p.v0 = x; p.i0 = i; }
+  } else if constexpr (OP == R_MINMAXIMUM) {
+    if (is_nan(x)) return;
+    if (p.i0 < 0 || x < p.v0 || (x == p.v0 && i < p.i0)) { p.v0 = x; p.i0 = i; }
+    if (p.i1 < 0 || x > p.v1 || (x == p.v1 && i < p.i1)) { p.v1 = x; p.i1 = i; }
+  }
+}
+
+// double-double add for combining compensated sums
+__device__ inline void dd_add(double &hi, double &lo, double bh, double bl) {
+  double s = hi + bh;
+  double bb = s - hi;
+  double e = (hi - (s - bb)) + (bh - bb);
+  e += lo + bl;
+  hi = s + e;
+  lo = e - (hi - s);
+}
+
+template <int OP, class Tin, class Tacc> __device__ inline void part_combine(Part<Tin, Tacc> &p, const Part<Tin, Tacc> &q) {
+  if constexpr (is_sum(OP)) {
+    if constexpr (IsFloat<Tacc>::v) {
+      double hi = (double)p.s, lo = (double)p.c;
+      dd_add(hi, lo, (double)q.s, (double)q.c);
+      // keep the pair in Tacc: for float accumulators hold hi/lo as the
+      // float-rounded pair; the final value is formed in double
+      p.s = (Tacc)hi;
+      p.c = (Tacc)((hi - (double)p.s) + lo);
+    } else {
+      p.s = wadd(p.s, q.s);
+    }
+    p.cnt += q.cnt;
+  } else if constexpr (is_prod(OP)) {
+    p.s = wmul(p.s, q.s);
+    p.cnt += q.cnt;
+  } else if constexpr (is_minmax(OP)) {
+    if (q.i0 >= 0 && (p.i0 < 0 || pref<is_max(OP)>(q.v0, q.i0, p.v0, p.i0))) { p.v0 = q.v0; p.i0 = q.i0; }
+  } else if constexpr (OP == R_MINMAXIMUM) {
+    if (q.i0 >= 0 && (p.i0 < 0 || q.v0 < p.v0 || (q.v0 == p.v0 && q.i0 < p.i0))) { p.v0 = q.v0; p.i0 = q.i0; }
+    if (q.i1 >= 0 && (p.i1 < 0 || q.v1 > p.v1 || (q.v1 == p.v1 && q.i1 < p.i1))) { p.v1 = q.v1; p.i1 = q.i1; }
+  }
+}
+
+template <class T> __device__ inline T shfl_xor(T v, int m) {
+  if constexpr (sizeof(T) == 8) {
+    int64_t x;
+    __builtin_memcpy(&x, &v, 8);
+    int lo = __shfl_xor((int)(x & 0xffffffff), m, 64), hi = __shfl_xor((int)(x >> 32), m, 64);
+    x = ((int64_t)(uint32_t)lo) | ((int64_t)hi << 32);
+    T r;
+    __builtin_memcpy(&r, &x, 8);
+    return r;
+  } else if constexpr (sizeof(T) == 4) {
+    int x;
+    __builtin_memcpy(&x, &v, 4);
+    x = __shfl_xor(x, m, 64);
+    T r;
+    __builtin_memcpy(&r, &x, 4);
+    return r;
+  } else {
+    int x = (int)v;
+    x = __shfl_xor(x, m, 64);
+    return (T)x;
+  }
+}
+
+template <int OP, class Tin, class Tacc> __device__ inline void wave_reduce(Part<Tin, Tacc> &p) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    Part<Tin, Tacc> q;
+    q.s = shfl_xor(p.s, m);
+    q.c = shfl_xor(p.c, m);
+    q.cnt = shfl_xor(p.cnt, m);
+    q.v0 = shfl_xor(p.v0, m);
+    q.i0 = shfl_xor(p.i0, m);
+    q.v1 = shfl_xor(p.v1, m);
+    q.i1 = shfl_xor(p.i1, m);
+    part_combine<OP>(p, q);
+  }
+}
+
+// ---------------------------------------------------------------- outputs
+struct Outs {
+  void *p[4];
+  pdlhip_value bv[4];
+};
+
+template <class T> __device__ inline void put(void *base, int64_t off, T v) { ((T *)base)[off] = v; }
+
+template <class Tout> __device__ inline Tout avg_div(Tout tmp, int64_t cnt) {
+  if constexpr (IsFloat<Tout>::v) return tmp / (Tout)cnt;
+  else if constexpr (sizeof(Tout) == 8 && (Tout)-1 > (Tout)0) return tmp / (uint64_t)cnt;
+  else return (Tout)((int64_t)tmp / cnt);
+}
+
+template <int OP, class Tin, class Tout, class Tacc>
+__device__ inline void part_store(const Part<Tin, Tacc> &p, const Outs &o, const int64_t *off, bool bad) {
+  if constexpr (OP == R_SUM || OP == R_DSUM || OP == R_PROD || OP == R_DPROD) {
+    Tout v;
+    if constexpr (IsFloat<Tacc>::v && !is_prod(OP)) v = (Tout)((double)p.s + (double)p.c);
+    else v = (Tout)p.s;
+    if (bad && p.cnt == 0) v = bv_as<Tout>(o.bv[0]);
+    put<Tout>(o.p[0], off[1], v);
+  } else if constexpr (OP == R_AVERAGE || OP == R_DAVERAGE) {
+    Tout v;
+    if (p.cnt == 0) {
+      if (bad) v = bv_as<Tout>(o.bv[0]);
+      else v = IsFloat<Tin>::v ? (Tout)NAN : (Tout)0;
+    } else {
+      Tout tmp;
+      if constexpr (IsFloat<Tacc>::v) tmp = (Tout)((double)p.s + (double)p.c);
+      else tmp = (Tout)p.s;
+      v = avg_div<Tout>(tmp, p.cnt);
+    }
+    put<Tout>(o.p[0], off[1], v);
+  } else if constexpr (OP == R_MIN || OP == R_MAX) {
+    put<Tin>(o.p[0], off[1], p.i0 >= 0 ? p.v0 : bv_as<Tin>(o.bv[0]));
+  } else if constexpr (OP == R_MIN_IND || OP == R_MAX_IND) {
+    put<int64_t>(o.p[0], off[1], p.i0 >= 0 ? p.i0 : bv_as<int64_t>(o.bv[0]));
+  } else if constexpr (OP == R_MINMAXIMUM) {
+    bool f = p.i0 >= 0;
+    put<Tin>(o.p[0], off[1], f ? p.v0 : bv_as<Tin>(o.bv[0]));
+    put<Tin>(o.p[1], off[2], f ? p.v1 : bv_as<Tin>(o.bv[1]));
+    put<int64_t>(o.p[2], off[3], f ? p.i0 : bv_as<int64_t>(o.bv[2]));
+    put<int64_t>(o.p[3], off[4], f ? p.i1 : bv_as<int64_t>(o.bv[3]));
+  }
+}
+
+__device__ inline void out_offsets(const RedPlan &p, int64_t e, int64_t *off) {
+  for (int k = 0; k < p.npdl; k++) off[k] = p.off[k];
+  for (int d = 0; d < p.nd; d++) {
+    int64_t i = e % p.dims[d];
+    e /= p.dims[d];
+    for (int k = 0; k < p.npdl; k++) off[k] += i * p.inc[k][d];
+  }
+}
+
+// ---------------------------------------------------------------- wave
+constexpr int RT = 256;  // threads per block (4 waves)
+
+template <int OP, class Tin, class Tout, class Tacc, bool BAD, int VEC>
+__device__ inline void row_segment(Part<Tin, Tacc> &pt, const Tin *__restrict__ a, int64_t o, int64_t begin,
+                                   int64_t end, int64_t inc, int lane, const BadInfo &ba) {
+  const Tin abv = bv_as<Tin>(ba.bv);
+  int64_t i = begin + (int64_t)lane * VEC;
+  if constexpr (VEC > 1) {
+    // begin is a multiple of VEC and the row start is 16-byte aligned
+    for (; i + VEC <= end; i += 64 * VEC) {
+      Vec<Tin, VEC> x = *reinterpret_cast<const Vec<Tin, VEC> *>(a + o + i);
+#pragma unroll
+      for (int k = 0; k < VEC; k++) {
+        if constexpr (BAD) { if (isbad(x.v[k], abv, ba.isnan)) continue; }
+        part_push<OP, Tin, Tacc>(pt, x.v[k], i + k);
+      }
+    }
+    // tail: fewer than VEC*64 elements remain past the vector loop's reach
+    for (int64_t t = ((end - begin) / VEC) * VEC + begin + lane; t < end; t += 64) {
+      Tin x = a[o + t];
+      if constexpr (BAD) { if (isbad(x, abv, ba.isnan)) continue; }
+      part_push<OP, Tin, Tacc>(pt, x, t);
+    }
+  } else {
+    for (; i < end; i += 64) {
+      Tin x = a[o + i * inc];
+      if constexpr (BAD) { if (isbad(x, abv, ba.isnan)) continue; }
+      part_push<OP, Tin, Tacc>(pt, x, i);
+    }
+  }
+}
+
+template <int OP, class Tin, class Tout, class Tacc, bool BAD, int VEC>
+__global__ __launch_bounds__(RT) void red_wave(const Tin *__restrict__ a, RedPlan p, Outs o, BadInfo ba) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * (RT / 64);
+  for (int64_t e = (int64_t)blockIdx.x * (RT / 64) + (threadIdx.x >> 6); e < p.nout; e += nw) {
+    int64_t off[MAXP];
+    out_offsets(p, e, off);
+    Part<Tin, Tacc> pt;
+    part_init<OP, Tin, Tacc>(pt);
+    row_segment<OP, Tin, Tout, Tacc, BAD, VEC>(pt, a, off[0], 0, p.n, p.inc_n[0], lane, ba);
+    wave_reduce<OP, Tin, Tacc>(pt);
+    if (lane == 0) part_store<OP, Tin, Tout, Tacc>(pt, o, off, BAD);
+  }
+}
+
+// ---------------------------------------------------------------- split
+// stage 1: block (e, k) reduces chunk k of row e into parts[e * nchunk + k]
+template <int OP, class Tin, class Tout, class Tacc, bool BAD, int VEC>
+__global__ __launch_bounds__(RT) void red_split1(const Tin *__restrict__ a, RedPlan p, int64_t nchunk, int64_t clen,
+                                                 Part<Tin, Tacc> *parts, BadInfo ba) {
+  __shared__ Part<Tin, Tacc> sh[RT / 64];
+  const int64_t e = blockIdx.x / nchunk, k = blockIdx.x % nchunk;
+  int64_t off[MAXP];
+  out_offsets(p, e, off);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t cb = k * clen, ce = (cb + clen < p.n) ? cb + clen : p.n;
+  Part<Tin, Tacc> pt;
+  part_init<OP, Tin, Tacc>(pt);
+  // the 4 waves of the block take interleaved sub-chunks of the chunk
+  const int64_t sub = ((ce - cb) + 3) / 4;
+  const int64_t subr = ((sub + VEC - 1) / VEC) * VEC;
+  const int64_t sb = cb + w * subr, se = (sb + subr < ce) ? sb + subr : ce;
+  if (sb < se) row_segment<OP, Tin, Tout, Tacc, BAD, VEC>(pt, a, off[0], sb, se, p.inc_n[0], lane, ba);
+  wave_reduce<OP, Tin, Tacc>(pt);
+  if (lane == 0) sh[w] = pt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 1; j < RT / 64; j++) part_combine<OP>(pt, sh[j]);
+    parts[blockIdx.x] = pt;
+  }
+}
+
+// stage 2: one wave per row combines its chunk partials
+template <int OP, class Tin, class Tout, class Tacc, bool BAD>
+__global__ __launch_bounds__(RT) void red_split2(RedPlan p, int64_t nchunk, const Part<Tin, Tacc> *parts, Outs o) {
+  const int lane = threadIdx.x & 63;
+  const int64_t e = (int64_t)blockIdx.x * (RT / 64) + (threadIdx.x >> 6);
+  if (e >= p.nout) return;
+  int64_t off[MAXP];
+  out_offsets(p, e, off);
+  Part<Tin, Tacc> pt;
+  part_init<OP, Tin, Tacc>(pt);
+  for (int64_t k = lane; k < nchunk; k += 64) part_combine<OP>(pt, parts[e * nchunk + k]);
+  wave_reduce<OP, Tin, Tacc>(pt);
+  if (lane == 0) part_store<OP, Tin, Tout, Tacc>(pt, o, off, BAD);
+}
+
+// ---------------------------------------------------------------- thread
+template <int OP, class Tin, class Tout, class Tacc, bool BAD>
+__global__ __launch_bounds__(RT) void red_thread(const Tin *__restrict__ a, RedPlan p, Outs o, BadInfo ba,
+                                                 int *flag) {
+  const Tin abv = bv_as<Tin>(ba.bv);
+  for (int64_t e = (int64_t)blockIdx.x * RT + threadIdx.x; e < p.nout; e += (int64_t)gridDim.x * RT) {
+    int64_t off[MAXP];
+    out_offsets(p, e, off);
+    const Tin *ap = a + off[0];
+    const int64_t n = p.n, ia = p.inc_n[0];
+    if constexpr (OP == R_CUMUSUM || OP == R_DCUMUSUM || OP == R_CUMUPROD || OP == R_DCUMUPROD) {
+      // Ufunc.pd:120-140 in index order (PDL's exact rounding for floats)
+      Tout *bp = (Tout *)o.p[0] + off[1];
+      const int64_t ib = p.inc_n[1];
+      const Tout obv = bv_as<Tout>(o.bv[0]);
+      constexpr bool PR = (OP == R_CUMUPROD || OP == R_DCUMUPROD);
+      Tout tmp = PR ? Tout(1) : Tout(0);
+      for (int64_t i = 0; i < n; i++) {
+        Tin x = ap[i * ia];
+        if constexpr (BAD) { if (isbad(x, abv, ba.isnan)) { bp[i * ib] = obv; continue; } }
+        tmp = PR ? wmul(tmp, (Tout)x) : wadd(tmp, (Tout)x);
+        bp[i * ib] = tmp;
+      }
+    } else if constexpr (OP == R_MIN_N_IND || OP == R_MAX_N_IND) {
+      // Ufunc.pd:484-540: m passes, each skipping indices already chosen
+      int64_t *cp = (int64_t *)o.p[0] + off[1];
+      const int64_t ic = p.inc_m, m = p.m;
+      const int64_t cbv = bv_as<int64_t>(o.bv[0]);
+      Tin cur = 0;
+      for (int64_t mi = 0; mi < m; mi++) {
+        int64_t curind = n;
+        for (int64_t i = 0; i < n; i++) {
+          bool taken = false;
+          for (int64_t mm = 0; mm < mi; mm++) if (cp[mm * ic] == i) { taken = true; break; }
+          if (taken) continue;
+          Tin x = ap[i * ia];
+          if (BAD && isbad(x, abv, ba.isnan)) continue;
+          bool better = (OP == R_MIN_N_IND) ? (x < cur) : (x > cur);
+          if (curind == n || better || is_nan(cur)) { cur = x; curind = i; }
+        }
+        if (curind != n) cp[mi * ic] = curind;
+        else { cp[mi * ic] = cbv; if (flag) atomicOr(flag, 1); }
+      }
+    } else if constexpr (is_prod(OP) && IsFloat<Tacc>::v) {
+      // Ufunc.pd:104-108: sequential, stop at a zero running product
+      Tacc tmp = 1;
+      bool any = false;
+      for (int64_t i = 0; i < n; i++) {
+        Tin x = ap[i * ia];
+        if constexpr (BAD) { if (isbad(x, abv, ba.isnan)) continue; }
+        any = true;
+        tmp *= (Tacc)x;
+        if (tmp == 0) break;
+      }
+      Tout v = (Tout)tmp;
+      if (BAD && !any) v = bv_as<Tout>(o.bv[0]);
+      put<Tout>(o.p[0], off[1], v);
+    } else {
+      Part<Tin, Tacc> pt;
+      part_init<OP, Tin, Tacc>(pt);
+      for (int64_t i = 0; i < n; i++) {
+        Tin x = ap[i * ia];
+        if constexpr (BAD) { if (isbad(x, abv, ba.isnan)) continue; }
+        part_push<OP, Tin, Tacc>(pt, x, i);
+      }
+      part_store<OP, Tin, Tout, Tacc>(pt, o, off, BAD);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- dispatch
+template <int OP, class Tin, class Tout, class Tacc, bool BAD>
+static hipError_t red_launch(const RedPlan &p, const Tin *a, const Outs &o, const BadInfo &ba, int *flag,
+                             hipStream_t s) {
+  if (p.nout == 0) return hipSuccess;
+  constexpr bool SCAN = (OP == R_CUMUSUM || OP == R_DCUMUSUM || OP == R_CUMUPROD || OP == R_DCUMUPROD);
+  constexpr bool SEQ = SCAN || OP == R_MIN_N_IND || OP == R_MAX_N_IND || (is_prod(OP) && IsFloat<Tacc>::v);
+  if constexpr (!SEQ) {
+    if (p.n >= 64 && p.inc_n[0] == 1) {
+      constexpr int VEC = 16 / sizeof(Tin);
+      bool aligned = ((uintptr_t)(a + p.off[0]) % 16 == 0);
+      for (int d = 0; d < p.nd; d++) aligned = aligned && (p.inc[0][d] % VEC == 0);
+      if (p.nout < 1024 && p.n >= 65536) {
+        // long rows, few outputs: split each row into chunks
+        int64_t want = (2048 + p.nout - 1) / p.nout;
+        int64_t clen = (p.n + want - 1) / want;
+        if (clen < 16384) clen = 16384;
+        clen = ((clen + VEC * 256 - 1) / (VEC * 256)) * (VEC * 256);
+        int64_t nchunk = (p.n + clen - 1) / clen;
+        Part<Tin, Tacc> *parts = (Part<Tin, Tacc> *)scratch_alloc(sizeof(Part<Tin, Tacc>) * p.nout * nchunk);
+        if (!parts) return hipErrorOutOfMemory;
+        if (aligned)
+          hipLaunchKernelGGL((red_split1<OP, Tin, Tout, Tacc, BAD, VEC>), dim3((unsigned)(p.nout * nchunk)), dim3(RT),
+                             0, s, a, p, nchunk, clen, parts, ba);
+        else
+          hipLaunchKernelGGL((red_split1<OP, Tin, Tout, Tacc, BAD, 1>), dim3((unsigned)(p.nout * nchunk)), dim3(RT),
+                             0, s, a, p, nchunk, clen, parts, ba);
+        hipLaunchKernelGGL((red_split2<OP, Tin, Tout, Tacc, BAD>), dim3((unsigned)((p.nout + 3) / 4)), dim3(RT), 0, s,
+                           p, nchunk, parts, o);
+        hipError_t err = hipGetLastError();
+        scratch_free(parts);
+        return err;
+      }
+      int64_t blocks = (p.nout + 3) / 4;
+      if (blocks > 65536) blocks = 65536;
+      if (aligned)
+        hipLaunchKernelGGL((red_wave<OP, Tin, Tout, Tacc, BAD, VEC>), dim3((unsigned)blocks), dim3(RT), 0, s, a, p, o,
+                           ba);
+      else
+        hipLaunchKernelGGL((red_wave<OP, Tin, Tout, Tacc, BAD, 1>), dim3((unsigned)blocks), dim3(RT), 0, s, a, p, o,
+                           ba);
+      return hipGetLastError();
+    }
+  }
+  int64_t blocks = (p.nout + RT - 1) / RT;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL((red_thread<OP, Tin, Tout, Tacc, BAD>), dim3((unsigned)blocks), dim3(RT), 0, s, a, p, o, ba,
+                     flag);
+  return hipGetLastError();
+}
+
+template <int OP, class Tin, class Tout, class Tacc>
+static hipError_t red_bad(bool bad, const RedPlan &p, const Tin *a, const Outs &o, const BadInfo &ba, int *flag,
+                          hipStream_t s) {
+  return bad ? red_launch<OP, Tin, Tout, Tacc, true>(p, a, o, ba, flag, s)
+             : red_launch<OP, Tin, Tout, Tacc, false>(p, a, o, ba, flag, s);
+}
+
+template <class Tin>
+static hipError_t reduce_T(int op, const RedPlan &p, const void *va, void *const *outs, bool bad, const BadInfo &ba,
+                           const pdlhip_value *obv, int *flag, hipStream_t s) {
+  typedef typename IPlus<Tin>::O TI;
+  const Tin *a = (const Tin *)va;
+  Outs o;
+  for (int k = 0; k < 4; k++) { o.p[k] = outs[k]; o.bv[k] = obv[k]; }
+  switch (op) {
+  case R_SUM: return red_bad<R_SUM, Tin, TI, TI>(bad, p, a, o, ba, flag, s);
+  case R_PROD: return red_bad<R_PROD, Tin, TI, TI>(bad, p, a, o, ba, flag, s);
+  case R_DSUM: return red_bad<R_DSUM, Tin, double, double>(bad, p, a, o, ba, flag, s);
+  case R_DPROD: return red_bad<R_DPROD, Tin, double, double>(bad, p, a, o, ba, flag, s);
+  case R_CUMUSUM: return red_bad<R_CUMUSUM, Tin, TI, TI>(bad, p, a, o, ba, flag, s);
+  case R_CUMUPROD: return red_bad<R_CUMUPROD, Tin, TI, TI>(bad, p, a, o, ba, flag, s);
+  case R_DCUMUSUM: return red_bad<R_DCUMUSUM, Tin, double, double>(bad, p, a, o, ba, flag, s);
+  case R_DCUMUPROD: return red_bad<R_DCUMUPROD, Tin, double, double>(bad, p, a, o, ba, flag, s);
+  case R_AVERAGE: return red_bad<R_AVERAGE, Tin, TI, TI>(bad, p, a, o, ba, flag, s);
+  case R_DAVERAGE: return red_bad<R_DAVERAGE, Tin, double, double>(bad, p, a, o, ba, flag, s);
+  case R_MIN: return red_bad<R_MIN, Tin, Tin, Tin>(bad, p, a, o, ba, flag, s);
+  case R_MAX: return red_bad<R_MAX, Tin, Tin, Tin>(bad, p, a, o, ba, flag, s);
+  case R_MIN_IND: return red_bad<R_MIN_IND, Tin, int64_t, Tin>(bad, p, a, o, ba, flag, s);
+  case R_MAX_IND: return red_bad<R_MAX_IND, Tin, int64_t, Tin>(bad, p, a, o, ba, flag, s);
+  case R_MINMAXIMUM: return red_bad<R_MINMAXIMUM, Tin, Tin, Tin>(bad, p, a, o, ba, flag, s);
+  case R_MIN_N_IND: return red_bad<R_MIN_N_IND, Tin, int64_t, Tin>(bad, p, a, o, ba, flag, s);
+  case R_MAX_N_IND: return red_bad<R_MAX_N_IND, Tin, int64_t, Tin>(bad, p, a, o, ba, flag, s);
+  default: return hipErrorInvalidValue;
+  }
+}
+
+#define PDLHIP_RED_TYPE(TC) \
+  hipError_t launch_reduce_t##TC(int op, int otype, const RedPlan &p, const void *a, void *const *outs, bool bad, \
+                                 const BadInfo &ba, const pdlhip_value *obv, int *flag, hipStream_t s) { \
+    (void)otype; \
+    return reduce_T<CType<TC>::T>(op, p, a, outs, bad, ba, obv, flag, s); \
+  }
+PDLHIP_RED_TYPES_IN_THIS_TU
+
+}  // namespace pdlhip

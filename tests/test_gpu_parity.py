@@ -1,0 +1,361 @@
+"""Parity of the HIP path (through the C ABI) with the oracle on the same
+seeded inputs.  Bars: bit-exact for integer types, indices, bad-value
+propagation, IEEE single/double elementwise ops, min/max, float products and
+scans (PDL's sequential order); float sums within the stated compensated-sum
+bound against an FP64 truth; libm functions (power, atan2) within a few ulp
+of glibc (parity unpinned beyond the reference's own vectors)."""
+import numpy as np
+import pytest
+
+import pdl_amd
+from pdl_amd import core as C
+from oracle import pdlcpu as O
+
+from harness import check_case, load_cases
+from device_backend import DeviceBackend
+
+pytestmark = pytest.mark.gpu
+
+DOC = load_cases()
+INT_T = [O.SB, O.B, O.S, O.US, O.L, O.UL, O.IND, O.ULL, O.LL]
+ALL_T = INT_T + [O.F, O.D]
+NAMES = O.TYPE_NAMES
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    C.init(0)
+    C.set_matmult_precision("split16")
+    yield
+    C.set_matmult_precision("split16")
+
+
+@pytest.mark.parametrize("case", DOC["cases"], ids=[c["id"] for c in DOC["cases"]])
+def test_golden_vector_on_device(case):
+    err = check_case(DeviceBackend(), case)
+    assert err is None, "%s (%s): %s" % (case["id"], case["src"], err)
+
+
+# ------------------------------------------------------------------ helpers
+def rand(t, shape, rng, lo=None, hi=None):
+    dt = np.dtype(O.NP_TYPES[t])
+    if dt.kind in "iu":
+        info = np.iinfo(dt)
+        lo = info.min if lo is None else lo
+        hi = info.max if hi is None else hi
+        return rng.integers(lo, hi, size=shape, endpoint=True, dtype=dt)
+    return (rng.standard_normal(shape) * (10 if lo is None else 1)).astype(dt)
+
+
+def pair(arr, t, badflag=False):
+    return O.OPdl.from_numpy(arr, t, badflag=badflag), C.ndarray.from_numpy(arr, t, badflag=badflag)
+
+
+def view(p, v):
+    for step in v:
+        p = getattr(p, step[0])(*step[1:])
+    return p
+
+
+def run_both(op, inputs, nout=1, swap=0, m_size=None):
+    """inputs: list of (oracle_pdl, device_pdl); returns [(oracle_out, device_out)]"""
+    oo = [O.OPdl.null() for _ in range(nout)]
+    dd = [C.ndarray() for _ in range(nout)]
+    O.run(op, *[i[0] for i in inputs], *oo, swap=swap, m_size=m_size)
+    C._run(op, [i[1] for i in inputs] + dd, swap=swap, m_size=m_size)
+    return list(zip(oo, dd))
+
+
+def same(o, d, what=""):
+    assert O.TYPE_NAMES[o.dtype] == C.TYPE_NAMES[d.dtype], what
+    assert list(o.dims) == list(d.dims), what
+    a, b = o.gather(), d.numpy()
+    if a.dtype.kind == "f":
+        np.testing.assert_array_equal(a.view(np.uint8 if a.dtype.itemsize == 1 else
+                                             {4: np.uint32, 8: np.uint64}[a.dtype.itemsize]),
+                                      b.view({4: np.uint32, 8: np.uint64}[b.dtype.itemsize]), err_msg=what)
+    else:
+        np.testing.assert_array_equal(a, b, err_msg=what)
+    assert bool(o.state & O.BADVAL) == d.badflag(), what
+
+
+# ------------------------------------------------------------------ binops
+BIOPS_ALL = ["plus", "mult", "minus", "divide", "gt", "lt", "le", "ge", "eq", "ne", "modulo", "spaceship"]
+BIOPS_INT = ["shiftleft", "shiftright", "or2", "and2", "xor"]
+LAYOUTS = {
+    "flat": ([7, 131], [7, 131], []),
+    "scalar": ([1000], [], []),
+    "bcast": ([33, 1, 5], [33, 6, 1], []),
+    "xchg": ([70, 130, 3], [130, 1, 3], [("xchg", 0, 1)]),
+    "slice": ([40, 30], [20, 30], [("slice", (0, 39, 2), None)]),
+}
+
+
+def binop_inputs(op, t, layout, rng):
+    sa, sb, va = LAYOUTS[layout]
+    a = rand(t, sa[::-1], rng)
+    b = rand(t, sb[::-1] if sb else (), rng)
+    dt = np.dtype(O.NP_TYPES[t])
+    if op in ("divide", "modulo") and dt.kind in "iu":
+        b = np.where(b == 0, 1, b).astype(dt)
+        if dt.kind == "i":
+            b = np.where(b == -1, 3, b).astype(dt)
+    if op in ("shiftleft", "shiftright"):
+        b = rand(t, sb[::-1] if sb else (), rng, 0, 7)
+        if op == "shiftleft" and dt.kind == "i":
+            a = np.abs(a // 4).astype(dt)
+    if op == "modulo" and dt.kind == "f":
+        a = np.round(a * 10).astype(dt)
+        b = np.round(b).astype(dt)
+    A, Bp = pair(a, t), pair(b, t)
+    A = (view(A[0], va), view(A[1], va))
+    return [A, Bp]
+
+
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+@pytest.mark.parametrize("t", ALL_T, ids=[NAMES[t] for t in ALL_T])
+def test_binops_bit_exact(t, layout):
+    rng = np.random.default_rng(t * 100 + len(layout))
+    ops = BIOPS_ALL + (BIOPS_INT if t in INT_T else [])
+    for op in ops:
+        for swap in (0, 1):
+            ins = binop_inputs(op, t, layout, rng)
+            if op in ("divide", "modulo", "shiftleft", "shiftright") and swap:
+                continue
+            (o, d), = run_both(op, ins, swap=swap)
+            same(o, d, "%s %s %s swap=%d" % (op, NAMES[t], layout, swap))
+
+
+@pytest.mark.parametrize("t", [O.F, O.D], ids=["float", "double"])
+def test_power_atan2_close_to_glibc(t):
+    """libm on the device vs glibc: parity unpinned beyond t/ops.t; bound stated"""
+    rng = np.random.default_rng(5)
+    a = np.abs(rand(t, (50, 40), rng)) + 0.5
+    b = rand(t, (50, 40), rng) / 4
+    tol = 4e-7 if t == O.F else 4e-15
+    for op in ("power", "atan2"):
+        (o, d), = run_both(op, [pair(a, t), pair(b, t)])
+        x, y = o.gather().astype(np.float64), d.numpy().astype(np.float64)
+        assert np.all(np.abs(x - y) <= tol * np.abs(x) + 1e-300), op
+
+
+@pytest.mark.parametrize("t", ALL_T, ids=[NAMES[t] for t in ALL_T])
+def test_binops_bad_values(t):
+    rng = np.random.default_rng(11 + t)
+    a = rand(t, (9, 64), rng)
+    b = rand(t, (9, 64), rng)
+    bv = O.default_badval(t)
+    a.reshape(-1)[::7] = bv
+    b.reshape(-1)[::5] = bv
+    for op in ("plus", "mult", "gt", "power" if t >= O.F else "or2", "modulo"):
+        if op == "modulo" and t >= O.F:
+            continue
+        bb = np.where(b == 0, 1, b).astype(b.dtype) if op == "modulo" else b
+        for fa, fb in ((True, False), (False, True), (True, True)):
+            (o, d), = run_both(op, [pair(a, t, fa), pair(bb, t, fb)])
+            same(o, d, "%s %s bad %s%s" % (op, NAMES[t], fa, fb))
+
+
+def test_mixed_types_convert_on_device():
+    rng = np.random.default_rng(3)
+    m = rng.integers(0, 2, size=(10, 300)).astype(np.uint8)
+    x = rng.standard_normal((10, 300)).astype(np.float32)
+    (o, d), = run_both("mult", [pair(x, O.F), pair(m, O.B)])
+    same(o, d, "float*byte")
+    (o, d), = run_both("plus", [pair(m, O.B), pair(rng.integers(-5, 5, (300,)).astype(np.int16), O.S)])
+    same(o, d, "byte+short")
+    # output of another type: computed in the output's type, written back
+    oo, dd = O.OPdl.from_numpy(np.zeros((10, 300)), O.D), C.ndarray.from_numpy(np.zeros((10, 300)), O.D)
+    O.run("plus", *pair(x, O.F)[:1], O.OPdl.from_numpy(m, O.B), oo)
+    C._run("plus", [pair(x, O.F)[1], C.ndarray.from_numpy(m, O.B), dd])
+    same(oo, dd, "into double")
+
+
+def test_inplace_through_slice():
+    rng = np.random.default_rng(4)
+    x = rand(O.L, (6, 50), rng)
+    ox, dx = pair(x, O.L)
+    ov, dv = ox.slice(None, (1, 5, 2)), dx.slice(None, (1, 5, 2))
+    one_o, one_d = pair(np.int8(3), O.SB)
+    O.run("plus", ov, one_o, ov)
+    C._run("plus", [dv, one_d, dv])
+    same(ox, dx, "inplace slice")
+
+
+# ------------------------------------------------------------------ reductions
+RED_EXACT = ["minimum", "maximum", "minimum_ind", "maximum_ind"]
+
+
+def red_layouts(rng, t):
+    return {
+        "rows": (rand(t, (37, 500), rng), []),               # wave kernel
+        "short": (rand(t, (300, 9), rng), []),                # thread kernel
+        "xchg": (rand(t, (400, 70), rng), [("xchg", 0, 1)]),  # strided rows
+        "long": (rand(t, (3, 200000), rng), []),              # split kernel
+        "empty": (rand(t, (4, 0), rng), []),
+    }
+
+
+def truth_sum(a_view, op, badval=None):
+    """sum / mean over PDL dim 0 in long double (numpy pairwise): the truth a
+    float result is scored against"""
+    x = a_view.gather().astype(np.longdouble)
+    good = np.ones(x.shape, bool) if badval is None else (a_view.gather() != np.asarray(badval).astype(a_view.gather().dtype))
+    s = np.where(good, x, 0).sum(axis=-1)
+    if op in ("average", "daverage"):
+        cnt = good.sum(axis=-1)
+        with np.errstate(all="ignore"):
+            s = np.where(cnt > 0, s / np.maximum(cnt, 1), np.nan)
+    absum = np.where(good, np.abs(x), 0).sum(axis=-1)
+    return s.astype(np.float64), absum.astype(np.float64)
+
+
+def check_sum(d, truth, absum, what, bad_expected=None):
+    """|got - truth| <= 2 eps |truth| + 64 eps^2 sum|a|  (compensated sum,
+    one rounding to the output type, one more for average's division)"""
+    y = d.numpy().astype(np.float64)
+    eps = 2.0 ** -23 if d.dtype == O.F else 2.0 ** -52
+    bound = 2 * eps * np.abs(truth) + 64 * eps * eps * absum + 1e-300
+    ok = (np.abs(truth - y) <= bound) | (np.isnan(truth) & np.isnan(y))
+    if bad_expected is not None:
+        ok = np.where(bad_expected, y == bad_expected_value(d), ok)
+    assert ok.all(), (what, truth[~ok][:5], y[~ok][:5])
+
+
+def bad_expected_value(d):
+    return np.float64(np.asarray(C.default_badvalue(d.dtype)).astype(C.NP_TYPES[d.dtype]))
+
+
+@pytest.mark.parametrize("t", ALL_T, ids=[NAMES[t] for t in ALL_T])
+def test_reductions(t):
+    rng = np.random.default_rng(21 + t)
+    for lname, (a, v) in red_layouts(rng, t).items():
+        if t >= O.F and a.size:
+            a.reshape(-1)[::97] = np.nan if lname != "long" else a.reshape(-1)[::97]
+        A = pair(a, t)
+        A = (view(A[0], v), view(A[1], v))
+        ops = ["sumover", "prodover", "dsumover", "average", "daverage", "cumusumover", "dcumusumover",
+               "cumuprodover"] + RED_EXACT
+        for op in ops:
+            if lname == "long" and op in ("cumusumover", "dcumusumover", "cumuprodover", "prodover"):
+                continue
+            (o, d), = run_both(op, [A])
+            what = "%s %s %s" % (op, NAMES[t], lname)
+            floatsum = op in ("sumover", "average", "dsumover", "daverage") and t >= O.F
+            if floatsum:
+                assert list(o.dims) == list(d.dims) and o.dtype == d.dtype, what
+                truth, absum = truth_sum(A[0], op)
+                check_sum(d, truth, absum, what)
+            elif op in ("dsumover", "daverage") and t in (O.IND, O.ULL, O.LL):
+                # 64-bit integers do not convert to double exactly: compare to truth
+                truth, absum = truth_sum(A[0], op)
+                check_sum(d, truth, absum, what)
+            else:
+                same(o, d, what)
+        outs = run_both("minmaximum", [A], nout=4)
+        for k, (o, d) in enumerate(outs):
+            same(o, d, "minmaximum %d %s %s" % (k, NAMES[t], lname))
+
+
+@pytest.mark.parametrize("t", ALL_T, ids=[NAMES[t] for t in ALL_T])
+def test_reductions_bad_values(t):
+    rng = np.random.default_rng(31 + t)
+    a = rand(t, (50, 300), rng)
+    bv = O.default_badval(t)
+    a.reshape(-1)[::3] = bv
+    a[7, :] = bv                       # an all-bad row
+    A = pair(a, t, badflag=True)
+    for op in ["sumover", "prodover", "average", "cumusumover", "minimum", "maximum", "minimum_ind",
+               "maximum_ind", "dsumover"]:
+        (o, d), = run_both(op, [A])
+        what = "%s %s bad" % (op, NAMES[t])
+        if (t >= O.F and op in ("sumover", "average")) or (op == "dsumover" and t in (O.IND, O.ULL, O.LL, O.F, O.D)):
+            truth, absum = truth_sum(A[0], op, badval=bv)
+            allbad = (a == np.asarray(bv).astype(a.dtype)).all(axis=-1)
+            check_sum(d, truth, absum, what, bad_expected=allbad)
+            assert d.badflag() == bool(o.state & O.BADVAL), what
+        else:
+            same(o, d, what)
+    for k, (o, d) in enumerate(run_both("minmaximum", [A], nout=4)):
+        same(o, d, "minmaximum bad %d" % k)
+    for op in ("minimum_n_ind", "maximum_n_ind"):
+        (o, d), = run_both(op, [pair(a[:, :40], t, True)], m_size=5)
+        same(o, d, op)
+
+
+# ------------------------------------------------------------------ matmult
+def mm_pair(h, t_, w, t, rng, views=((), ())):
+    a = rand(t, (h, t_), rng, -3, 3) if t in INT_T else rand(t, (h, t_), rng)
+    b = rand(t, (t_, w), rng, -3, 3) if t in INT_T else rand(t, (t_, w), rng)
+    A, B = pair(a, t), pair(b, t)
+    A = (view(A[0], views[0]), view(A[1], views[0]))
+    B = (view(B[0], views[1]), view(B[1], views[1]))
+    return A, B, a, b
+
+
+@pytest.mark.parametrize("t", ALL_T, ids=[NAMES[t] for t in ALL_T])
+def test_matmult_pdl_order_bit_exact(t):
+    rng = np.random.default_rng(41 + t)
+    C.set_matmult_precision("pdlorder")
+    try:
+        for (h, tt, w) in [(5, 5, 5), (70, 130, 90), (1, 64, 3), (64, 1, 64), (0, 4, 4)]:
+            A, B, _, _ = mm_pair(h, tt, w, t, rng)
+            (o, d), = run_both("matmult", [A, B])
+            same(o, d, "matmult %s %s" % (NAMES[t], (h, tt, w)))
+        # batched over a broadcast dim, transposed operand view
+        a = rand(t, (6, 20, 30), rng, -3, 3)
+        b = rand(t, (6, 40, 20), rng, -3, 3)
+        A, B = pair(a, t), pair(b, t)
+        B = (B[0].xchg(0, 1), B[1].xchg(0, 1))
+        (o, d), = run_both("matmult", [A, B])
+        same(o, d, "batched/transposed")
+    finally:
+        C.set_matmult_precision("split16")
+
+
+@pytest.mark.parametrize("mode", ["fp32", "split16"])
+def test_matmult_float_modes_accuracy(mode):
+    rng = np.random.default_rng(7)
+    C.set_matmult_precision(mode)
+    try:
+        a = rng.standard_normal((300, 257)).astype(np.float32)
+        b = rng.standard_normal((257, 190)).astype(np.float32)
+        d = C.matmult(C.ndarray.from_numpy(a), C.ndarray.from_numpy(b)).numpy().astype(np.float64)
+        truth = a.astype(np.float64) @ b.astype(np.float64)
+        err = np.linalg.norm(d - truth) / np.linalg.norm(truth)
+        assert err < 2e-6, err
+    finally:
+        C.set_matmult_precision("split16")
+
+
+def test_matmult_bad_values_exact():
+    rng = np.random.default_rng(8)
+    for t in (O.F, O.D, O.L):
+        a = rand(t, (20, 33), rng, -3, 3)
+        b = rand(t, (33, 17), rng, -3, 3)
+        a[3, 5] = O.default_badval(t)
+        b[10, 4] = O.default_badval(t)
+        (o, d), = run_both("matmult", [pair(a, t, True), pair(b, t)])
+        same(o, d, "matmult bad %s" % NAMES[t])
+
+
+# ------------------------------------------------------------------ readdata entry
+def test_readdata_entry_runs_a_planned_broadcast():
+    """pdlhip_readdata over a broadcast struct built as PDL's redodims would"""
+    import ctypes
+    from pdl_amd import _lib as L
+    rng = np.random.default_rng(9)
+    a = rng.standard_normal((8, 16)).astype(np.float32)
+    b = rng.standard_normal((16,)).astype(np.float32)
+    da, db = C.ndarray.from_numpy(a), C.ndarray.from_numpy(b)
+    dc = C.ndarray.new_dense(O.F, [16, 8])
+    brc = L.Broadcast()
+    brc.npdls, brc.ndims = 3, 2
+    brc.dims[0], brc.dims[1] = 16, 8
+    incs = [1, 1, 1, 16, 0, 16]          # incs[d*npdls + p]
+    for k, v in enumerate(incs):
+        brc.incs[k] = v
+    cs = [da._to_c(), db._to_c(), dc._to_c()]
+    arr = (L.PP * 3)(*[ctypes.pointer(c) for c in cs])
+    L.check(L.lib().pdlhip_readdata(b"plus", O.F, 0, ctypes.byref(brc), arr, None))
+    np.testing.assert_array_equal(dc.numpy(), a + b[None, :])
