@@ -124,6 +124,99 @@ def run_gpu(args):
     }
 
 
+def run_matmult(n=8192, nrows_check=16, reps=3):
+    """Precision leg: float32 A,B (n,n) normal(0,1), and the same with rows of
+    A scaled by exp(3*N(0,1)) (lognormal(0,3)); split-FP16 MFMA vs FP32 SIMT
+    (FMA) vs opt-in FP8, each with TFLOP/s and relative Frobenius error vs
+    the FP64 product on a sample of rows; PDL's own FP32 loop (the oracle)
+    scored on the same rows.  Double (n/2,n/2) on the FP64 SIMT path."""
+    from pdl_amd import core as C
+    from oracle import pdlcpu as O
+
+    rng = np.random.default_rng(11)
+    res = {"n": n, "check_rows": nrows_check}
+    rows = np.sort(rng.choice(n, nrows_check, replace=False))
+    for case in ("normal", "lognormal_rows"):
+        a = C.random_normal(n, n, dtype=C.L.F, seed=21)
+        b = C.random_normal(n, n, dtype=C.L.F, seed=22)
+        if case == "lognormal_rows":
+            s = np.exp(3.0 * rng.standard_normal(n)).astype(np.float32)   # per row h of A
+            sd = C.ndarray.from_numpy(s.reshape(n, 1), C.L.F)               # dims (1,n): broadcast over t
+            a2 = C.ndarray.new_dense(C.L.F, [n, n])
+            C._run("mult", [a, sd, a2])
+            a = a2
+        ah = a.numpy()
+        bh = b.numpy()
+        truth = ah[rows].astype(np.float64) @ bh.astype(np.float64)
+        tn = np.linalg.norm(truth)
+        out = {}
+        for mode in ("split16", "fp32", "fp8"):
+            C.set_matmult_precision(mode)
+            c = C.ndarray.new_dense(C.L.F, [n, n])
+            C._run("matmult", [a, b, c])          # warm
+            C.synchronize()
+            e0, e1 = C.Event(), C.Event()
+            e0.record()
+            for _ in range(reps):
+                C._run("matmult", [a, b, c])
+            e1.record()
+            ms = e0.elapsed_ms(e1) / reps
+            got = c.numpy()[rows].astype(np.float64)
+            out[mode] = {"ms": round(ms, 3), "TFLOP/s": round(2.0 * n ** 3 / (ms * 1e-3) / 1e12, 1),
+                         "rel_err": float(np.linalg.norm(got - truth) / tn)}
+        C.set_matmult_precision("split16")
+        # PDL's FP32 loop on the sampled rows (sequential float accumulation)
+        oa = O.OPdl.from_numpy(np.ascontiguousarray(ah[rows]), O.F)
+        ob = O.OPdl.from_numpy(bh, O.F)
+        oc = O.OPdl.null()
+        t0 = time.perf_counter()
+        O.run("matmult", oa, ob, oc)
+        dt = time.perf_counter() - t0
+        out["pdl_cpu_fp32"] = {"rel_err": float(np.linalg.norm(oc.gather().astype(np.float64) - truth) / tn),
+                               "GFLOP/s_1core": round(2.0 * nrows_check * n * n / dt / 1e9, 2)}
+        out["split16_vs_fp32_speedup"] = round(out["fp32"]["ms"] / out["split16"]["ms"], 2)
+        res[case] = out
+    # double on the FP64 path
+    m = n // 2
+    a = C.random_uniform(m, m, dtype=C.L.D, seed=31, lo=-1, hi=1)
+    b = C.random_uniform(m, m, dtype=C.L.D, seed=32, lo=-1, hi=1)
+    c = C.ndarray.new_dense(C.L.D, [m, m])
+    C._run("matmult", [a, b, c])
+    C.synchronize()
+    e0, e1 = C.Event(), C.Event()
+    e0.record()
+    C._run("matmult", [a, b, c])
+    e1.record()
+    ms = e0.elapsed_ms(e1)
+    res["f64"] = {"n": m, "ms": round(ms, 3), "TFLOP/s": round(2.0 * m ** 3 / (ms * 1e-3) / 1e12, 2)}
+    return res
+
+
+def run_single(args):
+    """one op alone, for counter runs of its kernel"""
+    from pdl_amd import core as C
+
+    C.init(0)
+    if args.only == "copy":
+        src = C.random_uniform(256, 1024, 1024, dtype=C.L.F, seed=1)      # 1 GiB
+        dst = C.ndarray.new_dense(C.L.F, src.dims)
+        f = lambda: C._run("assgn", [src, dst])
+    else:
+        x = C.random_normal(4096, 4096, 16, dtype=C.L.F, seed=1)
+        y = C.random_normal(4096, 16, dtype=C.L.F, seed=2)
+        z = C.ndarray.new_dense(C.L.F, [4096, 4096, 16])
+        if args.only == "binop":
+            xv, yv = x.xchg(0, 1), y.dummy(1)
+            f = lambda: C._run("mult", [xv, yv, z])
+        else:
+            o = C.ndarray.new_dense(C.L.F, [4096, 16])
+            f = lambda: C._run(args.only, [x, o])
+    for _ in range(args.steps):
+        f()
+    C.synchronize()
+    print(json.dumps({"only": args.only, "steps": args.steps}))
+
+
 def run_cpu_baseline(frac_k=2):
     """The oracle (the C restatement of PDL's CPU path) on a bounded sample of
     the same workload: x (4096,4096,frac_k) instead of 16 k-planes, the int
@@ -160,7 +253,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-matmult", action="store_true")
+    ap.add_argument("--only", default=None, help="run one op alone (binop, sumover, copy ...) for counters")
     args = ap.parse_args()
+    if args.only:
+        run_single(args)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus %d refused)" % args.gpus,
               file=sys.stderr)
@@ -179,6 +277,8 @@ def main():
                      "frac": per["frac"], "traffic": None, "kernel": r["dominant"]},
         "per_op": r["per_op"], "checks": r["check"],
     }
+    if not args.no_matmult:
+        out["matmult"] = run_matmult()
     if not args.no_cpu:
         out["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(out))

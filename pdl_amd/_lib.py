@@ -50,7 +50,9 @@ class Broadcast(ctypes.Structure):
 class PDLError(RuntimeError):
     """an error returned through the C ABI (PDL's barf)"""
 
-    def __init__(self, kind, message):
+    def __init__(self, kind, message=None):
+        if message is None:
+            kind, message = EUSERERROR, kind
         super().__init__(message)
         self.kind = kind
 

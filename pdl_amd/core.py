@@ -127,7 +127,7 @@ class ndarray:
         """Upload a numpy array (C order: numpy shape = reversed PDL dims)."""
         arr = np.asarray(arr)
         t = typecode(dtype) if dtype is not None else _FROM_NP.get(arr.dtype, L.D)
-        host = np.ascontiguousarray(arr, dtype=NP_TYPES[t])
+        host = np.array(arr, dtype=NP_TYPES[t], order="C", copy=True)
         p = cls()
         p.dtype = t
         p.dims = list(host.shape[::-1])

@@ -106,20 +106,22 @@ __global__ __launch_bounds__(EW_THREADS) void ew_elems(const T *__restrict__ a, 
 
 // ---------------------------------------------------------------- tile
 // Tiles of TI (dim 0, output-contiguous) x TJ (dim q, where a transposed
-// input is contiguous).  tmask bit k: input k is read through LDS.
+// input is contiguous).  TMASK bit k: input k is read through LDS.  For
+// 4-byte types each thread moves 16-byte vectors on both sides: the
+// transposed input is read 4-wide along j, the output written 4-wide along i.
 constexpr int TI = 64, TJ = 64;
 
-template <class T, int OP>
+template <class T, int OP, int TMASK, int V>
 __global__ __launch_bounds__(EW_THREADS) void ew_tile(const T *__restrict__ a, const T *__restrict__ b,
-                                                     T *__restrict__ c, EwPlan p, int q, int tmask,
-                                                     int64_t ntile_i, int64_t ntile_j) {
-  __shared__ T lds[2][TI][TJ + 1];
+                                                     T *__restrict__ c, EwPlan p, int q, int64_t ntile_i,
+                                                     int64_t ntile_j) {
+  constexpr int NL = ((TMASK & 1) ? 1 : 0) + ((TMASK & 2) ? 1 : 0);
+  __shared__ T lds[NL][TI][TJ + 1];
   int64_t blk = blockIdx.x;
   const int64_t ti = blk % ntile_i;
   blk /= ntile_i;
   const int64_t tj = blk % ntile_j;
   int64_t rest = blk / ntile_j;
-  // offsets of the remaining dims
   int64_t o[3] = {p.off[0], p.off[1], p.off[2]};
   for (int d = 1; d < p.nd; d++) {
     if (d == q) continue;
@@ -129,33 +131,90 @@ __global__ __launch_bounds__(EW_THREADS) void ew_tile(const T *__restrict__ a, c
   }
   const int64_t i0 = ti * TI, j0 = tj * TJ;
   const int64_t ni = p.dims[0], nj = p.dims[q];
+  const bool full = (i0 + TI <= ni) && (j0 + TJ <= nj);
   const T *src[2] = {a, b};
-  // load transposed inputs: consecutive threads along j (their unit stride)
+  // load: lanes along j, V elements each
+  constexpr int LJ = TJ / V;                  // threads per tile row
+  constexpr int LR = EW_THREADS / LJ;         // rows per pass
   {
-    const int jj = threadIdx.x % TJ;
-    const int ii0 = threadIdx.x / TJ;
+    const int jj = (threadIdx.x % LJ) * V;
+    const int ii0 = threadIdx.x / LJ;
 #pragma unroll
     for (int k = 0; k < 2; k++) {
-      if (!(tmask & (1 << k))) continue;
-      const int64_t inc_i = p.inc[k][0], inc_j = p.inc[k][q];
-      for (int ii = ii0; ii < TI; ii += EW_THREADS / TJ) {
+      if (!(TMASK & (1 << k))) continue;
+      constexpr int dummy = 0;
+      (void)dummy;
+      const int slot = (k == 1 && (TMASK & 1)) ? 1 : 0;
+      const int64_t inc_i = p.inc[k][0];
+#pragma unroll
+      for (int ii = ii0; ii < TI; ii += LR) {
         const int64_t gi = i0 + ii, gj = j0 + jj;
-        if (gi < ni && gj < nj) lds[k][ii][jj] = src[k][o[k] + gi * inc_i + gj * inc_j];
+        const T *sp = src[k] + o[k] + gi * inc_i + gj;
+        if (V > 1 && full) {
+          Vec<T, V> v = *reinterpret_cast<const Vec<T, V> *>(sp);
+#pragma unroll
+          for (int e = 0; e < V; e++) lds[slot][ii][jj + e] = v.v[e];
+        } else {
+#pragma unroll
+          for (int e = 0; e < V; e++)
+            if (gi < ni && gj + e < nj) lds[slot][ii][jj + e] = sp[e];
+        }
       }
     }
   }
   __syncthreads();
-  // compute + store: consecutive threads along i (the output's unit stride)
-  const int ii = threadIdx.x % TI;
-  const int jj0 = threadIdx.x / TI;
+  // compute + store: lanes along i, V elements each
+  constexpr int LI = TI / V;
+  constexpr int LC = EW_THREADS / LI;
+  const int ii = (threadIdx.x % LI) * V;
+  const int jj0 = threadIdx.x / LI;
   const int64_t gi = i0 + ii;
-  if (gi >= ni) return;
-  for (int jj = jj0; jj < TJ; jj += EW_THREADS / TI) {
+#pragma unroll
+  for (int jj = jj0; jj < TJ; jj += LC) {
     const int64_t gj = j0 + jj;
-    if (gj >= nj) break;
-    T x = (tmask & 1) ? lds[0][ii][jj] : a[o[0] + gi * p.inc[0][0] + gj * p.inc[0][q]];
-    T y = (tmask & 2) ? lds[1][ii][jj] : b[o[1] + gi * p.inc[1][0] + gj * p.inc[1][q]];
-    c[o[2] + gi * p.inc[2][0] + gj * p.inc[2][q]] = binop<OP, T>(x, y);
+    T x[V], y[V], r[V];
+#pragma unroll
+    for (int e = 0; e < V; e++) {
+      if constexpr (TMASK & 1) x[e] = lds[0][ii + e][jj];
+      if constexpr (TMASK & 2) y[e] = lds[(TMASK & 1) ? 1 : 0][ii + e][jj];
+    }
+    if (V > 1 && full) {
+      if constexpr (!(TMASK & 1)) {
+        const T *ap = a + o[0] + gi * p.inc[0][0] + gj * p.inc[0][q];
+        if (p.inc[0][0] == 0) {
+#pragma unroll
+          for (int e = 0; e < V; e++) x[e] = ap[0];
+        } else {
+          Vec<T, V> v = *reinterpret_cast<const Vec<T, V> *>(ap);
+#pragma unroll
+          for (int e = 0; e < V; e++) x[e] = v.v[e];
+        }
+      }
+      if constexpr (!(TMASK & 2)) {
+        const T *bp = b + o[1] + gi * p.inc[1][0] + gj * p.inc[1][q];
+        if (p.inc[1][0] == 0) {
+#pragma unroll
+          for (int e = 0; e < V; e++) y[e] = bp[0];
+        } else {
+          Vec<T, V> v = *reinterpret_cast<const Vec<T, V> *>(bp);
+#pragma unroll
+          for (int e = 0; e < V; e++) y[e] = v.v[e];
+        }
+      }
+      Vec<T, V> out;
+#pragma unroll
+      for (int e = 0; e < V; e++) out.v[e] = binop<OP, T>(x[e], y[e]);
+      *reinterpret_cast<Vec<T, V> *>(c + o[2] + gi + gj * p.inc[2][q]) = out;
+    } else {
+#pragma unroll
+      for (int e = 0; e < V; e++) {
+        if (gi + e >= ni || gj >= nj) continue;
+        if constexpr (!(TMASK & 1)) x[e] = a[o[0] + (gi + e) * p.inc[0][0] + gj * p.inc[0][q]];
+        if constexpr (!(TMASK & 2)) y[e] = b[o[1] + (gi + e) * p.inc[1][0] + gj * p.inc[1][q]];
+        r[e] = binop<OP, T>(x[e], y[e]);
+        c[o[2] + (gi + e) * p.inc[2][0] + gj * p.inc[2][q]] = r[e];
+      }
+    }
   }
 }
 
@@ -207,9 +266,28 @@ static hipError_t binop_T_OP(const EwPlan &p, const void *va, const void *vb, vo
         int64_t nti = (p.dims[0] + TI - 1) / TI, ntj = (p.dims[q] + TJ - 1) / TJ;
         int64_t nrest = p.n / (p.dims[0] * p.dims[q]);
         int64_t nblk = nti * ntj * nrest;
+        // 16-byte vectors need every row start of every operand 16-byte aligned
+        constexpr int V4 = (sizeof(T) == 4) ? 4 : ((sizeof(T) == 8) ? 2 : 1);
+        bool vec = V4 > 1;
+        for (int k = 0; k < 2; k++)  // an input read directly must be unit-stride or broadcast along i
+          if (!(tmask & (1 << k)) && p.inc[k][0] != 0 && p.inc[k][0] != 1) vec = false;
+        for (int k = 0; k < 3 && vec; k++) {
+          const void *base = k == 0 ? (const void *)a : k == 1 ? (const void *)b : (const void *)c;
+          if (((uintptr_t)base + p.off[k] * sizeof(T)) % 16) vec = false;
+          for (int d = 0; d < p.nd && vec; d++) {
+            bool unit = (k < 2 && (tmask & (1 << k))) ? (d == q) : (d == 0);
+            if (!unit && p.inc[k][d] % V4) vec = false;
+          }
+        }
         if (nblk < (int64_t)1 << 31) {
-          hipLaunchKernelGGL((ew_tile<T, OP>), dim3((unsigned)nblk), dim3(EW_THREADS), 0, s, a, b, c, p, q, tmask,
-                             nti, ntj);
+#define TILE_LAUNCH(TM, VV) hipLaunchKernelGGL((ew_tile<T, OP, TM, VV>), dim3((unsigned)nblk), dim3(EW_THREADS), 0, s, \
+                                               a, b, c, p, q, nti, ntj)
+          if (vec) {
+            if (tmask == 1) TILE_LAUNCH(1, V4); else if (tmask == 2) TILE_LAUNCH(2, V4); else TILE_LAUNCH(3, V4);
+          } else {
+            if (tmask == 1) TILE_LAUNCH(1, 1); else if (tmask == 2) TILE_LAUNCH(2, 1); else TILE_LAUNCH(3, 1);
+          }
+#undef TILE_LAUNCH
           return hipGetLastError();
         }
       }

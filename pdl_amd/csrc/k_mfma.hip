@@ -1,6 +1,22 @@
-// MFMA matmult paths for float ndarrays (gfx950).  Until the split-FP16 and
-// FP8 kernels are in place this reports hipErrorNotSupported and the caller
-// uses the FP32 SIMT kernel.
+// MFMA matmult for float ndarrays on gfx950 (lib/PDL/Primitive.pd:191-245
+// computes the same C[h][w] = sum_t A[h][t] B[t][w] with a scalar loop).
+//
+// Split-FP16 ("split16"): each float operand element x, pre-scaled by a
+// power of two per row of A / per column of B so that the row's largest
+// magnitude lies in [2^14, 2^15), is written as hi = fp16(x), lo = fp16(x - hi)
+// (22 significant bits together).  The GEMM accumulates hi*hi + hi*lo + lo*hi
+// in FP32 on the matrix cores (v_mfma_f32_16x16x32_f16) and the epilogue
+// undoes the scaling exactly (powers of two).  The dropped lo*lo term is
+// below 2^-22 relative per product.
+//
+// FP8 ("fp8", opt-in): the same pre-scaling puts each row/column's max in
+// [128, 256) (below e4m3's largest normal, 448) and a single e4m3 product per
+// tile runs on v_mfma_f32_16x16x32_fp8_fp8.
+//
+// Layout: packed operands are [row][Kp] with K contiguous, rows padded to the
+// 128-row block tile and K to 32 (zeros).  Block = 256 threads = 4 waves in a
+// 2x2 arrangement, each wave owns a 64x64 piece of the 128x128 C tile as
+// 4x4 MFMA 16x16 tiles.
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
@@ -8,9 +24,263 @@
 
 namespace pdlhip {
 
+void *scratch_alloc(size_t bytes);
+void scratch_free(void *p);
+
+typedef _Float16 half_t;
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int MB = 128, NB = 128, KB = 32, KPAD = KB + 8;
+
+// power of two that brings max|row| into [2^(e-1), 2^e) with e = target
+__device__ inline float pow2_scale(float m, int target) {
+  if (!(m > 0.0f) || !isfinite(m)) return 1.0f;
+  int e;
+  frexpf(m, &e);                 // m = f * 2^e, f in [0.5, 1)
+  int x = target - e;
+  x = x > 120 ? 120 : (x < -120 ? -120 : x);
+  return ldexpf(1.0f, x);
+}
+
+// max |x| over k of each of R rows: one wave per row when k is the unit-stride
+// index, else one thread per row (adjacent threads on adjacent rows)
+__global__ __launch_bounds__(256) void absmax_rows_wave(const float *__restrict__ src, int64_t o, int64_t inc_r,
+                                                        int64_t inc_k, int64_t R, int64_t K, float *smax) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= R) return;
+  float m = 0.0f;
+  for (int64_t k = lane; k < K; k += 64) m = fmaxf(m, fabsf(src[o + r * inc_r + k * inc_k]));
+  for (int s = 32; s >= 1; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
+  if (lane == 0) smax[r] = m;
+}
+
+__global__ __launch_bounds__(256) void absmax_rows_thread(const float *__restrict__ src, int64_t o, int64_t inc_r,
+                                                          int64_t inc_k, int64_t R, int64_t K, float *smax) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= R) return;
+  float m = 0.0f;
+  for (int64_t k = 0; k < K; k++) m = fmaxf(m, fabsf(src[o + r * inc_r + k * inc_k]));
+  smax[r] = m;
+}
+
+// scale[r] from smax[r]; zero for padded rows
+__global__ __launch_bounds__(256) void make_scales(float *s, int64_t R, int64_t Rp, int target) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= Rp) return;
+  s[r] = r < R ? pow2_scale(s[r], target) : 1.0f;
+}
+
+// pack a (R x K) operand into hi/lo fp16 [Rp][Kp] through a 64x64 LDS tile so
+// that both the strided read and the K-contiguous write are coalesced
+template <bool FP8>
+__global__ __launch_bounds__(256) void pack_tile(const float *__restrict__ src, int64_t o, int64_t inc_r,
+                                                 int64_t inc_k, int64_t R, int64_t K, int64_t Kp,
+                                                 const float *__restrict__ scale, half_t *__restrict__ hi,
+                                                 half_t *__restrict__ lo, uint8_t *__restrict__ q8,
+                                                 int64_t ntk) {
+  __shared__ float t[64][65];
+  const int64_t r0 = (blockIdx.x / ntk) * 64, k0 = (blockIdx.x % ntk) * 64;
+  const bool kfast = (inc_k == 1 || inc_k == -1 || inc_r != 1);
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    int rr, kk;
+    if (kfast) { kk = e % 64; rr = e / 64; } else { rr = e % 64; kk = e / 64; }
+    const int64_t gr = r0 + rr, gk = k0 + kk;
+    t[rr][kk] = (gr < R && gk < K) ? src[o + gr * inc_r + gk * inc_k] : 0.0f;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int kk = e % 64, rr = e / 64;
+    const int64_t gr = r0 + rr, gk = k0 + kk;
+    if (gk >= Kp) continue;
+    const float x = t[rr][kk] * scale[gr];
+    if constexpr (FP8) {
+      q8[gr * Kp + gk] = (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false) & 0xff);
+    } else {
+      const half_t h = (half_t)x;
+      hi[gr * Kp + gk] = h;
+      lo[gr * Kp + gk] = (half_t)(x - (float)h);
+    }
+  }
+}
+
+__device__ inline void mm_batch_offsets_m(const MmPlan &p, int64_t bidx, int64_t *o) {
+  o[0] = p.off[0]; o[1] = p.off[1]; o[2] = p.off[2];
+  for (int d = 0; d < p.nd; d++) {
+    int64_t i = bidx % p.dims[d];
+    bidx /= p.dims[d];
+    o[0] += i * p.inc[0][d]; o[1] += i * p.inc[1][d]; o[2] += i * p.inc[2][d];
+  }
+}
+
+__global__ __launch_bounds__(256) void split16_gemm(const half_t *__restrict__ Ahi, const half_t *__restrict__ Alo,
+                                                   const half_t *__restrict__ Bhi, const half_t *__restrict__ Blo,
+                                                   const float *__restrict__ sA, const float *__restrict__ sB,
+                                                   float *__restrict__ c, MmPlan p, int64_t Kp, int64_t ntn) {
+  __shared__ half_t lA[2][MB][KPAD];
+  __shared__ half_t lB[2][NB][KPAD];
+  const int64_t tm = blockIdx.x / ntn, tn = blockIdx.x % ntn;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  const half_t *ah = Ahi + tm * MB * Kp, *al = Alo + tm * MB * Kp;
+  const half_t *bh = Bhi + tn * NB * Kp, *bl = Blo + tn * NB * Kp;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int64_t k0 = 0; k0 < Kp; k0 += KB) {
+    // 128 rows x 32 halves = 512 half8 per array; 2 per thread
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      const int idx = tid + 256 * s;
+      const int row = idx >> 2, seg = (idx & 3) * 8;
+      *reinterpret_cast<half8 *>(&lA[0][row][seg]) = *reinterpret_cast<const half8 *>(ah + row * Kp + k0 + seg);
+      *reinterpret_cast<half8 *>(&lA[1][row][seg]) = *reinterpret_cast<const half8 *>(al + row * Kp + k0 + seg);
+      *reinterpret_cast<half8 *>(&lB[0][row][seg]) = *reinterpret_cast<const half8 *>(bh + row * Kp + k0 + seg);
+      *reinterpret_cast<half8 *>(&lB[1][row][seg]) = *reinterpret_cast<const half8 *>(bl + row * Kp + k0 + seg);
+    }
+    __syncthreads();
+    half8 fah[4], fal[4], fbh[4], fbl[4];
+    const int lr = lane & 15, lk = (lane >> 4) * 8;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      fah[i] = *reinterpret_cast<const half8 *>(&lA[0][wr * 64 + i * 16 + lr][lk]);
+      fal[i] = *reinterpret_cast<const half8 *>(&lA[1][wr * 64 + i * 16 + lr][lk]);
+      fbh[i] = *reinterpret_cast<const half8 *>(&lB[0][wc * 64 + i * 16 + lr][lk]);
+      fbl[i] = *reinterpret_cast<const half8 *>(&lB[1][wc * 64 + i * 16 + lr][lk]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fal[i], fbh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah[i], fbl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah[i], fbh[j], acc[i][j], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+  // D layout of a 16x16 tile: lane l, reg r -> row 4*(l/16)+r, col l%16
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int64_t h = tm * MB + wr * 64 + i * 16 + 4 * (lane >> 4) + r;
+        const int64_t wcol = tn * NB + wc * 64 + j * 16 + (lane & 15);
+        if (h < p.H && wcol < p.W)
+          c[p.off[2] + wcol * p.c_w + h * p.c_h] = (acc[i][j][r] * (1.0f / sA[h])) * (1.0f / sB[wcol]);
+      }
+}
+
+// FP8 e4m3: one product per tile
+__global__ __launch_bounds__(256) void fp8_gemm(const uint8_t *__restrict__ A8, const uint8_t *__restrict__ B8,
+                                               const float *__restrict__ sA, const float *__restrict__ sB,
+                                               float *__restrict__ c, MmPlan p, int64_t Kp, int64_t ntn) {
+  __shared__ uint8_t lA[MB][KB + 16];
+  __shared__ uint8_t lB[NB][KB + 16];
+  const int64_t tm = blockIdx.x / ntn, tn = blockIdx.x % ntn;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  const uint8_t *a8 = A8 + tm * MB * Kp, *b8 = B8 + tn * NB * Kp;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int64_t k0 = 0; k0 < Kp; k0 += KB) {
+    // 128 rows x 32 bytes = 256 x 16 B per array: one per thread
+    {
+      const int row = tid >> 1, seg = (tid & 1) * 16;
+      *reinterpret_cast<uint4 *>(&lA[row][seg]) = *reinterpret_cast<const uint4 *>(a8 + row * Kp + k0 + seg);
+      *reinterpret_cast<uint4 *>(&lB[row][seg]) = *reinterpret_cast<const uint4 *>(b8 + row * Kp + k0 + seg);
+    }
+    __syncthreads();
+    const int lr = lane & 15, lk = (lane >> 4) * 8;
+    long fa[4], fb[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      fa[i] = *reinterpret_cast<const long *>(&lA[wr * 64 + i * 16 + lr][lk]);
+      fb[i] = *reinterpret_cast<const long *>(&lB[wc * 64 + i * 16 + lr][lk]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int64_t h = tm * MB + wr * 64 + i * 16 + 4 * (lane >> 4) + r;
+        const int64_t wcol = tn * NB + wc * 64 + j * 16 + (lane & 15);
+        if (h < p.H && wcol < p.W)
+          c[p.off[2] + wcol * p.c_w + h * p.c_h] = (acc[i][j][r] * (1.0f / sA[h])) * (1.0f / sB[wcol]);
+      }
+}
+
+static hipError_t prep_operand(const float *src, int64_t o, int64_t inc_r, int64_t inc_k, int64_t R, int64_t K,
+                               int64_t Rp, int64_t Kp, float *scale, half_t *hi, half_t *lo, uint8_t *q8, bool fp8,
+                               hipStream_t s) {
+  if (inc_k == 1 || inc_k == -1)
+    hipLaunchKernelGGL(absmax_rows_wave, dim3((unsigned)((R + 3) / 4)), dim3(256), 0, s, src, o, inc_r, inc_k, R, K,
+                       scale);
+  else
+    hipLaunchKernelGGL(absmax_rows_thread, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, src, o, inc_r, inc_k,
+                       R, K, scale);
+  hipLaunchKernelGGL(make_scales, dim3((unsigned)((Rp + 255) / 256)), dim3(256), 0, s, scale, R, Rp, fp8 ? 8 : 15);
+  const int64_t ntr = (Rp + 63) / 64, ntk = (Kp + 63) / 64;
+  if (fp8)
+    hipLaunchKernelGGL((pack_tile<true>), dim3((unsigned)(ntr * ntk)), dim3(256), 0, s, src, o, inc_r, inc_k, R, K, Kp,
+                       scale, hi, lo, q8, ntk);
+  else
+    hipLaunchKernelGGL((pack_tile<false>), dim3((unsigned)(ntr * ntk)), dim3(256), 0, s, src, o, inc_r, inc_k, R, K,
+                       Kp, scale, hi, lo, q8, ntk);
+  return hipGetLastError();
+}
+
 hipError_t launch_mfma_f32(const MmPlan &p, const float *a, const float *b, float *c, int mode, hipStream_t s) {
-  (void)p; (void)a; (void)b; (void)c; (void)mode; (void)s;
-  return hipErrorNotSupported;
+  // one matrix product per call (broadcast batches use the SIMT kernel), and
+  // only where a 128x128 tile is mostly useful
+  if (p.nbatch != 1 || p.H < 64 || p.W < 64 || p.T < 32) return hipErrorNotSupported;
+  const bool fp8 = (mode == PDLHIP_MM_FP8);
+  const int64_t Hp = (p.H + MB - 1) / MB * MB, Wp = (p.W + NB - 1) / NB * NB, Kp = (p.T + KB - 1) / KB * KB;
+  const size_t esz = fp8 ? 1 : 2, ncopy = fp8 ? 1 : 2;
+  char *buf = (char *)scratch_alloc((size_t)(Hp + Wp) * Kp * esz * ncopy + (size_t)(Hp + Wp) * sizeof(float) + 256);
+  if (!buf) return hipErrorOutOfMemory;
+  float *sA = (float *)buf, *sB = sA + Hp;
+  char *q = buf + ((size_t)(Hp + Wp) * sizeof(float) + 255) / 256 * 256;
+  half_t *Ahi = nullptr, *Alo = nullptr, *Bhi = nullptr, *Blo = nullptr;
+  uint8_t *A8 = nullptr, *B8 = nullptr;
+  if (fp8) {
+    A8 = (uint8_t *)q;
+    B8 = A8 + Hp * Kp;
+  } else {
+    Ahi = (half_t *)q;
+    Alo = Ahi + Hp * Kp;
+    Bhi = Alo + Hp * Kp;
+    Blo = Bhi + Wp * Kp;
+  }
+  int64_t o[3] = {p.off[0], p.off[1], p.off[2]};
+  // A rows = h (stride a_h), k = t (stride a_t); B "rows" = w (b_w), k = t (b_t)
+  hipError_t e = prep_operand(a, o[0], p.a_h, p.a_t, p.H, p.T, Hp, Kp, sA, Ahi, Alo, A8, fp8, s);
+  if (e == hipSuccess) e = prep_operand(b, o[1], p.b_w, p.b_t, p.W, p.T, Wp, Kp, sB, Bhi, Blo, B8, fp8, s);
+  const int64_t ntm = Hp / MB, ntn = Wp / NB;
+  if (e == hipSuccess) {
+    if (fp8)
+      hipLaunchKernelGGL(fp8_gemm, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, A8, B8, sA, sB, c, p, Kp, ntn);
+    else
+      hipLaunchKernelGGL(split16_gemm, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, Ahi, Alo, Bhi, Blo, sA, sB, c, p,
+                         Kp, ntn);
+    e = hipGetLastError();
+  }
+  scratch_free(buf);
+  return e;
 }
 
 }  // namespace pdlhip

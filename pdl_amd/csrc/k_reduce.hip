@@ -94,11 +94,15 @@ template <int OP, class Tin, class Tacc> __device__ inline void part_push(Part<T
     p.s = wmul(p.s, (Tacc)x);
     p.cnt++;
   } else if constexpr (is_minmax(OP)) {
-    if (p.i0 < 0 || pref<is_max(OP)>(x, i, p.v0, p.i0)) { p.v0 = x; p.i0 = i; }
+    // elements reach a lane in increasing index order, so PDL's sequential
+    // rule (Ufunc.pd:458-460) is exact here: replace on strict improvement or
+    // when the current value is NaN
+    bool better = is_max(OP) ? (x > p.v0) : (x < p.v0);
+    if (p.i0 < 0 || better || is_nan(p.v0)) { p.v0 = x; p.i0 = i; }
   } else if constexpr (OP == R_MINMAXIMUM) {
     if (is_nan(x)) return;
-    if (p.i0 < 0 || x < p.v0 || (x == p.v0 && i < p.i0)) { p.v0 = x; p.i0 = i; }
-    if (p.i1 < 0 || x > p.v1 || (x == p.v1 && i < p.i1)) { p.v1 = x; p.i1 = i; }
+    if (p.i0 < 0 || x < p.v0) { p.v0 = x; p.i0 = i; }
+    if (p.i1 < 0 || x > p.v1) { p.v1 = x; p.i1 = i; }
   }
 }
 
@@ -156,6 +160,25 @@ template <class T> __device__ inline T shfl_xor(T v, int m) {
     int x = (int)v;
     x = __shfl_xor(x, m, 64);
     return (T)x;
+  }
+}
+
+template <class T> __device__ inline T shfl_up(T v, int d) {
+  if constexpr (sizeof(T) == 8) {
+    int64_t x;
+    __builtin_memcpy(&x, &v, 8);
+    int lo = __shfl_up((int)(x & 0xffffffff), d, 64), hi = __shfl_up((int)(x >> 32), d, 64);
+    x = ((int64_t)(uint32_t)lo) | ((int64_t)hi << 32);
+    T r;
+    __builtin_memcpy(&r, &x, 8);
+    return r;
+  } else {
+    int x;
+    __builtin_memcpy(&x, &v, 4);
+    x = __shfl_up(x, d, 64);
+    T r;
+    __builtin_memcpy(&r, &x, 4);
+    return r;
   }
 }
 
@@ -320,6 +343,94 @@ __global__ __launch_bounds__(RT) void red_split2(RedPlan p, int64_t nchunk, cons
   if (lane == 0) part_store<OP, Tin, Tout, Tacc>(pt, o, off, BAD);
 }
 
+// ---------------------------------------------------------------- scan
+// Integer cumusumover/cumuprodover with unit-stride rows: one block per row,
+// chunks of 4096 staged through LDS (coalesced in and out), a sequential scan
+// of 16 elements per thread, then wave and block prefix combines.  Wrapping
+// integer + and * are associative, so the result has PDL's exact bits.  A bad
+// element contributes the identity and is written as the output's bad value
+// (Ufunc.pd:127-132).
+constexpr int SC_IT = 16, SC_CH = RT * SC_IT;
+__device__ inline int sc_pad(int i) { return i + (i >> 4); }
+
+template <int OP, class Tin, class Tout, bool BAD>
+__global__ __launch_bounds__(RT) void scan_rows(const Tin *__restrict__ a, RedPlan p, Outs o, BadInfo ba) {
+  constexpr bool PR = (OP == R_CUMUPROD);
+  __shared__ Tout sh[SC_CH + SC_CH / 16];
+  __shared__ uint8_t badm[BAD ? SC_CH : 1];
+  __shared__ Tout wsum[RT / 64];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const Tin abv = bv_as<Tin>(ba.bv);
+  const Tout obv = bv_as<Tout>(o.bv[0]);
+  const Tout ident = PR ? Tout(1) : Tout(0);
+  for (int64_t e = blockIdx.x; e < p.nout; e += gridDim.x) {
+    int64_t off[MAXP];
+    out_offsets(p, e, off);
+    const Tin *ap = a + off[0];
+    Tout *bp = (Tout *)o.p[0] + off[1];
+    Tout carry = ident;
+    for (int64_t c0 = 0; c0 < p.n; c0 += SC_CH) {
+      const int len = (p.n - c0 < SC_CH) ? (int)(p.n - c0) : SC_CH;
+#pragma unroll
+      for (int k = 0; k < SC_IT; k++) {
+        const int idx = tid + RT * k;
+        Tout v = ident;
+        if (idx < len) {
+          Tin x = ap[c0 + idx];
+          bool isb = false;
+          if constexpr (BAD) { isb = isbad(x, abv, ba.isnan); badm[idx] = isb; }
+          if (!isb) v = (Tout)x;
+        }
+        sh[sc_pad(idx)] = v;
+      }
+      __syncthreads();
+      // sequential scan of this thread's 16 consecutive elements
+      Tout run = ident;
+      const int base = tid * SC_IT;
+#pragma unroll
+      for (int j = 0; j < SC_IT; j++) {
+        run = PR ? wmul(run, sh[sc_pad(base + j)]) : wadd(run, sh[sc_pad(base + j)]);
+        sh[sc_pad(base + j)] = run;
+      }
+      // inclusive wave scan of the thread totals
+      Tout incl = run;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        Tout t = shfl_up(incl, d);
+        if (lane >= d) incl = PR ? wmul(t, incl) : wadd(t, incl);
+      }
+      Tout excl = shfl_up(incl, 1);
+      if (lane == 0) excl = ident;
+      if (lane == 63) wsum[w] = incl;
+      __syncthreads();
+      Tout wpre = ident, tot = ident;
+#pragma unroll
+      for (int k = 0; k < RT / 64; k++) {
+        if (k < w) wpre = PR ? wmul(wpre, wsum[k]) : wadd(wpre, wsum[k]);
+        tot = PR ? wmul(tot, wsum[k]) : wadd(tot, wsum[k]);
+      }
+      Tout pre = PR ? wmul(wmul(carry, wpre), excl) : wadd(wadd(carry, wpre), excl);
+#pragma unroll
+      for (int j = 0; j < SC_IT; j++) {
+        Tout v = sh[sc_pad(base + j)];
+        sh[sc_pad(base + j)] = PR ? wmul(pre, v) : wadd(pre, v);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < SC_IT; k++) {
+        const int idx = tid + RT * k;
+        if (idx < len) {
+          Tout v = sh[sc_pad(idx)];
+          if constexpr (BAD) { if (badm[idx]) v = obv; }
+          bp[c0 + idx] = v;
+        }
+      }
+      carry = PR ? wmul(carry, tot) : wadd(carry, tot);
+      __syncthreads();
+    }
+  }
+}
+
 // ---------------------------------------------------------------- thread
 template <int OP, class Tin, class Tout, class Tacc, bool BAD>
 __global__ __launch_bounds__(RT) void red_thread(const Tin *__restrict__ a, RedPlan p, Outs o, BadInfo ba,
@@ -361,7 +472,12 @@ __global__ __launch_bounds__(RT) void red_thread(const Tin *__restrict__ a, RedP
           if (curind == n || better || is_nan(cur)) { cur = x; curind = i; }
         }
         if (curind != n) cp[mi * ic] = curind;
-        else { cp[mi * ic] = cbv; if (flag) atomicOr(flag, 1); }
+        else {
+          cp[mi * ic] = cbv;
+          // PDL clears c's bad flag at the start of every broadcast iteration
+          // and sets it on a shortfall, so the last iteration decides
+          if (flag && e == p.nout - 1) atomicOr(flag, 1);
+        }
       }
     } else if constexpr (is_prod(OP) && IsFloat<Tacc>::v) {
       // Ufunc.pd:104-108: sequential, stop at a zero running product
@@ -431,6 +547,13 @@ static hipError_t red_launch(const RedPlan &p, const Tin *a, const Outs &o, cons
       else
         hipLaunchKernelGGL((red_wave<OP, Tin, Tout, Tacc, BAD, 1>), dim3((unsigned)blocks), dim3(RT), 0, s, a, p, o,
                            ba);
+      return hipGetLastError();
+    }
+  }
+  if constexpr ((OP == R_CUMUSUM || OP == R_CUMUPROD) && !IsFloat<Tout>::v) {
+    if (p.inc_n[0] == 1 && p.inc_n[1] == 1 && p.n >= 256) {
+      int64_t blocks = p.nout < 65536 ? p.nout : 65536;
+      hipLaunchKernelGGL((scan_rows<OP, Tin, Tout, BAD>), dim3((unsigned)blocks), dim3(RT), 0, s, a, p, o, ba);
       return hipGetLastError();
     }
   }

