@@ -38,6 +38,16 @@ def algorithmic_bytes(nx, ny, nk, ni, nj, mi, mj):
             "i32_sumover": isum, "i32_cumusumover": iscan, "i16bad_sumover": bsum}
 
 
+def measured_traffic(op):
+    """HBM bytes per launch of this op's kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (scaled by a 1 GiB copy), or None"""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            return json.load(f)["bytes_per_launch"].get(op)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def make_int_inputs(ni, nj, mi, mj, seed=0):
     rng = np.random.default_rng(seed)
     i32 = rng.integers(-2**20, 2**20, size=(nj, ni), endpoint=True, dtype=np.int32)
@@ -274,7 +284,7 @@ def main():
                                "i16 (8192,8192) 1% BAD sumover",
                    "bytes_per_step": r["total_bytes"], "frac_of_hbm_peak": round(r["value"] / HBM_PEAK_GBS, 3)},
         "roofline": {"bound": "hbm", "achieved": per["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": per["frac"], "traffic": None, "kernel": r["dominant"]},
+                     "frac": per["frac"], "traffic": measured_traffic(r["dominant"]), "kernel": r["dominant"]},
         "per_op": r["per_op"], "checks": r["check"],
     }
     if not args.no_matmult:

@@ -36,6 +36,7 @@ def units():
         u.append(("k_matmult.hip", "mm_t%d" % t, ["-DPDLHIP_MM_TYPES_IN_THIS_TU=PDLHIP_MM_TYPE(%d)" % t]))
     u.append(("k_misc.hip", "misc", []))
     u.append(("k_mfma.hip", "mfma", []))
+    u.append(("k_batch.hip", "batch", []))
     u.append(("api.cpp", "api", ["-x", "hip"]))
     return u
 

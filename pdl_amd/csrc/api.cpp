@@ -24,6 +24,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/pdlhip.h"
@@ -60,7 +61,14 @@ static const char *type_name(int t) {
 }
 
 // ------------------------------------------------------------------ state
+struct Deferred {
+  std::vector<MmItem> items;
+  int T = -1;
+  bool fused = true;
+};
+
 struct Global {
+  Deferred *defer = nullptr;   // set while pdlhip_run_matmult_list plans its items
   int device = -1;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -521,6 +529,8 @@ static Dimset brc_dimset(const Brc &b, const int *which, int np) {
   return d;
 }
 
+static inline bool IsFloatCode(int T) { return T == PDLHIP_F || T == PDLHIP_D; }
+
 // ------------------------------------------------------------------ dispatch
 static hipError_t binop_dispatch(int T, int op, const EwPlan &p, const void *a, const void *b, void *c, bool bad,
                                  const BadInfo *bi, hipStream_t s) {
@@ -665,6 +675,26 @@ static pdlhip_error readdata(const Sig &s, int T, bool bad, const Brc &b, pdlhip
     }
     for (int q = 0; q < 3; q++) p.off[q] = b.offs[q];
     BadInfo bi[3] = {badinfo(pdls[0], 1), badinfo(pdls[1], 1), badinfo(pdls[2], 1)};
+    if (g.defer && !bad && (g.defer->T < 0 || g.defer->T == T)) {
+      // a list item: record one product per broadcast position, launch later
+      g.defer->T = T;
+      g.defer->fused = IsFloatCode(T) && g.mm_mode != PDLHIP_MM_PDLORDER;
+      for (int64_t bidx = 0; bidx < p.nbatch; bidx++) {
+        MmItem it;
+        it.a = pdls[0]->data; it.b = pdls[1]->data; it.c = pdls[2]->data;
+        it.H = p.H; it.W = p.W; it.T = p.T;
+        it.a_t = p.a_t; it.a_h = p.a_h; it.b_w = p.b_w; it.b_t = p.b_t; it.c_w = p.c_w; it.c_h = p.c_h;
+        it.oa = p.off[0]; it.ob = p.off[1]; it.oc = p.off[2];
+        int64_t r = bidx;
+        for (int d = 0; d < p.nd; d++) {
+          int64_t i = r % p.dims[d];
+          r /= p.dims[d];
+          it.oa += i * p.inc[0][d]; it.ob += i * p.inc[1][d]; it.oc += i * p.inc[2][d];
+        }
+        g.defer->items.push_back(it);
+      }
+      return ok();
+    }
     e = matmult_dispatch(T, p, pdls[0]->data, pdls[1]->data, pdls[2]->data, bad, bi, g.mm_mode, st);
   }
   if (e != hipSuccess) return hip_err(e, s.name.c_str());
@@ -830,6 +860,100 @@ static pdlhip_error run(const Sig &s, pdlhip_pdl **user, int npdls, const int64_
   return err;
 }
 
+// ------------------------------------------------------------------ autotuner
+// Tile shape of the list kernel per shape class: (type, fused, log2 of the
+// largest dim, log2 of the item count).  On a miss every candidate runs once
+// under events and the fastest is kept for the process.
+struct TuneKey {
+  int T, fused, lgdim, lgcount;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(T, fused, lgdim, lgcount) < std::tie(o.T, o.fused, o.lgdim, o.lgcount);
+  }
+};
+static std::map<TuneKey, int> g_tune;
+
+static int ilog2(int64_t v) {
+  int r = 0;
+  while (v > 1) { v >>= 1; r++; }
+  return r;
+}
+
+static pdlhip_error run_list(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlhip_pdl **c, int mode) {
+  const Sig *s = find_sig("matmult");
+  Deferred d;
+  g.defer = &d;
+  pdlhip_error err = ok();
+  for (int i = 0; i < n && !err.error; i++) {
+    pdlhip_pdl *v[3] = {a[i], b[i], c[i]};
+    err = run(*s, v, 3, nullptr);
+  }
+  g.defer = nullptr;
+  if (err.error || d.items.empty() || g.plan_only) return err;
+  const int nit = (int)d.items.size();
+  int64_t maxdim = 1;
+  for (auto &it : d.items) maxdim = std::max(maxdim, std::max(it.H, it.W));
+  // work items, largest inner dimension first
+  std::vector<int> order(nit);
+  for (int i = 0; i < nit; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return d.items[x].T > d.items[y].T; });
+  MmItem *d_items = (MmItem *)dev_alloc(sizeof(MmItem) * nit);
+  unsigned int *d_counter = (unsigned int *)dev_alloc(sizeof(unsigned int));
+  if (!d_items || !d_counter) return make_err(PDLHIP_EFATAL, "pdlhip: out of device memory");
+  hipError_t e = hipMemcpyAsync(d_items, d.items.data(), sizeof(MmItem) * nit, hipMemcpyHostToDevice, g.stream);
+  auto build_work = [&](int tile, std::vector<int3> &work) {
+    work.clear();
+    for (int i : order) {
+      const MmItem &it = d.items[i];
+      int64_t ntm = (it.H + tile - 1) / tile, ntn = (it.W + tile - 1) / tile;
+      for (int64_t tm = 0; tm < ntm; tm++)
+        for (int64_t tn = 0; tn < ntn; tn++) work.push_back(make_int3(i, (int)tm, (int)tn));
+    }
+  };
+  std::vector<int3> work32, work64;
+  build_work(32, work32);
+  build_work(64, work64);
+  int3 *d_w32 = (int3 *)dev_alloc(sizeof(int3) * std::max<size_t>(work32.size(), 1));
+  int3 *d_w64 = (int3 *)dev_alloc(sizeof(int3) * std::max<size_t>(work64.size(), 1));
+  if (!d_w32 || !d_w64) return make_err(PDLHIP_EFATAL, "pdlhip: out of device memory");
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_w32, work32.data(), sizeof(int3) * work32.size(), hipMemcpyHostToDevice, g.stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_w64, work64.data(), sizeof(int3) * work64.size(), hipMemcpyHostToDevice, g.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g.stream);   // host vectors may go out of scope after this
+  const bool each = (mode == PDLHIP_LIST_EACH);
+  auto launch = [&](int tile) {
+    const std::vector<int3> &w = tile == 32 ? work32 : work64;
+    int nblocks = (int)std::min<size_t>(w.size(), 2048);
+    return launch_matmult_list(d.T, d_items, d.items.data(), nit, tile == 32 ? d_w32 : d_w64, (unsigned)w.size(),
+                               d_counter, tile, d.fused, each, std::max(nblocks, 1), g.stream);
+  };
+  TuneKey key{d.T, d.fused ? 1 : 0, ilog2(maxdim), ilog2(nit)};
+  int tile = 64;
+  auto it = g_tune.find(key);
+  if (it != g_tune.end()) {
+    tile = it->second;
+  } else if (e == hipSuccess && !each) {
+    float best = 1e30f;
+    for (int cand : {32, 64}) {
+      (void)hipEventRecord(g.ev0, g.stream);
+      e = launch(cand);
+      (void)hipEventRecord(g.ev1, g.stream);
+      if (e != hipSuccess) break;
+      float ms = 0;
+      (void)hipEventSynchronize(g.ev1);
+      (void)hipEventElapsedTime(&ms, g.ev0, g.ev1);
+      if (ms < best) { best = ms; tile = cand; }
+    }
+    g_tune[key] = tile;
+  }
+  if (e == hipSuccess) e = launch(tile);
+  dev_free(d_items);
+  dev_free(d_counter);
+  dev_free(d_w32);
+  dev_free(d_w64);
+  return e == hipSuccess ? ok() : hip_err(e, "matmult list");
+}
+
 }  // namespace pdlhip
 
 // ==================================================================== C ABI
@@ -986,6 +1110,13 @@ pdlhip_error pdlhip_run_maximum_n_ind(pdlhip_pdl *a, pdlhip_pdl *c, int64_t m_si
 pdlhip_error pdlhip_run_matmult(pdlhip_pdl *a, pdlhip_pdl *b, pdlhip_pdl *c) {
   pdlhip_pdl *v[3] = {a, b, c};
   return pdlhip_run("matmult", v, 3, nullptr);
+}
+
+pdlhip_error pdlhip_run_matmult_list(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlhip_pdl **c, int mode) {
+  if (n < 0 || (n > 0 && (!a || !b || !c))) return make_err(PDLHIP_EUSERERROR, "pdlhip_run_matmult_list: bad arguments");
+  if (mode != PDLHIP_LIST_QUEUE && mode != PDLHIP_LIST_EACH)
+    return make_err(PDLHIP_EUSERERROR, "pdlhip_run_matmult_list: unknown mode %d", mode);
+  return run_list(n, a, b, c, mode);
 }
 
 pdlhip_error pdlhip_set_matmult_precision(int mode) {

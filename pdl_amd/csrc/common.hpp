@@ -73,6 +73,16 @@ struct MmPlan {
   int64_t off[3];
 };
 
+// one product of a matmult list (pdlhip_run_matmult_list): resolved
+// pointers, sizes, core strides and element offsets
+struct MmItem {
+  const void *a, *b;
+  void *c;
+  int64_t H, W, T;
+  int64_t a_t, a_h, b_w, b_t, c_w, c_h;
+  int64_t oa, ob, oc;
+};
+
 // type-code -> C type
 template <int TC> struct CType;
 template <> struct CType<PDLHIP_SB> { typedef int8_t T; };

@@ -29,6 +29,12 @@ PDLHIP_FOR_EACH_TYPE(X)
 PDLHIP_FOR_EACH_TYPE(X)
 #undef X
 
+// a list of products in one persistent work-queue launch, or one launch per
+// product (each = true) (k_batch.hip)
+hipError_t launch_matmult_list(int T, const MmItem *d_items, const MmItem *h_items, int nitems, const int3 *d_work,
+                               unsigned int nwork, unsigned int *d_counter, int tile, bool fused, bool each,
+                               int nblocks, hipStream_t s);
+
 // conversion / assignment between views (k_misc.hip): dst = (dtype) src with
 // PDL's bad-value mapping (pdlconv.c:163-174); also assgn (Ops.pd:382-397)
 hipError_t launch_convert(int stype, int dtype, const EwPlan &p, const void *src, void *dst, bool bad,
