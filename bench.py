@@ -202,6 +202,56 @@ def run_matmult(n=8192, nrows_check=16, reps=3):
     return res
 
 
+def run_ragged_matmult(count=4096, lo=8, hi=256, seed=0):
+    """4096 independent square products, n ~ U{8..256}, float32 and float64
+    entries uniform[-1,1]: one persistent work-queue launch vs one launch per
+    product; plus a broadcast (64,64)x(64,64) matmult over 65536."""
+    from pdl_amd import core as C
+
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(lo, hi, size=count, endpoint=True)
+    flops = float(np.sum(2.0 * ns.astype(np.float64) ** 3))
+    out = {"count": count, "n_range": [lo, hi], "flop": flops}
+    tot = int(np.sum(ns.astype(np.int64) ** 2))
+    for t, name in ((C.L.F, "f32"), (C.L.D, "f64")):
+        A = C.random_uniform(tot, dtype=t, seed=41, lo=-1, hi=1)
+        B = C.random_uniform(tot, dtype=t, seed=42, lo=-1, hi=1)
+        Cb = C.ndarray.new_dense(t, [tot])
+        As, Bs, Cs, off = [], [], [], 0
+        for n in ns:
+            n = int(n)
+            As.append(C.view_into(A, [n, n], off)); Bs.append(C.view_into(B, [n, n], off))
+            Cs.append(C.view_into(Cb, [n, n], off))
+            off += n * n
+        res = {}
+        for mode in ("queue", "each"):
+            C.matmult_list(As, Bs, Cs, mode=mode)          # warm (and autotune)
+            C.synchronize()
+            e0, e1 = C.Event(), C.Event()
+            e0.record()
+            C.matmult_list(As, Bs, Cs, mode=mode)
+            e1.record()
+            ms = e0.elapsed_ms(e1)
+            res[mode] = {"ms": round(ms, 3), "TFLOP/s": round(flops / (ms * 1e-3) / 1e12, 2)}
+        res["queue_vs_each"] = round(res["each"]["ms"] / res["queue"]["ms"], 2)
+        out[name] = res
+    # broadcast matmult (64,64) x (64,64) over a broadcast dim of 65536
+    a = C.random_uniform(64, 64, 65536, dtype=C.L.F, seed=51, lo=-1, hi=1)
+    b = C.random_uniform(64, 64, 65536, dtype=C.L.F, seed=52, lo=-1, hi=1)
+    c = C.ndarray.new_dense(C.L.F, [64, 64, 65536])
+    C._run("matmult", [a, b, c])
+    C.synchronize()
+    e0, e1 = C.Event(), C.Event()
+    e0.record()
+    C._run("matmult", [a, b, c])
+    e1.record()
+    ms = e0.elapsed_ms(e1)
+    nbytes = 3 * 64 * 64 * 65536 * 4
+    out["broadcast_64x64x65536_f32"] = {"ms": round(ms, 3), "TFLOP/s": round(2 * 64 ** 3 * 65536 / (ms * 1e-3) / 1e12, 2),
+                                        "GB/s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    return out
+
+
 def run_single(args):
     """one op alone, for counter runs of its kernel"""
     from pdl_amd import core as C
@@ -289,6 +339,7 @@ def main():
     }
     if not args.no_matmult:
         out["matmult"] = run_matmult()
+        out["ragged_matmult"] = run_ragged_matmult()
     if not args.no_cpu:
         out["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(out))

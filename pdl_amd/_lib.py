@@ -97,6 +97,8 @@ EXPORTS["pdlhip_run_minmaximum"] = ([PP] * 5, Error)
 EXPORTS["pdlhip_run_minimum_n_ind"] = ([PP, PP, ctypes.c_int64], Error)
 EXPORTS["pdlhip_run_maximum_n_ind"] = ([PP, PP, ctypes.c_int64], Error)
 EXPORTS["pdlhip_run_matmult"] = ([PP, PP, PP], Error)
+EXPORTS["pdlhip_run_matmult_list"] = ([ctypes.c_int, ctypes.POINTER(PP), ctypes.POINTER(PP), ctypes.POINTER(PP),
+                                      ctypes.c_int], Error)
 
 _lib = None
 

@@ -594,6 +594,33 @@ def matmult(x, y, c=None):
     return c
 
 
+def matmult_list(As, Bs, Cs=None, mode="queue"):
+    """[A_i x B_i] for a list of uneven products in one persistent
+    work-queue launch (mode "queue") or one launch per product ("each")."""
+    n = len(As)
+    if len(Bs) != n:
+        raise PDLError(L.EUSERERROR, "matmult_list: %d left and %d right operands" % (n, len(Bs)))
+    Cs = [ndarray() for _ in range(n)] if Cs is None else list(Cs)
+    _need()
+    ca = [topdl(a)._to_c() for a in As]
+    cb = [topdl(b)._to_c() for b in Bs]
+    cc = [c._to_c() for c in Cs]
+    created = [c.isnull() for c in Cs]
+    pa = (L.PP * n)(*[ctypes.pointer(x) for x in ca])
+    pb = (L.PP * n)(*[ctypes.pointer(x) for x in cb])
+    pc = (L.PP * n)(*[ctypes.pointer(x) for x in cc])
+    L.check(L.lib().pdlhip_run_matmult_list(n, pa, pb, pc, {"queue": 0, "each": 1}[mode]))
+    for c, s, cr in zip(Cs, cc, created):
+        c._from_c(s, cr)
+    return Cs
+
+
+def view_into(parent, dims, offs):
+    """a dense-strided view of `dims` starting at element `offs` of parent's
+    root buffer (for packing many small ndarrays into one allocation)"""
+    return parent._view(list(dims), _dense_incs(dims), offs)
+
+
 def set_matmult_precision(mode):
     """'split16' (default), 'fp32', 'pdlorder' or 'fp8' for float matmult"""
     m = {"split16": L.MM_SPLIT16, "fp32": L.MM_FP32, "pdlorder": L.MM_PDLORDER, "fp8": L.MM_FP8}[mode]

@@ -849,7 +849,14 @@ static pdlhip_error run(const Sig &s, pdlhip_pdl **user, int npdls, const int64_
       pdls[i]->state &= ~PDLHIP_NOMYDIMS;
     }
   }
+  // a list item whose operands needed converted copies runs now: its
+  // temporaries are released below, before a deferred launch would read them
+  Deferred *defer_saved = g.defer;
+  bool any_conv = false;
+  for (int i = 0; i < npdls; i++) any_conv = any_conv || converted[i];
+  if (any_conv) g.defer = nullptr;
   if (!err.error) err = readdata(s, T, inputs_bad, b, pdls, other);
+  g.defer = defer_saved;
   for (int i = s.nparents; i < npdls && !err.error; i++) {
     if (!converted[i]) continue;
     err = write_back(&conv[i], orig[i]);

@@ -123,6 +123,18 @@ template <int OP, class T> __device__ inline T binop(T a, T b) {
 // VEC elements loaded/stored as one 16-byte (or narrower) access
 template <class T, int VEC> struct alignas(sizeof(T) * VEC) Vec { T v[VEC]; };
 
+// 16-byte store that bypasses cache residency (streaming output)
+template <class T, int VEC> __device__ inline void store_nt(T *p, const Vec<T, VEC> &v) {
+  if constexpr (sizeof(T) * VEC == 16) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 w;
+    __builtin_memcpy(&w, &v, 16);
+    __builtin_nontemporal_store(w, reinterpret_cast<u32x4 *>(p));
+  } else {
+    *reinterpret_cast<Vec<T, VEC> *>(p) = v;
+  }
+}
+
 // offsets of element (i0, row) where `row` enumerates dims 1..nd-1
 struct RowOffsets {
   int64_t o[3];

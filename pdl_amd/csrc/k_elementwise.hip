@@ -42,7 +42,7 @@ __global__ __launch_bounds__(EW_THREADS) void ew_flat(const T *__restrict__ a, c
     }
 #pragma unroll
     for (int k = 0; k < VEC; k++) cv.v[k] = binop<OP, T>(av.v[k], bv.v[k]);
-    reinterpret_cast<Vec<T, VEC> *>(c)[v] = cv;
+    store_nt<T, VEC>(c + v * VEC, cv);
   }
   // tail
   int64_t t = nvec * VEC + (int64_t)blockIdx.x * EW_THREADS + threadIdx.x;
@@ -109,9 +109,14 @@ __global__ __launch_bounds__(EW_THREADS) void ew_elems(const T *__restrict__ a, 
 // input is contiguous).  TMASK bit k: input k is read through LDS.  For
 // 4-byte types each thread moves 16-byte vectors on both sides: the
 // transposed input is read 4-wide along j, the output written 4-wide along i.
-constexpr int TI = 64, TJ = 64;
+// 4-byte types use 32 (i) x 128 (j) tiles: 512-byte runs on the transposed
+// read, 128-byte runs on the streaming write; others 64 x 64.
+template <class T> struct TileShape { static constexpr int I = 64, J = 64; };
+template <> struct TileShape<float> { static constexpr int I = 32, J = 128; };
+template <> struct TileShape<int32_t> { static constexpr int I = 32, J = 128; };
+template <> struct TileShape<uint32_t> { static constexpr int I = 32, J = 128; };
 
-template <class T, int OP, int TMASK, int V>
+template <class T, int OP, int TMASK, int V, int TI = TileShape<T>::I, int TJ = TileShape<T>::J>
 __global__ __launch_bounds__(EW_THREADS) void ew_tile(const T *__restrict__ a, const T *__restrict__ b,
                                                      T *__restrict__ c, EwPlan p, int q, int64_t ntile_i,
                                                      int64_t ntile_j) {
@@ -204,7 +209,7 @@ __global__ __launch_bounds__(EW_THREADS) void ew_tile(const T *__restrict__ a, c
       Vec<T, V> out;
 #pragma unroll
       for (int e = 0; e < V; e++) out.v[e] = binop<OP, T>(x[e], y[e]);
-      *reinterpret_cast<Vec<T, V> *>(c + o[2] + gi + gj * p.inc[2][q]) = out;
+      store_nt<T, V>(c + o[2] + gi + gj * p.inc[2][q], out);
     } else {
 #pragma unroll
       for (int e = 0; e < V; e++) {
@@ -263,6 +268,7 @@ static hipError_t binop_T_OP(const EwPlan &p, const void *va, const void *vb, vo
           if (p.inc[k][d] == 1 && (q < 0 || q == d)) { q = d; tmask |= 1 << k; }
       }
       if (q > 0 && tmask) {
+        constexpr int TI = TileShape<T>::I, TJ = TileShape<T>::J;
         int64_t nti = (p.dims[0] + TI - 1) / TI, ntj = (p.dims[q] + TJ - 1) / TJ;
         int64_t nrest = p.n / (p.dims[0] * p.dims[q]);
         int64_t nblk = nti * ntj * nrest;

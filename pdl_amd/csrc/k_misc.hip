@@ -33,10 +33,14 @@ __global__ __launch_bounds__(256) void k_convert(const S *__restrict__ src, D *_
   }
 }
 
-// contiguous same-type copy (the common assgn of dense data)
-template <class T>
+// contiguous same-type copy (the common assgn of dense data); 16-byte
+// vectors with streaming stores when both ends are 16-byte aligned
+template <class T, int VEC>
 __global__ __launch_bounds__(256) void k_copy_flat(const T *__restrict__ src, T *__restrict__ dst, int64_t n) {
-  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) dst[e] = src[e];
+  const int64_t nv = n / VEC, stride = (int64_t)gridDim.x * 256;
+  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < nv; v += stride)
+    store_nt<T, VEC>(dst + v * VEC, reinterpret_cast<const Vec<T, VEC> *>(src)[v]);
+  for (int64_t e = nv * VEC + (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += stride) dst[e] = src[e];
 }
 
 template <class D, class S>
@@ -48,8 +52,16 @@ static hipError_t convert_DS(const EwPlan &p, const void *src, void *dst, bool b
   if (blocks < 1) blocks = 1;
   if (!bad && sizeof(D) == sizeof(S) && IsFloat<D>::v == IsFloat<S>::v && (D)-1 == (S)-1 && p.nd == 1 &&
       p.inc[0][0] == 1 && p.inc[2][0] == 1) {
-    hipLaunchKernelGGL((k_copy_flat<D>), dim3((unsigned)blocks), dim3(256), 0, s, (const D *)src + p.off[0],
-                       (D *)dst + p.off[2], p.n);
+    const D *sp = (const D *)src + p.off[0];
+    D *dp = (D *)dst + p.off[2];
+    constexpr int VEC = 16 / sizeof(D);
+    if ((uintptr_t)sp % 16 == 0 && (uintptr_t)dp % 16 == 0) {
+      int64_t vb = (p.n / VEC + 255) / 256;
+      vb = vb < 1 ? 1 : (vb > 65536 ? 65536 : vb);
+      hipLaunchKernelGGL((k_copy_flat<D, VEC>), dim3((unsigned)vb), dim3(256), 0, s, sp, dp, p.n);
+    } else {
+      hipLaunchKernelGGL((k_copy_flat<D, 1>), dim3((unsigned)blocks), dim3(256), 0, s, sp, dp, p.n);
+    }
     return hipGetLastError();
   }
   if (bad)

@@ -262,7 +262,22 @@ __device__ inline void row_segment(Part<Tin, Tacc> &pt, const Tin *__restrict__ 
   const Tin abv = bv_as<Tin>(ba.bv);
   int64_t i = begin + (int64_t)lane * VEC;
   if constexpr (VEC > 1) {
-    // begin is a multiple of VEC and the row start is 16-byte aligned
+    // begin is a multiple of VEC and the row start is 16-byte aligned; two
+    // vectors in flight per lane, consumed in increasing index order
+    for (; i + 64 * VEC + VEC <= end; i += 128 * VEC) {
+      Vec<Tin, VEC> x0 = *reinterpret_cast<const Vec<Tin, VEC> *>(a + o + i);
+      Vec<Tin, VEC> x1 = *reinterpret_cast<const Vec<Tin, VEC> *>(a + o + i + 64 * VEC);
+#pragma unroll
+      for (int k = 0; k < VEC; k++) {
+        if constexpr (BAD) { if (isbad(x0.v[k], abv, ba.isnan)) continue; }
+        part_push<OP, Tin, Tacc>(pt, x0.v[k], i + k);
+      }
+#pragma unroll
+      for (int k = 0; k < VEC; k++) {
+        if constexpr (BAD) { if (isbad(x1.v[k], abv, ba.isnan)) continue; }
+        part_push<OP, Tin, Tacc>(pt, x1.v[k], i + 64 * VEC + k);
+      }
+    }
     for (; i + VEC <= end; i += 64 * VEC) {
       Vec<Tin, VEC> x = *reinterpret_cast<const Vec<Tin, VEC> *>(a + o + i);
 #pragma unroll

@@ -363,6 +363,29 @@ def test_matmult_bad_values_exact():
         same(o, d, "matmult bad %s" % NAMES[t])
 
 
+@pytest.mark.parametrize("t", [O.F, O.D, O.L], ids=["float", "double", "long"])
+def test_matmult_list_matches_single_products(t):
+    """one work-queue launch over uneven products == each product alone (PDL order: bit-exact vs the oracle)"""
+    rng = np.random.default_rng(12)
+    C.set_matmult_precision("pdlorder")
+    try:
+        sizes = [(int(rng.integers(1, 90)), int(rng.integers(1, 90)), int(rng.integers(1, 90))) for _ in range(40)]
+        As, Bs, oas, obs = [], [], [], []
+        for h, tt, w in sizes:
+            a = rand(t, (h, tt), rng, -3, 3)
+            b = rand(t, (tt, w), rng, -3, 3)
+            As.append(C.ndarray.from_numpy(a, t)); Bs.append(C.ndarray.from_numpy(b, t))
+            oas.append(O.OPdl.from_numpy(a, t)); obs.append(O.OPdl.from_numpy(b, t))
+        for mode in ("queue", "each"):
+            outs = C.matmult_list(As, Bs, mode=mode)
+            for k in range(len(sizes)):
+                oc = O.OPdl.null()
+                O.run("matmult", oas[k], obs[k], oc)
+                same(oc, outs[k], "list %s item %d %s" % (mode, k, sizes[k]))
+    finally:
+        C.set_matmult_precision("split16")
+
+
 # ------------------------------------------------------------------ readdata entry
 def test_readdata_entry_runs_a_planned_broadcast():
     """pdlhip_readdata over a broadcast struct built as PDL's redodims would"""
