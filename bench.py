@@ -252,11 +252,127 @@ def run_ragged_matmult(count=4096, lo=8, hi=256, seed=0):
     return out
 
 
+RAGGED_CHAIN = [("minus", 0, 1), ("power", ("node", 0), 2), ("mult", ("node", 1), 3), ("mult", ("node", 2), 4)]
+
+
+def run_ragged_chain(nvec=100000, reps=100, loop_sample=1000, seed=0):
+    """100000 float32 vectors, lengths lognormal(8,1.5) clipped to [1,2^20]:
+    per vector s = sumover((a-b)**2 * w * m) with a byte mask m, and the
+    int64 cumusumover of an int64 vector of the same length.  The fused chain
+    and the scan are captured in one HIP graph and replayed `reps` times
+    (p50/p99 per replay from HIP events between replays); compared with
+    per-op launches over the packed list (4 binops writing temporaries, then
+    the ragged sumover and scan) and with PDL's loop of eager ops per vector
+    (timed on the first `loop_sample` vectors, scaled to the list)."""
+    from pdl_amd import core as C
+
+    F, D, B, LL = C.L.F, C.L.D, C.L.B, C.L.LL
+    rng = np.random.default_rng(seed)
+    lens = np.clip(np.round(rng.lognormal(8.0, 1.5, nvec)), 1, 2 ** 20).astype(np.int64)
+    tot = int(lens.sum())
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    A = C.random_normal(tot, dtype=F, seed=61)
+    Bv = C.random_normal(tot, dtype=F, seed=62)
+    W = C.random_uniform(tot, dtype=F, seed=63, lo=0.0, hi=1.0)
+    M = C.random_uniform(tot, dtype=B, seed=64, lo=0.0, hi=2.0)       # 0/1 mask
+    I = C.random_uniform(tot, dtype=LL, seed=65, lo=-2.0 ** 40, hi=2.0 ** 40)
+    two = C.topdl(2)
+    views = lambda big: [C.view_into(big, [int(n)], int(o)) for n, o in zip(lens, offs)]
+    va, vb, vw, vm, vi = views(A), views(Bv), views(W), views(M), views(I)
+    items = [[a, b, two, w, m] for a, b, w, m in zip(va, vb, vw, vm)]
+    t0 = time.perf_counter()
+    rs = C.Ragged(items, RAGGED_CHAIN, "sumover")
+    rc = C.Ragged([[x] for x in vi], [], "cumusumover")
+    prep_s = time.perf_counter() - t0
+
+    def both():
+        rs.run()
+        rc.run()
+
+    g = C.Graph(both)
+    for _ in range(3):
+        g.launch()
+    C.synchronize()
+    ev = [C.Event() for _ in range(reps + 1)]
+    ev[0].record()
+    for k in range(reps):
+        g.launch()
+        ev[k + 1].record()
+    C.synchronize()
+    per = np.array([ev[k].elapsed_ms(ev[k + 1]) for k in range(reps)])
+    total_ms = ev[0].elapsed_ms(ev[reps]) / reps
+    nbytes = tot * (4 + 4 + 4 + 1) + nvec * 4 + tot * 8 * 2
+    out = {"vectors": nvec, "elements": tot, "max_len": int(lens.max()), "bytes_per_replay": nbytes,
+           "replays": reps, "prepare_s": round(prep_s, 2),
+           "graph": {"ms_mean": round(total_ms, 3), "p50_ms": round(float(np.percentile(per, 50)), 3),
+                     "p99_ms": round(float(np.percentile(per, 99)), 3),
+                     "GB/s": round(nbytes / (total_ms * 1e-3) / 1e9, 1),
+                     "frac": round(nbytes / (total_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)}}
+    # checks: int64 scans bit-exact against numpy (wrapping adds) on sampled
+    # vectors incl. the longest; sums against a float64 sum of float32 chain values
+    check_idx = sorted(set([int(np.argmax(lens))] + [int(i) for i in rng.integers(0, nvec, 24)]))
+    scan_ok, worst = True, 0.0
+    for i in check_idx:
+        x = vi[i].numpy()
+        scan_ok &= bool(np.array_equal(np.cumsum(x, dtype=np.int64), rc.outputs[i].numpy()))
+        z = (va[i].numpy() - vb[i].numpy()) ** 2 * vw[i].numpy() * vm[i].numpy().astype(np.float32)
+        truth = float(np.sum(z.astype(np.float64)))
+        got = float(rs.outputs[i].numpy())
+        worst = max(worst, abs(got - truth) / max(abs(truth), 1e-300))
+    out["checks"] = {"i64_cumusumover_exact": scan_ok, "sum_max_rel_err": worst, "sampled": len(check_idx)}
+    # per-op launches over the packed list
+    z1, z2, z3, z4 = (C.ndarray.new_dense(F, [tot]) for _ in range(4))
+    ru = C.Ragged([[v] for v in views(z4)], [], "sumover")
+
+    def unfused():
+        C._run("minus", [A, Bv, z1])
+        C._run("power", [z1, two, z2])
+        C._run("mult", [z2, W, z3])
+        C._run("mult", [z3, M, z4])
+        ru.run()
+        rc.run()
+
+    unfused()
+    C.synchronize()
+    ev2 = [C.Event() for _ in range(6)]
+    ev2[0].record()
+    for k in range(5):
+        unfused()
+        ev2[k + 1].record()
+    C.synchronize()
+    uper = np.array([ev2[k].elapsed_ms(ev2[k + 1]) for k in range(5)])
+    out["per_op_launches"] = {"ms_mean": round(float(uper.mean()), 3), "p50_ms": round(float(np.median(uper)), 3),
+                              "graph_speedup": round(float(uper.mean()) / total_ms, 2)}
+    same = all(float(ru.outputs[i].numpy()) == float(rs.outputs[i].numpy()) or
+               abs(float(ru.outputs[i].numpy()) - float(rs.outputs[i].numpy())) <=
+               4 * 2.0 ** -24 * abs(float(rs.outputs[i].numpy())) for i in check_idx)
+    out["checks"]["fused_vs_per_op_within_4ulp"] = bool(same)
+    del z1, z2, z3, z4
+    # PDL's loop: eager ops per vector (temporaries allocated per op)
+    ns = min(loop_sample, nvec)
+    C.synchronize()
+    t0 = time.perf_counter()
+    for i in range(ns):
+        z = (va[i] - vb[i]) ** 2 * vw[i] * vm[i]
+        s = z.sumover()
+        c = vi[i].cumusumover()
+    C.synchronize()
+    loop_s = (time.perf_counter() - t0) * nvec / ns
+    out["per_vector_loop"] = {"sample_vectors": ns, "s_scaled_to_list": round(loop_s, 3),
+                              "graph_speedup": round(loop_s * 1e3 / total_ms, 1)}
+    g.destroy()
+    rs.free(); rc.free(); ru.free()
+    return out
+
+
 def run_single(args):
     """one op alone, for counter runs of its kernel"""
     from pdl_amd import core as C
 
     C.init(0)
+    if args.only == "ragged":
+        print(json.dumps(run_ragged_chain()))
+        return
     if args.only == "copy":
         src = C.random_uniform(256, 1024, 1024, dtype=C.L.F, seed=1)      # 1 GiB
         dst = C.ndarray.new_dense(C.L.F, src.dims)
@@ -277,10 +393,11 @@ def run_single(args):
     print(json.dumps({"only": args.only, "steps": args.steps}))
 
 
-def run_cpu_baseline(frac_k=2):
+def run_cpu_baseline(frac_k=2, min_seconds=10.0):
     """The oracle (the C restatement of PDL's CPU path) on a bounded sample of
     the same workload: x (4096,4096,frac_k) instead of 16 k-planes, the int
-    arrays at 1/4 of their rows; PDL's autopthread rule with every host core."""
+    arrays at 1/4 of their rows, repeated for at least `min_seconds`; PDL's
+    autopthread rule with every host core."""
     from oracle import pdlcpu as O
 
     nthr = os.cpu_count() or 1
@@ -294,17 +411,23 @@ def run_cpu_baseline(frac_k=2):
     p32 = O.OPdl.from_numpy(i32h, O.L)
     p16 = O.OPdl.from_numpy(i16h, O.S, badflag=True)
     nb = algorithmic_bytes(NX, NY, NK, 4096, 1024, 8192, 2048)
+    reps = 0
     t0 = time.perf_counter()
-    z = O.OPdl.null()
-    O.run("mult", px, py, z, nthreads=nthr)
-    for op in ("sumover", "average", "minimum", "maximum"):
-        O.run(op, z, O.OPdl.null(), nthreads=nthr)
-    O.run("sumover", p32, O.OPdl.null(), nthreads=nthr)
-    O.run("cumusumover", p32, O.OPdl.null(), nthreads=nthr)
-    O.run("sumover", p16, O.OPdl.null(), nthreads=nthr)
-    dt = time.perf_counter() - t0
-    return {"value": round(sum(nb.values()) / dt / 1e9, 2), "unit": "GB/s", "cores": nthr, "kind": "port",
-            "sample": "same chain at x (4096,4096,%d), int32 (4096,1024), int16 (8192,2048); %.2f s" % (NK, dt)}
+    while True:
+        z = O.OPdl.null()
+        O.run("mult", px, py, z, nthreads=nthr)
+        for op in ("sumover", "average", "minimum", "maximum"):
+            O.run(op, z, O.OPdl.null(), nthreads=nthr)
+        O.run("sumover", p32, O.OPdl.null(), nthreads=nthr)
+        O.run("cumusumover", p32, O.OPdl.null(), nthreads=nthr)
+        O.run("sumover", p16, O.OPdl.null(), nthreads=nthr)
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt >= min_seconds:
+            break
+    return {"value": round(reps * sum(nb.values()) / dt / 1e9, 2), "unit": "GB/s", "cores": nthr, "kind": "port",
+            "sample": "the same chain at x (4096,4096,%d), int32 (4096,1024), int16 (8192,2048), repeated %d times "
+                      "in %.1f s" % (NK, reps, dt)}
 
 
 def main():
@@ -314,6 +437,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-matmult", action="store_true")
+    ap.add_argument("--no-ragged", action="store_true")
     ap.add_argument("--only", default=None, help="run one op alone (binop, sumover, copy ...) for counters")
     args = ap.parse_args()
     if args.only:
@@ -340,6 +464,8 @@ def main():
     if not args.no_matmult:
         out["matmult"] = run_matmult()
         out["ragged_matmult"] = run_ragged_matmult()
+    if not args.no_ragged:
+        out["ragged_chain"] = run_ragged_chain()
     if not args.no_cpu:
         out["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(out))

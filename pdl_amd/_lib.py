@@ -47,6 +47,11 @@ class Broadcast(ctypes.Structure):
                 ("inc_sizes", ctypes.c_int64 * 12)]
 
 
+class Node(ctypes.Structure):
+    """pdlhip_node: one binop of a fused ragged chain"""
+    _fields_ = [("op", ctypes.c_int32), ("x", ctypes.c_int32), ("y", ctypes.c_int32)]
+
+
 class PDLError(RuntimeError):
     """an error returned through the C ABI (PDL's barf)"""
 
@@ -97,6 +102,16 @@ EXPORTS["pdlhip_run_minmaximum"] = ([PP] * 5, Error)
 EXPORTS["pdlhip_run_minimum_n_ind"] = ([PP, PP, ctypes.c_int64], Error)
 EXPORTS["pdlhip_run_maximum_n_ind"] = ([PP, PP, ctypes.c_int64], Error)
 EXPORTS["pdlhip_run_matmult"] = ([PP, PP, PP], Error)
+EXPORTS["pdlhip_ragged_prepare"] = ([ctypes.c_int, ctypes.c_int, ctypes.POINTER(PP), ctypes.c_int,
+                                     ctypes.POINTER(Node), ctypes.c_char_p, ctypes.POINTER(PP),
+                                     ctypes.POINTER(ctypes.c_void_p)], Error)
+EXPORTS["pdlhip_ragged_run"] = ([ctypes.c_void_p], Error)
+EXPORTS["pdlhip_ragged_badflags"] = ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)], Error)
+EXPORTS["pdlhip_ragged_free"] = ([ctypes.c_void_p], Error)
+EXPORTS["pdlhip_graph_begin"] = ([], Error)
+EXPORTS["pdlhip_graph_end"] = ([ctypes.POINTER(ctypes.c_void_p)], Error)
+EXPORTS["pdlhip_graph_launch"] = ([ctypes.c_void_p], Error)
+EXPORTS["pdlhip_graph_destroy"] = ([ctypes.c_void_p], Error)
 EXPORTS["pdlhip_run_matmult_list"] = ([ctypes.c_int, ctypes.POINTER(PP), ctypes.POINTER(PP), ctypes.POINTER(PP),
                                       ctypes.c_int], Error)
 

@@ -37,6 +37,7 @@ def units():
     u.append(("k_misc.hip", "misc", []))
     u.append(("k_mfma.hip", "mfma", []))
     u.append(("k_batch.hip", "batch", []))
+    u.append(("k_ragged.hip", "ragged", []))
     u.append(("api.cpp", "api", ["-x", "hip"]))
     return u
 

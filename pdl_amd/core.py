@@ -615,6 +615,102 @@ def matmult_list(As, Bs, Cs=None, mode="queue"):
     return Cs
 
 
+class Ragged:
+    """A prepared list of 1-D ndarrays of uneven lengths, evaluated in a fixed
+    set of launches: per element a chain of binops over the input slots, then
+    `final` (sumover, dsumover, average, daverage, minimum, maximum,
+    cumusumover, cumuprodover) per item.
+
+    items:  list of tuples, one ndarray (1-D, or a 0-dim scalar slot) per slot
+    chain:  list of (op, x, y); op is an Ops.pd binop name, an operand is a
+            slot index k >= 0 or ("node", j) for the result of node j
+    final:  the Ufunc op applied to the chain's value
+    Equivalent to the per-ndarray loop
+        for a, b, ... in items: out_i = (chain evaluated eagerly).final()
+    with the same rounding at every step."""
+
+    def __init__(self, items, chain, final, outputs=None):
+        _need()
+        n = len(items)
+        nin = len(items[0]) if n else 1
+        if any(len(t) != nin for t in items):
+            raise PDLError(L.EUSERERROR, "Ragged: items have different numbers of inputs")
+        self.outputs = [ndarray() for _ in range(n)] if outputs is None else list(outputs)
+        cin = [topdl(x)._to_c() for t in items for x in t]
+        cout = [o._to_c() for o in self.outputs]
+        created = [o.isnull() for o in self.outputs]
+        pin = (L.PP * max(1, len(cin)))(*[ctypes.pointer(x) for x in cin])
+        pout = (L.PP * max(1, n))(*[ctypes.pointer(x) for x in cout])
+
+        def operand(v):
+            return -1 - v[1] if isinstance(v, tuple) else v
+        nodes = (L.Node * max(1, len(chain)))(*[L.Node(L.BIOPS.index(op), operand(x), operand(y))
+                                                for op, x, y in chain])
+        h = ctypes.c_void_p()
+        L.check(L.lib().pdlhip_ragged_prepare(n, nin, pin, len(chain), nodes, final.encode(), pout,
+                                              ctypes.byref(h)))
+        self._h = h
+        self._keep = [topdl(x) for t in items for x in t]
+        for o, s, cr in zip(self.outputs, cout, created):
+            o._from_c(s, cr)
+
+    def run(self):
+        L.check(L.lib().pdlhip_ragged_run(self._h))
+        return self.outputs
+
+    def badflags(self):
+        """after a run: per item, whether its output is BAD (minimum/maximum
+        of an item with no good value); sets those outputs' bad-value flag"""
+        n = len(self.outputs)
+        f = (ctypes.c_int32 * max(1, n))()
+        L.check(L.lib().pdlhip_ragged_badflags(self._h, f))
+        for o, b in zip(self.outputs, f):
+            if b:
+                o.state |= L.BADVAL
+        return [bool(f[i]) for i in range(n)]
+
+    def free(self):
+        if self._h:
+            L.check(L.lib().pdlhip_ragged_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A HIP graph of the work `fn` enqueues on the library's stream (only
+    launches that allocate nothing, such as Ragged.run)."""
+
+    def __init__(self, fn):
+        _need()
+        L.check(L.lib().pdlhip_graph_begin())
+        try:
+            fn()
+        finally:
+            g = ctypes.c_void_p()
+            e = L.lib().pdlhip_graph_end(ctypes.byref(g))
+        L.check(e)
+        self._g = g
+
+    def launch(self):
+        L.check(L.lib().pdlhip_graph_launch(self._g))
+
+    def destroy(self):
+        if self._g:
+            L.check(L.lib().pdlhip_graph_destroy(self._g))
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 def view_into(parent, dims, offs):
     """a dense-strided view of `dims` starting at element `offs` of parent's
     root buffer (for packing many small ndarrays into one allocation)"""

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <queue>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -961,6 +962,239 @@ static pdlhip_error run_list(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlhip_pdl *
   return e == hipSuccess ? ok() : hip_err(e, "matmult list");
 }
 
+// ------------------------------------------------------------------ ragged lists
+struct RaggedH {
+  bool scan = false;
+  int T = -1, op = -1, n = 0;
+  RagLaunch L;
+  void *d_items = nullptr, *d_segs = nullptr, *d_multi = nullptr, *d_order = nullptr, *d_bstart = nullptr,
+       *d_scratch = nullptr, *d_empty = nullptr;
+};
+
+static int op_code(const char *n) {
+  static const char *names[] = {"plus", "mult", "minus", "divide", "gt", "lt", "le", "ge", "eq", "ne", "shiftleft",
+                                "shiftright", "or2", "and2", "xor", "power", "atan2", "modulo", "spaceship"};
+  for (int i = 0; i < OP_NBINOPS; i++) if (!strcmp(names[i], n)) return i;
+  return -1;
+}
+static const char *op_name(int op) {
+  static const char *names[] = {"plus", "mult", "minus", "divide", "gt", "lt", "le", "ge", "eq", "ne", "shiftleft",
+                                "shiftright", "or2", "and2", "xor", "power", "atan2", "modulo", "spaceship"};
+  return (op >= 0 && op < OP_NBINOPS) ? names[op] : "?";
+}
+
+// PDL's trans type of one binop on operands of types ta, tb
+static int binop_type(int op, int ta, int tb) {
+  const Sig *s = find_sig(op_name(op));
+  pdlhip_pdl x, y, z;
+  std::memset(&x, 0, sizeof x); std::memset(&y, 0, sizeof y); std::memset(&z, 0, sizeof z);
+  x.datatype = ta; y.datatype = tb; z.state = PDLHIP_NOMYDIMS; z.datatype = -1;
+  pdlhip_pdl *v[3] = {&x, &y, &z};
+  return select_transtype(*s, v);
+}
+
+static pdlhip_error ragged_prepare(int n, int nin, pdlhip_pdl **in, int nnodes, const pdlhip_node *nodes,
+                                   const char *final_op, pdlhip_pdl **out, RaggedH **hout) {
+  if (n < 0 || nin < 1 || nin > RAG_MAXIN || nnodes < 0 || nnodes > RAG_MAXNODE || !final_op || (n && (!in || !out)))
+    return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: bad arguments");
+  struct { const char *name; int op; bool scan; } fins[] = {
+      {"sumover", R_SUM, false}, {"dsumover", R_DSUM, false}, {"average", R_AVERAGE, false},
+      {"daverage", R_DAVERAGE, false}, {"minimum", R_MIN, false}, {"maximum", R_MAX, false},
+      {"cumusumover", R_CUMUSUM, true}, {"cumuprodover", R_CUMUPROD, true}};
+  int fop = -1;
+  bool scan = false;
+  for (auto &f : fins) if (!strcmp(f.name, final_op)) { fop = f.op; scan = f.scan; }
+  if (fop < 0) return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: unsupported final op '%s'", final_op);
+  if (scan && (nnodes || nin != 1))
+    return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: %s takes one input and no chain", final_op);
+  RaggedH h;
+  h.scan = scan;
+  h.op = fop;
+  h.n = n;
+  std::memset(&h.L, 0, sizeof h.L);
+  RagProg &pg = h.L.prog;
+  pg.nin = nin;
+  pg.nnodes = nnodes;
+  // slot types and shapes, uniform over the items
+  std::vector<int> stype(nin, -1), scalar(nin, 0);
+  for (int k = 0; k < nin; k++) {
+    for (int i = 0; i < n; i++) {
+      const pdlhip_pdl *p = in[i * nin + k];
+      if (!p || (p->state & PDLHIP_NOMYDIMS))
+        return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: item %d input %d is null", i, k);
+      if (p->state & PDLHIP_BADVAL)
+        return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: item %d input %d has bad values; run the ops "
+                        "per ndarray", i, k);
+      if (!type_on_device(p->datatype))
+        return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: type %s not supported", type_name(p->datatype));
+      int sc = p->ndims == 0 ? 1 : 0;
+      if (i == 0) { stype[k] = p->datatype; scalar[k] = sc; }
+      else if (stype[k] != p->datatype || scalar[k] != sc)
+        return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: input %d differs in type or rank across items", k);
+      if (!sc && p->ndims != 1)
+        return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: item %d input %d is not 1-D", i, k);
+    }
+    pg.type[k] = stype[k];
+  }
+  // PDL's trans type of every node; the chain computes in one type
+  std::vector<int> ntype(nnodes);
+  for (int j = 0; j < nnodes; j++) {
+    const pdlhip_node &nd = nodes[j];
+    if (nd.op < 0 || nd.op >= OP_NBINOPS) return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: node %d op", j);
+    auto otype = [&](int x) -> int {
+      if (x >= 0 && x < nin) return stype[x];
+      if (x < 0 && -x - 1 < j) return ntype[-x - 1];
+      return -2;
+    };
+    int ta = otype(nd.x), tb = otype(nd.y);
+    if (ta < -1 || tb < -1) return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: node %d operand", j);
+    ntype[j] = binop_type(nd.op, ta, tb);
+    pg.node[j].op = nd.op; pg.node[j].x = nd.x; pg.node[j].y = nd.y;
+  }
+  const int T = nnodes ? ntype[nnodes - 1] : stype[0];
+  // left-deep form: node 0 combines two slots, every later node the previous
+  // result and one slot
+  pg.lin = 1;
+  pg.first = 0;
+  for (int j = 0; j < nnodes && pg.lin; j++) {
+    const int x = nodes[j].x, y = nodes[j].y;
+    if (j == 0) {
+      if (x < 0 || y < 0) pg.lin = 0;
+      pg.first = x;
+      pg.lslot[0] = y;
+      pg.lswap[0] = 0;
+    } else if (x == -j && y >= 0) {
+      pg.lslot[j] = y; pg.lswap[j] = 0;
+    } else if (y == -j && x >= 0) {
+      pg.lslot[j] = x; pg.lswap[j] = 1;
+    } else {
+      pg.lin = 0;
+    }
+  }
+  for (int j = 0; j < nnodes; j++)
+    if (ntype[j] != T)
+      return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: node %d computes in %s, the chain in %s; run the ops "
+                      "separately", j, type_name(ntype[j]), type_name(T));
+  if (!scan && T != PDLHIP_F && T != PDLHIP_D)
+    return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: %s of a %s chain: float or double only", final_op,
+                    type_name(T));
+  if (scan && T >= PDLHIP_F)
+    return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: %s of %s: integer types only", final_op, type_name(T));
+  h.T = T;
+  int otype;
+  if (fop == R_DSUM || fop == R_DAVERAGE) otype = PDLHIP_D;
+  else if (fop == R_MIN || fop == R_MAX) otype = T;
+  else otype = std::max<int>(PDLHIP_L, T);
+  // item table and segments
+  std::vector<RagItem> items(n);
+  std::vector<RagSeg> segs;
+  std::vector<int32_t> multi;
+  for (int i = 0; i < n; i++) {
+    RagItem &it = items[i];
+    std::memset(&it, 0, sizeof it);
+    int64_t len = -1;
+    for (int k = 0; k < nin; k++) {
+      const pdlhip_pdl *p = in[i * nin + k];
+      it.ptr[k] = (const char *)p->data + p->offs * type_size(p->datatype);
+      if (scalar[k]) { it.inc[k] = 0; continue; }
+      it.inc[k] = p->incs[0];
+      if (len >= 0 && p->dims[0] != len)
+        return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: item %d: lengths %lld and %lld differ", i,
+                        (long long)len, (long long)p->dims[0]);
+      len = p->dims[0];
+    }
+    if (len < 0) len = 1;
+    pdlhip_pdl *o = out[i];
+    if (o && (o->state & PDLHIP_NOMYDIMS)) {
+      // a null output is created, as PDL creates a reduction's output
+      o->datatype = otype;
+      o->has_badvalue = 0;
+      o->ndims = scan ? 1 : 0;
+      o->dims[0] = len;
+      o->incs[0] = 1;
+      o->offs = 0;
+      o->data = g.plan_only ? nullptr : dev_alloc((size_t)(scan ? std::max<int64_t>(len, 1) : 1) * type_size(otype));
+      if (!g.plan_only && !o->data) return make_err(PDLHIP_EFATAL, "pdlhip_ragged_prepare: out of device memory");
+      o->state &= ~PDLHIP_NOMYDIMS;
+    }
+    if (!o || (o->state & PDLHIP_NOMYDIMS) || o->datatype != otype)
+      return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: output %d must exist with type %s", i,
+                      type_name(otype));
+    int64_t on = 1;
+    for (int d = 0; d < o->ndims; d++) on *= o->dims[d];
+    if ((!scan && on != 1) || (scan && (o->ndims != 1 || o->dims[0] != len)))
+      return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: output %d has the wrong shape", i);
+    it.out = (char *)o->data + o->offs * type_size(o->datatype);
+    it.out_inc = scan ? o->incs[0] : 0;
+    it.len = len;
+    it.seg0 = (int64_t)segs.size();
+    it.nseg = std::max<int64_t>(1, (len + RAG_SEG - 1) / RAG_SEG);
+    for (int64_t k = 0; k < it.nseg; k++) {
+      RagSeg sg;
+      sg.item = i;
+      sg.single = it.nseg == 1;
+      sg.start = k * RAG_SEG;
+      sg.end = std::min<int64_t>(len, (k + 1) * RAG_SEG);
+      segs.push_back(sg);
+    }
+    if (it.nseg > 1) multi.push_back(i);
+  }
+  pg.out_bv = default_badval(otype);
+  h.L.nseg = (int64_t)segs.size();
+  h.L.nmulti = (int64_t)multi.size();
+  // scans: items to blocks longest-first onto the least loaded block (LPT);
+  // each block then runs its items longest-first
+  std::vector<int32_t> order, bstart;
+  if (scan) {
+    const int nblk = std::min(n, 1024);
+    std::vector<int32_t> byl(n);
+    for (int i = 0; i < n; i++) byl[i] = i;
+    std::stable_sort(byl.begin(), byl.end(), [&](int32_t x, int32_t y) { return items[x].len > items[y].len; });
+    std::vector<std::vector<int32_t>> bins(nblk);
+    std::priority_queue<std::pair<int64_t, int32_t>, std::vector<std::pair<int64_t, int32_t>>,
+                        std::greater<std::pair<int64_t, int32_t>>> pq;
+    for (int b = 0; b < nblk; b++) pq.push({0, b});
+    for (int32_t i : byl) {
+      auto top = pq.top();
+      pq.pop();
+      bins[top.second].push_back(i);
+      pq.push({top.first + items[i].len + 256, top.second});   // + per-item overhead
+    }
+    bstart.push_back(0);
+    for (auto &bn : bins) {
+      order.insert(order.end(), bn.begin(), bn.end());
+      bstart.push_back((int32_t)order.size());
+    }
+    h.L.nblk = nblk;
+  }
+  if (g.plan_only) { *hout = new RaggedH(h); return ok(); }
+  auto up = [&](const void *src, size_t bytes, void **dst) -> bool {
+    *dst = dev_alloc(std::max<size_t>(bytes, 1));
+    if (!*dst) return false;
+    return !bytes || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  bool okk = up(items.data(), sizeof(RagItem) * items.size(), &h.d_items) &&
+             up(segs.data(), sizeof(RagSeg) * segs.size(), &h.d_segs) &&
+             up(multi.data(), sizeof(int32_t) * multi.size(), &h.d_multi) &&
+             up(order.data(), sizeof(int32_t) * order.size(), &h.d_order) &&
+             up(bstart.data(), sizeof(int32_t) * bstart.size(), &h.d_bstart);
+  h.d_scratch = dev_alloc(std::max<size_t>(64 * segs.size(), 64));
+  h.d_empty = dev_alloc(std::max<size_t>(sizeof(int32_t) * n, 4));
+  if (!okk || !h.d_scratch || !h.d_empty) {
+    for (void *p : {h.d_items, h.d_segs, h.d_multi, h.d_order, h.d_bstart, h.d_scratch, h.d_empty}) dev_free(p);
+    return make_err(PDLHIP_EFATAL, "pdlhip_ragged_prepare: out of device memory");
+  }
+  h.L.empty = (int32_t *)h.d_empty;
+  h.L.items = (const RagItem *)h.d_items;
+  h.L.segs = (const RagSeg *)h.d_segs;
+  h.L.multi = (const int32_t *)h.d_multi;
+  h.L.order = (const int32_t *)h.d_order;
+  h.L.bstart = (const int32_t *)h.d_bstart;
+  h.L.scratch = h.d_scratch;
+  *hout = new RaggedH(h);
+  return ok();
+}
+
 }  // namespace pdlhip
 
 // ==================================================================== C ABI
@@ -1124,6 +1358,81 @@ pdlhip_error pdlhip_run_matmult_list(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlh
   if (mode != PDLHIP_LIST_QUEUE && mode != PDLHIP_LIST_EACH)
     return make_err(PDLHIP_EUSERERROR, "pdlhip_run_matmult_list: unknown mode %d", mode);
   return run_list(n, a, b, c, mode);
+}
+
+struct pdlhip_ragged { pdlhip::RaggedH h; };
+
+pdlhip_error pdlhip_ragged_prepare(int nitems, int ninputs, pdlhip_pdl **inputs, int nnodes, const pdlhip_node *nodes,
+                                   const char *final_op, pdlhip_pdl **outputs, pdlhip_ragged **handle) {
+  if (!handle) return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: no handle");
+  RETURN_IF(ensure_init());
+  RaggedH *h = nullptr;
+  pdlhip_error e = ragged_prepare(nitems, ninputs, inputs, nnodes, nodes, final_op, outputs, &h);
+  if (e.error) return e;
+  *handle = new pdlhip_ragged{*h};
+  delete h;
+  return ok();
+}
+
+pdlhip_error pdlhip_ragged_run(pdlhip_ragged *handle) {
+  if (!handle) return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_run: no handle");
+  RETURN_IF(ensure_init());
+  if (g.plan_only) return ok();
+  const RaggedH &h = handle->h;
+  hipError_t e = h.scan ? launch_ragged_scan(h.T, h.op, h.L, g.stream) : launch_ragged_reduce(h.T, h.op, h.L, g.stream);
+  return e == hipSuccess ? ok() : hip_err(e, "ragged run");
+}
+
+pdlhip_error pdlhip_ragged_badflags(pdlhip_ragged *handle, int32_t *flags) {
+  if (!handle || !flags) return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_badflags: bad arguments");
+  RETURN_IF(ensure_init());
+  const RaggedH &h = handle->h;
+  if (h.op != R_MIN && h.op != R_MAX) {
+    std::memset(flags, 0, sizeof(int32_t) * h.n);
+    return ok();
+  }
+  if (g.plan_only || h.n == 0) return ok();
+  hipError_t e = hipStreamSynchronize(g.stream);
+  if (e == hipSuccess) e = hipMemcpy(flags, h.d_empty, sizeof(int32_t) * h.n, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? ok() : hip_err(e, "ragged badflags");
+}
+
+pdlhip_error pdlhip_ragged_free(pdlhip_ragged *handle) {
+  if (!handle) return ok();
+  RaggedH &h = handle->h;
+  for (void *p : {h.d_items, h.d_segs, h.d_multi, h.d_order, h.d_bstart, h.d_scratch, h.d_empty}) dev_free(p);
+  delete handle;
+  return ok();
+}
+
+pdlhip_error pdlhip_graph_begin(void) {
+  RETURN_IF(ensure_init());
+  hipError_t e = hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal);
+  return e == hipSuccess ? ok() : hip_err(e, "hipStreamBeginCapture");
+}
+
+pdlhip_error pdlhip_graph_end(void **graph) {
+  RETURN_IF(ensure_init());
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture(g.stream, &gr);
+  if (e != hipSuccess) return hip_err(e, "hipStreamEndCapture");
+  hipGraphExec_t ex = nullptr;
+  e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  if (e != hipSuccess) return hip_err(e, "hipGraphInstantiate");
+  *graph = (void *)ex;
+  return ok();
+}
+
+pdlhip_error pdlhip_graph_launch(void *graph) {
+  RETURN_IF(ensure_init());
+  hipError_t e = hipGraphLaunch((hipGraphExec_t)graph, g.stream);
+  return e == hipSuccess ? ok() : hip_err(e, "hipGraphLaunch");
+}
+
+pdlhip_error pdlhip_graph_destroy(void *graph) {
+  hipError_t e = hipGraphExecDestroy((hipGraphExec_t)graph);
+  return e == hipSuccess ? ok() : hip_err(e, "hipGraphExecDestroy");
 }
 
 pdlhip_error pdlhip_set_matmult_precision(int mode) {

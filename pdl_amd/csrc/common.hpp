@@ -83,6 +83,44 @@ struct MmItem {
   int64_t oa, ob, oc;
 };
 
+// ragged lists (k_ragged.hip): item table, segments and a fused program
+constexpr int RAG_MAXIN = 6, RAG_MAXNODE = 12, RAG_SEG = 4096;
+struct RagNode { int32_t op, x, y, pad; };
+struct RagProg {
+  int32_t nin, nnodes;
+  int32_t type[RAG_MAXIN];
+  RagNode node[RAG_MAXNODE];     // general form: operand k >= 0 slot k, k < 0 node -k-1
+  // left-deep form (lin = 1): v = slot[first]; node k: v = op_k(v, slot[lslot[k]])
+  // (operands swapped when lswap[k])
+  int32_t lin, first;
+  int32_t lslot[RAG_MAXNODE], lswap[RAG_MAXNODE];
+  pdlhip_value out_bv;
+};
+struct RagItem {
+  const char *ptr[RAG_MAXIN];   // element 0 of the item in each input slot
+  int64_t inc[RAG_MAXIN];       // stride (0 for a 0-dim scalar slot)
+  char *out;
+  int64_t out_inc;
+  int64_t len, seg0, nseg;
+};
+struct RagSeg { int32_t item, single; int64_t start, end; };
+struct RagLaunch {
+  RagProg prog;
+  const RagItem *items;
+  // reductions: one wave per segment of <= RAG_SEG elements; items of several
+  // segments are finished by a combine over `multi`
+  const RagSeg *segs;
+  int64_t nseg;
+  const int32_t *multi;
+  int64_t nmulti;
+  void *scratch;                // 64 bytes per segment
+  int32_t *empty;               // minimum/maximum: per item, 1 when no good value was found
+  // scans: one block per item; block b runs items order[bstart[b] .. bstart[b+1])
+  const int32_t *order;
+  const int32_t *bstart;
+  int32_t nblk;
+};
+
 // type-code -> C type
 template <int TC> struct CType;
 template <> struct CType<PDLHIP_SB> { typedef int8_t T; };

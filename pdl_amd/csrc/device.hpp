@@ -106,6 +106,9 @@ template <int OP, class T> __device__ inline T binop(T a, T b) {
   else if constexpr (OP == OP_AND) { if constexpr (IsFloat<T>::v) return a; else return (T)(a & b); }
   else if constexpr (OP == OP_XOR) { if constexpr (IsFloat<T>::v) return a; else return (T)(a ^ b); }
   else if constexpr (OP == OP_POWER) {
+    // a**2 is a*a: the correctly rounded square, which is pow's exact value
+    // rounded once (NaN, infinities, signed zeros and overflow included)
+    if constexpr (IsFloat<T>::v) { if (b == T(2)) return a * a; }
     if constexpr (sizeof(T) == 4 && IsFloat<T>::v) return powf(a, b);
     else if constexpr (IsFloat<T>::v) return pow(a, b);
     else return a;

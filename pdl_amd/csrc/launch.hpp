@@ -35,6 +35,10 @@ hipError_t launch_matmult_list(int T, const MmItem *d_items, const MmItem *h_ite
                                unsigned int nwork, unsigned int *d_counter, int tile, bool fused, bool each,
                                int nblocks, hipStream_t s);
 
+// ragged lists (k_ragged.hip)
+hipError_t launch_ragged_reduce(int T, int op, const RagLaunch &L, hipStream_t s);
+hipError_t launch_ragged_scan(int T, int op, const RagLaunch &L, hipStream_t s);
+
 // conversion / assignment between views (k_misc.hip): dst = (dtype) src with
 // PDL's bad-value mapping (pdlconv.c:163-174); also assgn (Ops.pd:382-397)
 hipError_t launch_convert(int stype, int dtype, const EwPlan &p, const void *src, void *dst, bool bad,
