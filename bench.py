@@ -202,6 +202,66 @@ def run_matmult(n=8192, nrows_check=16, reps=3):
     return res
 
 
+def run_small(reps=20):
+    """The small CPU-runnable workload: a, b double (2048,2048) uniform[-1,1]
+    from PDL's RNG seeded 0, c = a*b + a, s = sumover(c); m = matmult of
+    double (512,512) x (512,512) drawn next from the same stream.  Device ms
+    per op (HIP events) next to PDL's CPU path (the oracle, autopthread on
+    every host core); c must match bit for bit, s and m are scored by their
+    relative error against a long-double truth, device and CPU alike."""
+    from pdl_amd import core as C
+    from oracle import pdlcpu as O
+
+    N, M = 2048, 512
+    D = C.L.D
+    u = O.random_uniform(2 * N * N + 2 * M * M, 0) * 2.0 - 1.0
+    ah, bh = u[:N * N].reshape(N, N), u[N * N:2 * N * N].reshape(N, N)
+    mah, mbh = u[2 * N * N:2 * N * N + M * M].reshape(M, M), u[2 * N * N + M * M:].reshape(M, M)
+    a, b = C.ndarray.from_numpy(ah, D), C.ndarray.from_numpy(bh, D)
+    ma, mb = C.ndarray.from_numpy(mah, D), C.ndarray.from_numpy(mbh, D)
+    t, c = C.ndarray.new_dense(D, [N, N]), C.ndarray.new_dense(D, [N, N])
+    s, m = C.ndarray.new_dense(D, [N]), C.ndarray.new_dense(D, [M, M])
+    ops = (("a*b+a", lambda: (C._run("mult", [a, b, t]), C._run("plus", [t, a, c]))),
+           ("sumover", lambda: C._run("sumover", [c, s])),
+           ("matmult", lambda: C._run("matmult", [ma, mb, m])))
+    gpu = {}
+    for name, f in ops:
+        f()
+        C.synchronize()
+        e0, e1 = C.Event(), C.Event()
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        gpu[name] = round(e0.elapsed_ms(e1) / reps, 4)
+    nthr = os.cpu_count() or 1
+    oa, ob = O.OPdl.from_numpy(ah, O.D), O.OPdl.from_numpy(bh, O.D)
+    oma, omb = O.OPdl.from_numpy(mah, O.D), O.OPdl.from_numpy(mbh, O.D)
+    cpu = {}
+    t0 = time.perf_counter()
+    ot, oc = O.OPdl.null(), O.OPdl.null()
+    O.run("mult", oa, ob, ot, nthreads=nthr)
+    O.run("plus", ot, oa, oc, nthreads=nthr)
+    cpu["a*b+a"] = round((time.perf_counter() - t0) * 1e3, 3)
+    t0 = time.perf_counter()
+    os_ = O.OPdl.null()
+    O.run("sumover", oc, os_, nthreads=nthr)
+    cpu["sumover"] = round((time.perf_counter() - t0) * 1e3, 3)
+    t0 = time.perf_counter()
+    om = O.OPdl.null()
+    O.run("matmult", oma, omb, om, nthreads=nthr)
+    cpu["matmult"] = round((time.perf_counter() - t0) * 1e3, 3)
+    cg = oc.gather()
+    s_truth = cg.astype(np.longdouble).sum(axis=-1).astype(np.float64)
+    m_truth = O.truth_matmult_ld(mah, mbh)
+    rel = lambda x, tr: float(np.linalg.norm(x - tr) / np.linalg.norm(tr))
+    return {"workload": "a,b double (2048,2048) U[-1,1] seed 0: c=a*b+a, s=sumover(c); matmult double 512^2",
+            "gpu_ms": gpu, "cpu_ms": cpu, "cpu_threads": nthr,
+            "c_bit_exact": bool(np.array_equal(c.numpy().view(np.uint64), cg.view(np.uint64))),
+            "s_rel_err": {"gpu": rel(s.numpy(), s_truth), "pdl_cpu": rel(os_.gather(), s_truth)},
+            "m_rel_err": {"gpu": rel(m.numpy(), m_truth), "pdl_cpu": rel(om.gather(), m_truth)}}
+
+
 def run_ragged_matmult(count=4096, lo=8, hi=256, seed=0):
     """4096 independent square products, n ~ U{8..256}, float32 and float64
     entries uniform[-1,1]: one persistent work-queue launch vs one launch per
@@ -373,6 +433,12 @@ def run_single(args):
     if args.only == "ragged":
         print(json.dumps(run_ragged_chain()))
         return
+    if args.only == "small":
+        print(json.dumps(run_small()))
+        return
+    if args.only == "matmult":
+        print(json.dumps({"matmult": run_matmult(), "ragged_matmult": run_ragged_matmult()}))
+        return
     if args.only == "copy":
         src = C.random_uniform(256, 1024, 1024, dtype=C.L.F, seed=1)      # 1 GiB
         dst = C.ndarray.new_dense(C.L.F, src.dims)
@@ -393,14 +459,34 @@ def run_single(args):
     print(json.dumps({"only": args.only, "steps": args.steps}))
 
 
-def run_cpu_baseline(frac_k=2, min_seconds=10.0):
+def host_cores():
+    """the host threads this process may use (its affinity set, at most 16:
+    the CPU share of one GPU on the measurement hosts) and the CPU model"""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return min(n, 16), model
+
+
+def run_cpu_baseline(frac_k=2, min_seconds=10.0, min_runs=5):
     """The oracle (the C restatement of PDL's CPU path) on a bounded sample of
     the same workload: x (4096,4096,frac_k) instead of 16 k-planes, the int
-    arrays at 1/4 of their rows, repeated for at least `min_seconds`; PDL's
-    autopthread rule with every host core."""
+    arrays at 1/4 of their rows; PDL's autopthread rule over the host cores.
+    One warm-up pass, then passes until `min_seconds` and `min_runs` are
+    reached; the rate is taken from the median pass."""
     from oracle import pdlcpu as O
 
-    nthr = os.cpu_count() or 1
+    nthr, model = host_cores()
     rng = np.random.default_rng(3)
     NX, NY, NK = 4096, 4096, frac_k
     x = rng.standard_normal((NK, NY, NX), dtype=np.float32)
@@ -411,9 +497,9 @@ def run_cpu_baseline(frac_k=2, min_seconds=10.0):
     p32 = O.OPdl.from_numpy(i32h, O.L)
     p16 = O.OPdl.from_numpy(i16h, O.S, badflag=True)
     nb = algorithmic_bytes(NX, NY, NK, 4096, 1024, 8192, 2048)
-    reps = 0
-    t0 = time.perf_counter()
-    while True:
+
+    def one_pass():
+        t0 = time.perf_counter()
         z = O.OPdl.null()
         O.run("mult", px, py, z, nthreads=nthr)
         for op in ("sumover", "average", "minimum", "maximum"):
@@ -421,13 +507,16 @@ def run_cpu_baseline(frac_k=2, min_seconds=10.0):
         O.run("sumover", p32, O.OPdl.null(), nthreads=nthr)
         O.run("cumusumover", p32, O.OPdl.null(), nthreads=nthr)
         O.run("sumover", p16, O.OPdl.null(), nthreads=nthr)
-        reps += 1
-        dt = time.perf_counter() - t0
-        if dt >= min_seconds:
-            break
-    return {"value": round(reps * sum(nb.values()) / dt / 1e9, 2), "unit": "GB/s", "cores": nthr, "kind": "port",
-            "sample": "the same chain at x (4096,4096,%d), int32 (4096,1024), int16 (8192,2048), repeated %d times "
-                      "in %.1f s" % (NK, reps, dt)}
+        return time.perf_counter() - t0
+
+    one_pass()
+    times = []
+    while len(times) < min_runs or sum(times) < min_seconds:
+        times.append(one_pass())
+    med = float(np.median(times))
+    return {"value": round(sum(nb.values()) / med / 1e9, 2), "unit": "GB/s", "cores": nthr, "kind": "port",
+            "sample": "the same chain at x (4096,4096,%d), int32 (4096,1024), int16 (8192,2048): median of %d passes "
+                      "(%.1f s in all) on %s" % (NK, len(times), sum(times), model)}
 
 
 def main():
@@ -461,6 +550,7 @@ def main():
                      "frac": per["frac"], "traffic": measured_traffic(r["dominant"]), "kernel": r["dominant"]},
         "per_op": r["per_op"], "checks": r["check"],
     }
+    out["small"] = run_small()
     if not args.no_matmult:
         out["matmult"] = run_matmult()
         out["ragged_matmult"] = run_ragged_matmult()

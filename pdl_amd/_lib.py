@@ -103,8 +103,11 @@ EXPORTS["pdlhip_run_minimum_n_ind"] = ([PP, PP, ctypes.c_int64], Error)
 EXPORTS["pdlhip_run_maximum_n_ind"] = ([PP, PP, ctypes.c_int64], Error)
 EXPORTS["pdlhip_run_matmult"] = ([PP, PP, PP], Error)
 EXPORTS["pdlhip_ragged_prepare"] = ([ctypes.c_int, ctypes.c_int, ctypes.POINTER(PP), ctypes.c_int,
-                                     ctypes.POINTER(Node), ctypes.c_char_p, ctypes.POINTER(PP),
+                                     ctypes.POINTER(Node), ctypes.c_char_p, ctypes.POINTER(PP), ctypes.POINTER(PP),
                                      ctypes.POINTER(ctypes.c_void_p)], Error)
+EXPORTS["pdlhip_chain_prepare"] = ([ctypes.c_int, ctypes.POINTER(PP), ctypes.c_int, ctypes.POINTER(Node),
+                                    ctypes.c_char_p, PP, PP, ctypes.POINTER(ctypes.c_void_p)], Error)
+EXPORTS["pdlhip_ragged_count"] = ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)], Error)
 EXPORTS["pdlhip_ragged_run"] = ([ctypes.c_void_p], Error)
 EXPORTS["pdlhip_ragged_badflags"] = ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)], Error)
 EXPORTS["pdlhip_ragged_free"] = ([ctypes.c_void_p], Error)

@@ -615,59 +615,27 @@ def matmult_list(As, Bs, Cs=None, mode="queue"):
     return Cs
 
 
-class Ragged:
-    """A prepared list of 1-D ndarrays of uneven lengths, evaluated in a fixed
-    set of launches: per element a chain of binops over the input slots, then
-    `final` (sumover, dsumover, average, daverage, minimum, maximum,
-    cumusumover, cumuprodover) per item.
+def _nodes(chain):
+    def operand(v):
+        return -1 - v[1] if isinstance(v, tuple) else v
+    return (L.Node * max(1, len(chain)))(*[L.Node(L.BIOPS.index(op), operand(x), operand(y)) for op, x, y in chain])
 
-    items:  list of tuples, one ndarray (1-D, or a 0-dim scalar slot) per slot
-    chain:  list of (op, x, y); op is an Ops.pd binop name, an operand is a
-            slot index k >= 0 or ("node", j) for the result of node j
-    final:  the Ufunc op applied to the chain's value
-    Equivalent to the per-ndarray loop
-        for a, b, ... in items: out_i = (chain evaluated eagerly).final()
-    with the same rounding at every step."""
 
-    def __init__(self, items, chain, final, outputs=None):
-        _need()
-        n = len(items)
-        nin = len(items[0]) if n else 1
-        if any(len(t) != nin for t in items):
-            raise PDLError(L.EUSERERROR, "Ragged: items have different numbers of inputs")
-        self.outputs = [ndarray() for _ in range(n)] if outputs is None else list(outputs)
-        cin = [topdl(x)._to_c() for t in items for x in t]
-        cout = [o._to_c() for o in self.outputs]
-        created = [o.isnull() for o in self.outputs]
-        pin = (L.PP * max(1, len(cin)))(*[ctypes.pointer(x) for x in cin])
-        pout = (L.PP * max(1, n))(*[ctypes.pointer(x) for x in cout])
+class _Prepared:
+    """a prepared ragged list / chain handle (pdlhip_ragged)"""
 
-        def operand(v):
-            return -1 - v[1] if isinstance(v, tuple) else v
-        nodes = (L.Node * max(1, len(chain)))(*[L.Node(L.BIOPS.index(op), operand(x), operand(y))
-                                                for op, x, y in chain])
-        h = ctypes.c_void_p()
-        L.check(L.lib().pdlhip_ragged_prepare(n, nin, pin, len(chain), nodes, final.encode(), pout,
-                                              ctypes.byref(h)))
-        self._h = h
-        self._keep = [topdl(x) for t in items for x in t]
-        for o, s, cr in zip(self.outputs, cout, created):
-            o._from_c(s, cr)
+    _h = None
 
     def run(self):
         L.check(L.lib().pdlhip_ragged_run(self._h))
-        return self.outputs
+        return self
 
-    def badflags(self):
-        """after a run: per item, whether its output is BAD (minimum/maximum
-        of an item with no good value); sets those outputs' bad-value flag"""
-        n = len(self.outputs)
-        f = (ctypes.c_int32 * max(1, n))()
+    def _flags(self):
+        n = ctypes.c_int64()
+        L.check(L.lib().pdlhip_ragged_count(self._h, ctypes.byref(n)))
+        f = (ctypes.c_int32 * max(1, n.value))()
         L.check(L.lib().pdlhip_ragged_badflags(self._h, f))
-        for o, b in zip(self.outputs, f):
-            if b:
-                o.state |= L.BADVAL
-        return [bool(f[i]) for i in range(n)]
+        return [bool(f[i]) for i in range(n.value)]
 
     def free(self):
         if self._h:
@@ -679,6 +647,104 @@ class Ragged:
             self.free()
         except Exception:
             pass
+
+
+class Ragged(_Prepared):
+    """A prepared list of 1-D ndarrays of uneven lengths, evaluated in a fixed
+    set of launches: per element a chain of binops over the input slots,
+    optionally kept (values=True), then `final` (sumover, dsumover, average,
+    daverage, minimum, maximum, cumusumover, cumuprodover, or "none") per
+    item.
+
+    items:  list of tuples, one ndarray (1-D, or a 0-dim scalar slot) per slot
+    chain:  list of (op, x, y); op is an Ops.pd binop name, an operand is a
+            slot index k >= 0 or ("node", j) for the result of node j
+    final:  the Ufunc op applied to the chain's value
+    Equivalent to the per-ndarray loop
+        for a, b, ... in items: out_i = (chain evaluated eagerly).final()
+    with the same rounding at every step."""
+
+    def __init__(self, items, chain, final, outputs=None, values=None):
+        _need()
+        n = len(items)
+        nin = len(items[0]) if n else 1
+        if any(len(t) != nin for t in items):
+            raise PDLError(L.EUSERERROR, "Ragged: items have different numbers of inputs")
+        objs = [topdl(x) for t in items for x in t]
+        cin = [o._to_c() for o in objs]
+        pin = (L.PP * max(1, len(cin)))(*[ctypes.pointer(x) for x in cin])
+        self.outputs = None
+        if final != "none":
+            self.outputs = [ndarray() for _ in range(n)] if outputs is None else list(outputs)
+        self.values = None
+        if values is not None and values is not False:
+            self.values = [ndarray() for _ in range(n)] if values is True else list(values)
+        cout = [o._to_c() for o in self.outputs] if self.outputs is not None else []
+        cval = [o._to_c() for o in self.values] if self.values is not None else []
+        pout = (L.PP * max(1, n))(*[ctypes.pointer(x) for x in cout]) if self.outputs is not None else None
+        pval = (L.PP * max(1, n))(*[ctypes.pointer(x) for x in cval]) if self.values is not None else None
+        created = [o.isnull() for o in (self.outputs or [])] + [o.isnull() for o in (self.values or [])]
+        h = ctypes.c_void_p()
+        L.check(L.lib().pdlhip_ragged_prepare(n, nin, pin, len(chain), _nodes(chain), final.encode(), pout, pval,
+                                              ctypes.byref(h)))
+        self._h = h
+        self._keep = objs
+        for o, s, cr in zip((self.outputs or []) + (self.values or []), cout + cval, created):
+            o._from_c(s, cr)
+
+    def badflags(self):
+        """after a run: per item, whether its output is BAD (minimum/maximum
+        of an item with no good value); sets those outputs' bad-value flag"""
+        f = self._flags()
+        for o, b in zip(self.outputs or [], f):
+            if b:
+                o.state |= L.BADVAL
+        return f
+
+
+class Chain(_Prepared):
+    """A chain of binops over ndarrays that broadcast together, fused into one
+    pass: `value` (value=True or an ndarray) receives the chain's result over
+    every broadcast dim, `result` the final Ufunc op along dim 0.  For
+    example c = a*b + a and s = sumover(c) in one pass:
+
+        ch = Chain([a, b], [("mult", 0, 1), ("plus", ("node", 0), 0)], "sumover", value=True)
+        ch.run(); c, s = ch.value, ch.result
+    """
+
+    def __init__(self, inputs, chain, final="none", value=None, result=None):
+        _need()
+        objs = [topdl(x) for x in inputs]
+        cin = [o._to_c() for o in objs]
+        pin = (L.PP * len(cin))(*[ctypes.pointer(x) for x in cin])
+        self.value = None
+        if value is not None and value is not False:
+            self.value = ndarray() if value is True else value
+        self.result = None
+        if final != "none":
+            self.result = ndarray() if result is None else result
+        cv = self.value._to_c() if self.value is not None else None
+        cr = self.result._to_c() if self.result is not None else None
+        created = (self.value.isnull() if self.value is not None else False,
+                   self.result.isnull() if self.result is not None else False)
+        h = ctypes.c_void_p()
+        L.check(L.lib().pdlhip_chain_prepare(len(cin), pin, len(chain), _nodes(chain), final.encode(),
+                                             ctypes.pointer(cv) if cv is not None else None,
+                                             ctypes.pointer(cr) if cr is not None else None, ctypes.byref(h)))
+        self._h = h
+        self._keep = objs
+        if self.value is not None:
+            self.value._from_c(cv, created[0])
+        if self.result is not None:
+            self.result._from_c(cr, created[1])
+
+    def badflags(self):
+        """after a run: whether any row's minimum/maximum found no good value;
+        sets the result's bad-value flag then"""
+        bad = any(self._flags())
+        if bad and self.result is not None:
+            self.result.state |= L.BADVAL
+        return bad
 
 
 class Graph:

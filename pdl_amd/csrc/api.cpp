@@ -971,12 +971,6 @@ struct RaggedH {
        *d_scratch = nullptr, *d_empty = nullptr;
 };
 
-static int op_code(const char *n) {
-  static const char *names[] = {"plus", "mult", "minus", "divide", "gt", "lt", "le", "ge", "eq", "ne", "shiftleft",
-                                "shiftright", "or2", "and2", "xor", "power", "atan2", "modulo", "spaceship"};
-  for (int i = 0; i < OP_NBINOPS; i++) if (!strcmp(names[i], n)) return i;
-  return -1;
-}
 static const char *op_name(int op) {
   static const char *names[] = {"plus", "mult", "minus", "divide", "gt", "lt", "le", "ge", "eq", "ne", "shiftleft",
                                 "shiftright", "or2", "and2", "xor", "power", "atan2", "modulo", "spaceship"};
@@ -993,61 +987,47 @@ static int binop_type(int op, int ta, int tb) {
   return select_transtype(*s, v);
 }
 
-static pdlhip_error ragged_prepare(int n, int nin, pdlhip_pdl **in, int nnodes, const pdlhip_node *nodes,
-                                   const char *final_op, pdlhip_pdl **out, RaggedH **hout) {
-  if (n < 0 || nin < 1 || nin > RAG_MAXIN || nnodes < 0 || nnodes > RAG_MAXNODE || !final_op || (n && (!in || !out)))
-    return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: bad arguments");
-  struct { const char *name; int op; bool scan; } fins[] = {
-      {"sumover", R_SUM, false}, {"dsumover", R_DSUM, false}, {"average", R_AVERAGE, false},
-      {"daverage", R_DAVERAGE, false}, {"minimum", R_MIN, false}, {"maximum", R_MAX, false},
-      {"cumusumover", R_CUMUSUM, true}, {"cumuprodover", R_CUMUPROD, true}};
-  int fop = -1;
-  bool scan = false;
-  for (auto &f : fins) if (!strcmp(f.name, final_op)) { fop = f.op; scan = f.scan; }
-  if (fop < 0) return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: unsupported final op '%s'", final_op);
-  if (scan && (nnodes || nin != 1))
-    return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: %s takes one input and no chain", final_op);
-  RaggedH h;
-  h.scan = scan;
-  h.op = fop;
-  h.n = n;
+// the op applied per item after the chain ("none": the chain's values only)
+struct Fin { const char *name; int op; bool scan; };
+static const Fin *find_final(const char *name) {
+  static const Fin fins[] = {
+      {"none", RAG_NOREDUCE, false}, {"sumover", R_SUM, false}, {"dsumover", R_DSUM, false},
+      {"average", R_AVERAGE, false}, {"daverage", R_DAVERAGE, false}, {"minimum", R_MIN, false},
+      {"maximum", R_MAX, false}, {"cumusumover", R_CUMUSUM, true}, {"cumuprodover", R_CUMUPROD, true}};
+  if (!name) return nullptr;
+  for (const Fin &f : fins) if (!strcmp(f.name, name)) return &f;
+  return nullptr;
+}
+
+// The fused program of a chain over input slots of the given types: PDL's
+// trans type of every node (each must equal the chain's), the left-deep form
+// when the program has one, and the type of the final op's output.
+static pdlhip_error chain_setup(const char *who, int nin, const int *stype, int nnodes, const pdlhip_node *nodes,
+                                const Fin &f, RaggedH &h, int *otype) {
+  h.scan = f.scan;
+  h.op = f.op;
   std::memset(&h.L, 0, sizeof h.L);
   RagProg &pg = h.L.prog;
   pg.nin = nin;
   pg.nnodes = nnodes;
-  // slot types and shapes, uniform over the items
-  std::vector<int> stype(nin, -1), scalar(nin, 0);
   for (int k = 0; k < nin; k++) {
-    for (int i = 0; i < n; i++) {
-      const pdlhip_pdl *p = in[i * nin + k];
-      if (!p || (p->state & PDLHIP_NOMYDIMS))
-        return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: item %d input %d is null", i, k);
-      if (p->state & PDLHIP_BADVAL)
-        return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: item %d input %d has bad values; run the ops "
-                        "per ndarray", i, k);
-      if (!type_on_device(p->datatype))
-        return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: type %s not supported", type_name(p->datatype));
-      int sc = p->ndims == 0 ? 1 : 0;
-      if (i == 0) { stype[k] = p->datatype; scalar[k] = sc; }
-      else if (stype[k] != p->datatype || scalar[k] != sc)
-        return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: input %d differs in type or rank across items", k);
-      if (!sc && p->ndims != 1)
-        return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: item %d input %d is not 1-D", i, k);
-    }
+    if (!type_on_device(stype[k]))
+      return make_err(PDLHIP_EUSERERROR, "%s: type %s not supported", who, type_name(stype[k]));
     pg.type[k] = stype[k];
   }
-  // PDL's trans type of every node; the chain computes in one type
+  if (f.scan && (nnodes || nin != 1))
+    return make_err(PDLHIP_EUSERERROR, "%s: %s takes one input and no chain", who, f.name);
   std::vector<int> ntype(nnodes);
   for (int j = 0; j < nnodes; j++) {
     const pdlhip_node &nd = nodes[j];
-    if (nd.op < 0 || nd.op >= OP_NBINOPS) return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: node %d op", j);
-    auto otype = [&](int x) -> int {
+    if (nd.op < 0 || nd.op >= OP_NBINOPS) return make_err(PDLHIP_EUSERERROR, "%s: node %d op", who, j);
+    auto otp = [&](int x) -> int {
       if (x >= 0 && x < nin) return stype[x];
       if (x < 0 && -x - 1 < j) return ntype[-x - 1];
       return -2;
     };
-    int ta = otype(nd.x), tb = otype(nd.y);
-    if (ta < -1 || tb < -1) return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: node %d operand", j);
+    const int ta = otp(nd.x), tb = otp(nd.y);
+    if (ta < -1 || tb < -1) return make_err(PDLHIP_EUSERERROR, "%s: node %d operand", who, j);
     ntype[j] = binop_type(nd.op, ta, tb);
     pg.node[j].op = nd.op; pg.node[j].x = nd.x; pg.node[j].y = nd.y;
   }
@@ -1073,79 +1053,78 @@ static pdlhip_error ragged_prepare(int n, int nin, pdlhip_pdl **in, int nnodes, 
   }
   for (int j = 0; j < nnodes; j++)
     if (ntype[j] != T)
-      return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: node %d computes in %s, the chain in %s; run the ops "
-                      "separately", j, type_name(ntype[j]), type_name(T));
-  if (!scan && T != PDLHIP_F && T != PDLHIP_D)
-    return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: %s of a %s chain: float or double only", final_op,
-                    type_name(T));
-  if (scan && T >= PDLHIP_F)
-    return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: %s of %s: integer types only", final_op, type_name(T));
+      return make_err(PDLHIP_EUSERERROR, "%s: node %d computes in %s, the chain in %s; run the ops separately", who, j,
+                      type_name(ntype[j]), type_name(T));
+  if (!f.scan && T != PDLHIP_F && T != PDLHIP_D)
+    return make_err(PDLHIP_EUSERERROR, "%s: %s of a %s chain: float or double only", who, f.name, type_name(T));
+  if (f.scan && T >= PDLHIP_F)
+    return make_err(PDLHIP_EUSERERROR, "%s: %s of %s: integer types only", who, f.name, type_name(T));
   h.T = T;
-  int otype;
-  if (fop == R_DSUM || fop == R_DAVERAGE) otype = PDLHIP_D;
-  else if (fop == R_MIN || fop == R_MAX) otype = T;
-  else otype = std::max<int>(PDLHIP_L, T);
-  // item table and segments
-  std::vector<RagItem> items(n);
+  if (f.op == RAG_NOREDUCE) *otype = -1;
+  else if (f.op == R_DSUM || f.op == R_DAVERAGE) *otype = PDLHIP_D;
+  else if (f.op == R_MIN || f.op == R_MAX) *otype = T;
+  else *otype = std::max<int>(PDLHIP_L, T);
+  pg.out_bv = default_badval(*otype >= 0 ? *otype : T);
+  return ok();
+}
+
+// a null output is created with `type` and `dims`, as PDL creates an op's
+// output; an existing one must have that type and those dims
+static pdlhip_error output_for(const char *who, const char *what, int64_t idx, pdlhip_pdl *o, int type, int ndims,
+                               const int64_t *dims) {
+  if (!o) return make_err(PDLHIP_EUSERERROR, "%s: %s %lld is missing", who, what, (long long)idx);
+  if (o->state & PDLHIP_NOMYDIMS) {
+    int64_t n = 1;
+    o->datatype = type;
+    o->has_badvalue = 0;
+    o->ndims = ndims;
+    for (int d = 0; d < ndims; d++) {
+      o->dims[d] = dims[d];
+      o->incs[d] = n;
+      n *= dims[d];
+    }
+    o->offs = 0;
+    o->data = g.plan_only ? nullptr : dev_alloc((size_t)std::max<int64_t>(n, 1) * type_size(type));
+    if (!g.plan_only && !o->data) return make_err(PDLHIP_EFATAL, "%s: out of device memory", who);
+    o->state &= ~PDLHIP_NOMYDIMS;
+    return ok();
+  }
+  if (o->datatype != type)
+    return make_err(PDLHIP_EUSERERROR, "%s: %s %lld must exist with type %s", who, what, (long long)idx,
+                    type_name(type));
+  bool same = o->ndims == ndims;
+  for (int d = 0; same && d < ndims; d++) same = o->dims[d] == dims[d];
+  if (!same) return make_err(PDLHIP_EUSERERROR, "%s: %s %lld has the wrong shape", who, what, (long long)idx);
+  return ok();
+}
+
+// segments, the scans' block assignment and the device copies of an item
+// table
+static pdlhip_error ragged_finish(const char *who, RaggedH &h, std::vector<RagItem> &items, RaggedH **hout) {
+  const int n = (int)items.size();
+  h.n = n;
   std::vector<RagSeg> segs;
   std::vector<int32_t> multi;
   for (int i = 0; i < n; i++) {
     RagItem &it = items[i];
-    std::memset(&it, 0, sizeof it);
-    int64_t len = -1;
-    for (int k = 0; k < nin; k++) {
-      const pdlhip_pdl *p = in[i * nin + k];
-      it.ptr[k] = (const char *)p->data + p->offs * type_size(p->datatype);
-      if (scalar[k]) { it.inc[k] = 0; continue; }
-      it.inc[k] = p->incs[0];
-      if (len >= 0 && p->dims[0] != len)
-        return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: item %d: lengths %lld and %lld differ", i,
-                        (long long)len, (long long)p->dims[0]);
-      len = p->dims[0];
-    }
-    if (len < 0) len = 1;
-    pdlhip_pdl *o = out[i];
-    if (o && (o->state & PDLHIP_NOMYDIMS)) {
-      // a null output is created, as PDL creates a reduction's output
-      o->datatype = otype;
-      o->has_badvalue = 0;
-      o->ndims = scan ? 1 : 0;
-      o->dims[0] = len;
-      o->incs[0] = 1;
-      o->offs = 0;
-      o->data = g.plan_only ? nullptr : dev_alloc((size_t)(scan ? std::max<int64_t>(len, 1) : 1) * type_size(otype));
-      if (!g.plan_only && !o->data) return make_err(PDLHIP_EFATAL, "pdlhip_ragged_prepare: out of device memory");
-      o->state &= ~PDLHIP_NOMYDIMS;
-    }
-    if (!o || (o->state & PDLHIP_NOMYDIMS) || o->datatype != otype)
-      return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: output %d must exist with type %s", i,
-                      type_name(otype));
-    int64_t on = 1;
-    for (int d = 0; d < o->ndims; d++) on *= o->dims[d];
-    if ((!scan && on != 1) || (scan && (o->ndims != 1 || o->dims[0] != len)))
-      return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: output %d has the wrong shape", i);
-    it.out = (char *)o->data + o->offs * type_size(o->datatype);
-    it.out_inc = scan ? o->incs[0] : 0;
-    it.len = len;
     it.seg0 = (int64_t)segs.size();
-    it.nseg = std::max<int64_t>(1, (len + RAG_SEG - 1) / RAG_SEG);
+    it.nseg = std::max<int64_t>(1, (it.len + RAG_SEG - 1) / RAG_SEG);
     for (int64_t k = 0; k < it.nseg; k++) {
       RagSeg sg;
       sg.item = i;
       sg.single = it.nseg == 1;
       sg.start = k * RAG_SEG;
-      sg.end = std::min<int64_t>(len, (k + 1) * RAG_SEG);
+      sg.end = std::min<int64_t>(it.len, (k + 1) * RAG_SEG);
       segs.push_back(sg);
     }
-    if (it.nseg > 1) multi.push_back(i);
+    if (it.nseg > 1 && h.op != RAG_NOREDUCE) multi.push_back(i);
   }
-  pg.out_bv = default_badval(otype);
   h.L.nseg = (int64_t)segs.size();
   h.L.nmulti = (int64_t)multi.size();
   // scans: items to blocks longest-first onto the least loaded block (LPT);
   // each block then runs its items longest-first
   std::vector<int32_t> order, bstart;
-  if (scan) {
+  if (h.scan) {
     const int nblk = std::min(n, 1024);
     std::vector<int32_t> byl(n);
     for (int i = 0; i < n; i++) byl[i] = i;
@@ -1182,7 +1161,7 @@ static pdlhip_error ragged_prepare(int n, int nin, pdlhip_pdl **in, int nnodes, 
   h.d_empty = dev_alloc(std::max<size_t>(sizeof(int32_t) * n, 4));
   if (!okk || !h.d_scratch || !h.d_empty) {
     for (void *p : {h.d_items, h.d_segs, h.d_multi, h.d_order, h.d_bstart, h.d_scratch, h.d_empty}) dev_free(p);
-    return make_err(PDLHIP_EFATAL, "pdlhip_ragged_prepare: out of device memory");
+    return make_err(PDLHIP_EFATAL, "%s: out of device memory", who);
   }
   h.L.empty = (int32_t *)h.d_empty;
   h.L.items = (const RagItem *)h.d_items;
@@ -1193,6 +1172,178 @@ static pdlhip_error ragged_prepare(int n, int nin, pdlhip_pdl **in, int nnodes, 
   h.L.scratch = h.d_scratch;
   *hout = new RaggedH(h);
   return ok();
+}
+
+static char *elem_ptr(const pdlhip_pdl *p, int64_t off) {
+  return (char *)p->data + off * type_size(p->datatype);
+}
+
+static pdlhip_error ragged_prepare(int n, int nin, pdlhip_pdl **in, int nnodes, const pdlhip_node *nodes,
+                                   const char *final_op, pdlhip_pdl **out, pdlhip_pdl **values, RaggedH **hout) {
+  const char *who = "pdlhip_ragged_prepare";
+  if (n < 0 || nin < 1 || nin > RAG_MAXIN || nnodes < 0 || nnodes > RAG_MAXNODE || (n && !in))
+    return make_err(PDLHIP_EUSERERROR, "%s: bad arguments", who);
+  const Fin *f = find_final(final_op);
+  if (!f) return make_err(PDLHIP_EUSERERROR, "%s: unsupported final op '%s'", who, final_op ? final_op : "");
+  if (f->op == RAG_NOREDUCE && !values) return make_err(PDLHIP_EUSERERROR, "%s: 'none' needs value outputs", who);
+  if (f->op != RAG_NOREDUCE && n && !out) return make_err(PDLHIP_EUSERERROR, "%s: bad arguments", who);
+  if (f->scan && values) return make_err(PDLHIP_EUSERERROR, "%s: %s has no chain values", who, f->name);
+  // slot types and shapes, uniform over the items
+  std::vector<int> stype(nin, PDLHIP_D), scalar(nin, 0);
+  for (int k = 0; k < nin; k++) {
+    for (int i = 0; i < n; i++) {
+      const pdlhip_pdl *p = in[i * nin + k];
+      if (!p || (p->state & PDLHIP_NOMYDIMS))
+        return make_err(PDLHIP_EUSERERROR, "%s: item %d input %d is null", who, i, k);
+      if (p->state & PDLHIP_BADVAL)
+        return make_err(PDLHIP_EUSERERROR, "%s: item %d input %d has bad values; run the ops per ndarray", who, i, k);
+      const int sc = p->ndims == 0 ? 1 : 0;
+      if (i == 0) { stype[k] = p->datatype; scalar[k] = sc; }
+      else if (stype[k] != p->datatype || scalar[k] != sc)
+        return make_err(PDLHIP_EUSERERROR, "%s: input %d differs in type or rank across items", who, k);
+      if (!sc && p->ndims != 1) return make_err(PDLHIP_EUSERERROR, "%s: item %d input %d is not 1-D", who, i, k);
+    }
+  }
+  RaggedH h;
+  int otype = -1;
+  RETURN_IF(chain_setup(who, nin, stype.data(), nnodes, nodes, *f, h, &otype));
+  h.L.prog.store = values != nullptr;
+  std::vector<RagItem> items(n);
+  for (int i = 0; i < n; i++) {
+    RagItem &it = items[i];
+    std::memset(&it, 0, sizeof it);
+    int64_t len = -1;
+    for (int k = 0; k < nin; k++) {
+      const pdlhip_pdl *p = in[i * nin + k];
+      it.ptr[k] = elem_ptr(p, p->offs);
+      if (scalar[k]) { it.inc[k] = 0; continue; }
+      it.inc[k] = p->incs[0];
+      if (len >= 0 && p->dims[0] != len)
+        return make_err(PDLHIP_EUSERERROR, "%s: item %d: lengths %lld and %lld differ", who, i, (long long)len,
+                        (long long)p->dims[0]);
+      len = p->dims[0];
+    }
+    if (len < 0) len = 1;
+    it.len = len;
+    if (f->op != RAG_NOREDUCE) {
+      RETURN_IF(output_for(who, "output", i, out[i], otype, f->scan ? 1 : 0, &len));
+      it.out = elem_ptr(out[i], out[i]->offs);
+      it.out_inc = f->scan ? out[i]->incs[0] : 0;
+    }
+    if (values) {
+      RETURN_IF(output_for(who, "value output", i, values[i], h.T, 1, &len));
+      it.val = elem_ptr(values[i], values[i]->offs);
+      it.val_inc = values[i]->incs[0];
+    }
+  }
+  return ragged_finish(who, h, items, hout);
+}
+
+// A chain over ndarrays that broadcast together (implicit dims, size-1 dims
+// repeat), reduced along dim 0 by `final_op` into `result` (dims 1.. of the
+// broadcast) and/or kept whole in `value` (all broadcast dims).  Each row
+// along dim 0 is one item of a ragged launch of equal lengths; with no
+// reduction, rows that are contiguous in every operand are merged first.
+static pdlhip_error chain_prepare(int nin, pdlhip_pdl **in, int nnodes, const pdlhip_node *nodes,
+                                  const char *final_op, pdlhip_pdl *value, pdlhip_pdl *result, RaggedH **hout) {
+  const char *who = "pdlhip_chain_prepare";
+  if (nin < 1 || nin > RAG_MAXIN || nnodes < 0 || nnodes > RAG_MAXNODE || !in)
+    return make_err(PDLHIP_EUSERERROR, "%s: bad arguments", who);
+  const Fin *f = find_final(final_op);
+  if (!f) return make_err(PDLHIP_EUSERERROR, "%s: unsupported final op '%s'", who, final_op ? final_op : "");
+  if (f->op == RAG_NOREDUCE && !value) return make_err(PDLHIP_EUSERERROR, "%s: 'none' needs a value output", who);
+  if (f->op != RAG_NOREDUCE && !result) return make_err(PDLHIP_EUSERERROR, "%s: %s needs a result", who, f->name);
+  if (f->scan && value) return make_err(PDLHIP_EUSERERROR, "%s: %s has no chain values", who, f->name);
+  std::vector<int> stype(nin);
+  int nd = 1;
+  for (int k = 0; k < nin; k++) {
+    const pdlhip_pdl *p = in[k];
+    if (!p || (p->state & PDLHIP_NOMYDIMS)) return make_err(PDLHIP_EUSERERROR, "%s: input %d is null", who, k);
+    if (p->state & PDLHIP_BADVAL)
+      return make_err(PDLHIP_EUSERERROR, "%s: input %d has bad values; run the ops one by one", who, k);
+    stype[k] = p->datatype;
+    nd = std::max(nd, p->ndims);
+  }
+  // broadcast dims (pdl_broadcast_dim_checks: sizes must agree or be 1)
+  std::vector<int64_t> bd(nd, 1);
+  for (int k = 0; k < nin; k++)
+    for (int d = 0; d < in[k]->ndims; d++) {
+      const int64_t s = in[k]->dims[d];
+      if (s == 1) continue;
+      if (bd[d] != 1 && bd[d] != s)
+        return make_err(PDLHIP_EUSERERROR, "%s: mismatched broadcast dim %d: size %lld vs %lld", who, d,
+                        (long long)bd[d], (long long)s);
+      bd[d] = s;
+    }
+  RaggedH h;
+  int otype = -1;
+  RETURN_IF(chain_setup(who, nin, stype.data(), nnodes, nodes, *f, h, &otype));
+  h.L.prog.store = value != nullptr;
+  if (value) RETURN_IF(output_for(who, "value output", 0, value, h.T, nd, bd.data()));
+  if (f->op != RAG_NOREDUCE) {
+    if (f->scan) RETURN_IF(output_for(who, "result", 0, result, otype, nd, bd.data()));
+    else RETURN_IF(output_for(who, "result", 0, result, otype, nd - 1, bd.data() + 1));
+  }
+  // per operand (inputs, value, result) strides over the broadcast dims; 0
+  // where the operand repeats
+  const int nop = nin + 2;
+  std::vector<std::vector<int64_t>> inc(nop, std::vector<int64_t>(nd, 0));
+  std::vector<int64_t> off(nop, 0);
+  for (int k = 0; k < nin; k++) {
+    off[k] = in[k]->offs;
+    for (int d = 0; d < in[k]->ndims; d++) inc[k][d] = in[k]->dims[d] == 1 ? 0 : in[k]->incs[d];
+  }
+  if (value) {
+    off[nin] = value->offs;
+    for (int d = 0; d < nd; d++) inc[nin][d] = bd[d] == 1 ? 0 : value->incs[d];
+  }
+  if (f->op != RAG_NOREDUCE) {
+    off[nin + 1] = result->offs;
+    if (f->scan)
+      for (int d = 0; d < nd; d++) inc[nin + 1][d] = bd[d] == 1 ? 0 : result->incs[d];
+    else
+      for (int d = 1; d < nd; d++) inc[nin + 1][d] = bd[d] == 1 ? 0 : result->incs[d - 1];
+  }
+  // no reduction: fold dim d+1 into dim d where every operand is contiguous
+  // across them
+  std::vector<int64_t> dims = bd;
+  if (f->op == RAG_NOREDUCE) {
+    for (int d = 0; d + 1 < (int)dims.size();) {
+      bool merge = true;
+      for (int k = 0; k < nop && merge; k++) merge = inc[k][d + 1] == inc[k][d] * dims[d];
+      if (!merge) { d++; continue; }
+      dims[d] *= dims[d + 1];
+      dims.erase(dims.begin() + d + 1);
+      for (auto &v : inc) v.erase(v.begin() + d + 1);
+    }
+  }
+  const int rd = (int)dims.size();
+  int64_t rows = 1;
+  for (int d = 1; d < rd; d++) rows *= dims[d];
+  if (rows > INT32_MAX) return make_err(PDLHIP_EUSERERROR, "%s: too many rows", who);
+  std::vector<RagItem> items((size_t)rows);
+  std::vector<int64_t> idx(rd, 0);
+  for (int64_t r = 0; r < rows; r++) {
+    RagItem &it = items[r];
+    std::memset(&it, 0, sizeof it);
+    int64_t o[RAG_MAXIN + 2];
+    for (int k = 0; k < nop; k++) {
+      o[k] = off[k];
+      for (int d = 1; d < rd; d++) o[k] += idx[d] * inc[k][d];
+    }
+    for (int k = 0; k < nin; k++) {
+      it.ptr[k] = elem_ptr(in[k], o[k]);
+      it.inc[k] = inc[k][0];
+    }
+    it.len = dims[0];
+    if (value) { it.val = elem_ptr(value, o[nin]); it.val_inc = inc[nin][0]; }
+    if (f->op != RAG_NOREDUCE) { it.out = elem_ptr(result, o[nin + 1]); it.out_inc = f->scan ? inc[nin + 1][0] : 0; }
+    for (int d = 1; d < rd; d++) {
+      if (++idx[d] < dims[d]) break;
+      idx[d] = 0;
+    }
+  }
+  return ragged_finish(who, h, items, hout);
 }
 
 }  // namespace pdlhip
@@ -1363,11 +1514,25 @@ pdlhip_error pdlhip_run_matmult_list(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlh
 struct pdlhip_ragged { pdlhip::RaggedH h; };
 
 pdlhip_error pdlhip_ragged_prepare(int nitems, int ninputs, pdlhip_pdl **inputs, int nnodes, const pdlhip_node *nodes,
-                                   const char *final_op, pdlhip_pdl **outputs, pdlhip_ragged **handle) {
+                                   const char *final_op, pdlhip_pdl **outputs, pdlhip_pdl **values,
+                                   pdlhip_ragged **handle) {
   if (!handle) return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: no handle");
   RETURN_IF(ensure_init());
   RaggedH *h = nullptr;
-  pdlhip_error e = ragged_prepare(nitems, ninputs, inputs, nnodes, nodes, final_op, outputs, &h);
+  pdlhip_error e = ragged_prepare(nitems, ninputs, inputs, nnodes, nodes, final_op, outputs, values, &h);
+  if (e.error) return e;
+  *handle = new pdlhip_ragged{*h};
+  delete h;
+  return ok();
+}
+
+pdlhip_error pdlhip_chain_prepare(int ninputs, pdlhip_pdl **inputs, int nnodes, const pdlhip_node *nodes,
+                                  const char *final_op, pdlhip_pdl *value, pdlhip_pdl *result,
+                                  pdlhip_ragged **handle) {
+  if (!handle) return make_err(PDLHIP_EUSERERROR, "pdlhip_chain_prepare: no handle");
+  RETURN_IF(ensure_init());
+  RaggedH *h = nullptr;
+  pdlhip_error e = chain_prepare(ninputs, inputs, nnodes, nodes, final_op, value, result, &h);
   if (e.error) return e;
   *handle = new pdlhip_ragged{*h};
   delete h;
@@ -1395,6 +1560,12 @@ pdlhip_error pdlhip_ragged_badflags(pdlhip_ragged *handle, int32_t *flags) {
   hipError_t e = hipStreamSynchronize(g.stream);
   if (e == hipSuccess) e = hipMemcpy(flags, h.d_empty, sizeof(int32_t) * h.n, hipMemcpyDeviceToHost);
   return e == hipSuccess ? ok() : hip_err(e, "ragged badflags");
+}
+
+pdlhip_error pdlhip_ragged_count(pdlhip_ragged *handle, int64_t *nitems) {
+  if (!handle || !nitems) return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_count: bad arguments");
+  *nitems = handle->h.n;
+  return ok();
 }
 
 pdlhip_error pdlhip_ragged_free(pdlhip_ragged *handle) {

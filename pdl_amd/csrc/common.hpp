@@ -85,6 +85,7 @@ struct MmItem {
 
 // ragged lists (k_ragged.hip): item table, segments and a fused program
 constexpr int RAG_MAXIN = 6, RAG_MAXNODE = 12, RAG_SEG = 4096;
+constexpr int RAG_NOREDUCE = -1;   // final op "none": the chain's values only
 struct RagNode { int32_t op, x, y, pad; };
 struct RagProg {
   int32_t nin, nnodes;
@@ -94,13 +95,16 @@ struct RagProg {
   // (operands swapped when lswap[k])
   int32_t lin, first;
   int32_t lslot[RAG_MAXNODE], lswap[RAG_MAXNODE];
+  int32_t store;                 // write every chain value to the item's `val`
   pdlhip_value out_bv;
 };
 struct RagItem {
   const char *ptr[RAG_MAXIN];   // element 0 of the item in each input slot
   int64_t inc[RAG_MAXIN];       // stride (0 for a 0-dim scalar slot)
-  char *out;
+  char *out;                    // reduction result (scans: the scanned vector)
   int64_t out_inc;
+  char *val;                    // chain values (when stored), the chain's type
+  int64_t val_inc;
   int64_t len, seg0, nseg;
 };
 struct RagSeg { int32_t item, single; int64_t start, end; };

@@ -43,8 +43,8 @@ __device__ inline float pow2_scale(float m, int target) {
   return ldexpf(1.0f, x);
 }
 
-// max |x| over k of each of R rows: one wave per row when k is the unit-stride
-// index, else one thread per row (adjacent threads on adjacent rows)
+// max |x| over k of each of R rows, one wave per row, when k is the
+// unit-stride index
 __global__ __launch_bounds__(256) void absmax_rows_wave(const float *__restrict__ src, int64_t o, int64_t inc_r,
                                                         int64_t inc_k, int64_t R, int64_t K, float *smax) {
   const int lane = threadIdx.x & 63;
@@ -56,13 +56,36 @@ __global__ __launch_bounds__(256) void absmax_rows_wave(const float *__restrict_
   if (lane == 0) smax[r] = m;
 }
 
-__global__ __launch_bounds__(256) void absmax_rows_thread(const float *__restrict__ src, int64_t o, int64_t inc_r,
-                                                          int64_t inc_k, int64_t R, int64_t K, float *smax) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (r >= R) return;
+// max |x| per row of one 64x64 tile (read as pack_tile reads it, coalesced
+// along whichever index is unit-stride), folded into smax with an atomic max
+// on the bit pattern (non-negative floats order like their bits); smax starts
+// at zero (zero_u32).  For operands whose k index is strided, where a thread
+// per row would walk K serially.
+__global__ __launch_bounds__(256) void absmax_tile(const float *__restrict__ src, int64_t o, int64_t inc_r,
+                                                   int64_t inc_k, int64_t R, int64_t K, unsigned *smax,
+                                                   int64_t ntk) {
+  __shared__ float t[64][65];
+  const int64_t r0 = (blockIdx.x / ntk) * 64, k0 = (blockIdx.x % ntk) * 64;
+  const bool kfast = (inc_k == 1 || inc_k == -1 || inc_r != 1);
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    int rr, kk;
+    if (kfast) { kk = e % 64; rr = e / 64; } else { rr = e % 64; kk = e / 64; }
+    const int64_t gr = r0 + rr, gk = k0 + kk;
+    t[rr][kk] = (gr < R && gk < K) ? fabsf(src[o + gr * inc_r + gk * inc_k]) : 0.0f;
+  }
+  __syncthreads();
+  const int rr = threadIdx.x >> 2, part = threadIdx.x & 3;
   float m = 0.0f;
-  for (int64_t k = 0; k < K; k++) m = fmaxf(m, fabsf(src[o + r * inc_r + k * inc_k]));
-  smax[r] = m;
+#pragma unroll
+  for (int i = 0; i < 16; i++) m = fmaxf(m, t[rr][part * 16 + i]);
+  m = fmaxf(m, __shfl_xor(m, 1, 64));
+  m = fmaxf(m, __shfl_xor(m, 2, 64));
+  if (part == 0 && r0 + rr < R) atomicMax(&smax[r0 + rr], __float_as_uint(m));
+}
+
+__global__ __launch_bounds__(256) void zero_u32(unsigned *p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
 // scale[r] from smax[r]; zero for padded rows
@@ -114,74 +137,132 @@ __device__ inline void mm_batch_offsets_m(const MmPlan &p, int64_t bidx, int64_t
   }
 }
 
+// block -> C tile.  The 8 XCDs take blocks round-robin (block b runs on XCD
+// b % 8), so each XCD is given one contiguous share of the tile order; within
+// a share, tiles go in groups of 8 tile-rows, column by column, so the blocks
+// resident on one XCD at a time reuse the same A row-panels and B
+// column-panels from that XCD's L2.
+__device__ inline void tile_of(int64_t b, int64_t ntm, int64_t ntn, int64_t &tm, int64_t &tn) {
+  const int64_t nb = ntm * ntn;
+  const int64_t t = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+  constexpr int64_t G = 8;
+  const int64_t g = t / (G * ntn), r = t % (G * ntn);
+  const int64_t rows = ntm - g * G < G ? ntm - g * G : G;
+  tm = g * G + r % rows;
+  tn = r / rows;
+}
+
+// The k loop is software-pipelined over two LDS stages: while the matrix
+// cores work on stage k from LDS, the operands of step k+1 (fetched into
+// registers during step k-1) are written to the other stage and those of
+// step k+2 are fetched, so each step has one barrier and global latency is
+// hidden behind a whole step.  WM = 16-row A fragments per wave: a wave
+// computes (16*WM) x 64 of C, the block (32*WM) x 128.
+template <int WM>
 __global__ __launch_bounds__(256) void split16_gemm(const half_t *__restrict__ Ahi, const half_t *__restrict__ Alo,
                                                    const half_t *__restrict__ Bhi, const half_t *__restrict__ Blo,
                                                    const float *__restrict__ sA, const float *__restrict__ sB,
-                                                   float *__restrict__ c, MmPlan p, int64_t Kp, int64_t ntn) {
-  __shared__ half_t lA[2][MB][KPAD];
-  __shared__ half_t lB[2][NB][KPAD];
-  const int64_t tm = blockIdx.x / ntn, tn = blockIdx.x % ntn;
+                                                   float *__restrict__ c, MmPlan p, int64_t Kp, int64_t ntm,
+                                                   int64_t ntn) {
+  constexpr int BM = 32 * WM, AF = BM / 64;   // block rows; A half8 per thread per array per k-step
+  __shared__ half_t lA[2][2][BM][KPAD];       // [stage][hi/lo]
+  __shared__ half_t lB[2][2][NB][KPAD];
+  int64_t tm, tn;
+  tile_of(blockIdx.x, ntm, ntn, tm, tn);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wr = w >> 1, wc = w & 1;
-  const half_t *ah = Ahi + tm * MB * Kp, *al = Alo + tm * MB * Kp;
+  const half_t *ah = Ahi + tm * BM * Kp, *al = Alo + tm * BM * Kp;
   const half_t *bh = Bhi + tn * NB * Kp, *bl = Blo + tn * NB * Kp;
-  f32x4 acc[4][4];
+  f32x4 acc[WM][4];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < WM; i++)
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int64_t k0 = 0; k0 < Kp; k0 += KB) {
-    // 128 rows x 32 halves = 512 half8 per array; 2 per thread
+  // rows x 32 halves = 4 half8 per row
+  half8 ra[AF][2], rb[2][2];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int s = 0; s < AF; s++) {
+      const int idx = tid + 256 * s;
+      const int64_t off = (int64_t)(idx >> 2) * Kp + k0 + (idx & 3) * 8;
+      ra[s][0] = *reinterpret_cast<const half8 *>(ah + off);
+      ra[s][1] = *reinterpret_cast<const half8 *>(al + off);
+    }
 #pragma unroll
     for (int s = 0; s < 2; s++) {
       const int idx = tid + 256 * s;
-      const int row = idx >> 2, seg = (idx & 3) * 8;
-      *reinterpret_cast<half8 *>(&lA[0][row][seg]) = *reinterpret_cast<const half8 *>(ah + row * Kp + k0 + seg);
-      *reinterpret_cast<half8 *>(&lA[1][row][seg]) = *reinterpret_cast<const half8 *>(al + row * Kp + k0 + seg);
-      *reinterpret_cast<half8 *>(&lB[0][row][seg]) = *reinterpret_cast<const half8 *>(bh + row * Kp + k0 + seg);
-      *reinterpret_cast<half8 *>(&lB[1][row][seg]) = *reinterpret_cast<const half8 *>(bl + row * Kp + k0 + seg);
+      const int64_t off = (int64_t)(idx >> 2) * Kp + k0 + (idx & 3) * 8;
+      rb[s][0] = *reinterpret_cast<const half8 *>(bh + off);
+      rb[s][1] = *reinterpret_cast<const half8 *>(bl + off);
     }
-    __syncthreads();
-    half8 fah[4], fal[4], fbh[4], fbl[4];
+  };
+  auto stage = [&](int st) {
+#pragma unroll
+    for (int s = 0; s < AF; s++) {
+      const int idx = tid + 256 * s;
+      *reinterpret_cast<half8 *>(&lA[st][0][idx >> 2][(idx & 3) * 8]) = ra[s][0];
+      *reinterpret_cast<half8 *>(&lA[st][1][idx >> 2][(idx & 3) * 8]) = ra[s][1];
+    }
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      const int idx = tid + 256 * s;
+      *reinterpret_cast<half8 *>(&lB[st][0][idx >> 2][(idx & 3) * 8]) = rb[s][0];
+      *reinterpret_cast<half8 *>(&lB[st][1][idx >> 2][(idx & 3) * 8]) = rb[s][1];
+    }
+  };
+  fetch(0);
+  stage(0);
+  if (KB < Kp) fetch(KB);
+  __syncthreads();
+  int cur = 0;
+  for (int64_t k0 = 0; k0 < Kp; k0 += KB) {
     const int lr = lane & 15, lk = (lane >> 4) * 8;
+    half8 fbh[4], fbl[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      fah[i] = *reinterpret_cast<const half8 *>(&lA[0][wr * 64 + i * 16 + lr][lk]);
-      fal[i] = *reinterpret_cast<const half8 *>(&lA[1][wr * 64 + i * 16 + lr][lk]);
-      fbh[i] = *reinterpret_cast<const half8 *>(&lB[0][wc * 64 + i * 16 + lr][lk]);
-      fbl[i] = *reinterpret_cast<const half8 *>(&lB[1][wc * 64 + i * 16 + lr][lk]);
+    for (int j = 0; j < 4; j++) {
+      fbh[j] = *reinterpret_cast<const half8 *>(&lB[cur][0][wc * 64 + j * 16 + lr][lk]);
+      fbl[j] = *reinterpret_cast<const half8 *>(&lB[cur][1][wc * 64 + j * 16 + lr][lk]);
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < WM; i++) {
+      const half8 fah = *reinterpret_cast<const half8 *>(&lA[cur][0][wr * 16 * WM + i * 16 + lr][lk]);
+      const half8 fal = *reinterpret_cast<const half8 *>(&lA[cur][1][wr * 16 * WM + i * 16 + lr][lk]);
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fal[i], fbh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah[i], fbl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah[i], fbh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fal, fbh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah, fbl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah, fbh[j], acc[i][j], 0, 0, 0);
       }
+    }
+    // the other stage was last read in the previous step, before the barrier
+    if (k0 + KB < Kp) stage(cur ^ 1);
+    if (k0 + 2 * KB < Kp) fetch(k0 + 2 * KB);
     __syncthreads();
+    cur ^= 1;
   }
   // D layout of a 16x16 tile: lane l, reg r -> row 4*(l/16)+r, col l%16
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < WM; i++)
 #pragma unroll
     for (int j = 0; j < 4; j++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int64_t h = tm * MB + wr * 64 + i * 16 + 4 * (lane >> 4) + r;
+        const int64_t h = tm * BM + wr * 16 * WM + i * 16 + 4 * (lane >> 4) + r;
         const int64_t wcol = tn * NB + wc * 64 + j * 16 + (lane & 15);
         if (h < p.H && wcol < p.W)
           c[p.off[2] + wcol * p.c_w + h * p.c_h] = (acc[i][j][r] * (1.0f / sA[h])) * (1.0f / sB[wcol]);
       }
 }
 
-// FP8 e4m3: one product per tile
+// FP8 e4m3: one product per tile, pipelined like split16_gemm
 __global__ __launch_bounds__(256) void fp8_gemm(const uint8_t *__restrict__ A8, const uint8_t *__restrict__ B8,
                                                const float *__restrict__ sA, const float *__restrict__ sB,
-                                               float *__restrict__ c, MmPlan p, int64_t Kp, int64_t ntn) {
+                                               float *__restrict__ c, MmPlan p, int64_t Kp, int64_t ntm,
+                                               int64_t ntn) {
   __shared__ uint8_t lA[MB][KB + 16];
   __shared__ uint8_t lB[NB][KB + 16];
-  const int64_t tm = blockIdx.x / ntn, tn = blockIdx.x % ntn;
+  int64_t tm, tn;
+  tile_of(blockIdx.x, ntm, ntn, tm, tn);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wr = w >> 1, wc = w & 1;
   const uint8_t *a8 = A8 + tm * MB * Kp, *b8 = B8 + tn * NB * Kp;
@@ -190,14 +271,18 @@ __global__ __launch_bounds__(256) void fp8_gemm(const uint8_t *__restrict__ A8, 
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // 128 rows x 32 bytes = 256 x 16 B per array: one per thread
+  const int row = tid >> 1, seg = (tid & 1) * 16;
+  uint4 ra = *reinterpret_cast<const uint4 *>(a8 + row * Kp + seg);
+  uint4 rb = *reinterpret_cast<const uint4 *>(b8 + row * Kp + seg);
   for (int64_t k0 = 0; k0 < Kp; k0 += KB) {
-    // 128 rows x 32 bytes = 256 x 16 B per array: one per thread
-    {
-      const int row = tid >> 1, seg = (tid & 1) * 16;
-      *reinterpret_cast<uint4 *>(&lA[row][seg]) = *reinterpret_cast<const uint4 *>(a8 + row * Kp + k0 + seg);
-      *reinterpret_cast<uint4 *>(&lB[row][seg]) = *reinterpret_cast<const uint4 *>(b8 + row * Kp + k0 + seg);
-    }
+    *reinterpret_cast<uint4 *>(&lA[row][seg]) = ra;
+    *reinterpret_cast<uint4 *>(&lB[row][seg]) = rb;
     __syncthreads();
+    if (k0 + KB < Kp) {
+      ra = *reinterpret_cast<const uint4 *>(a8 + row * Kp + k0 + KB + seg);
+      rb = *reinterpret_cast<const uint4 *>(b8 + row * Kp + k0 + KB + seg);
+    }
     const int lr = lane & 15, lk = (lane >> 4) * 8;
     long fa[4], fb[4];
 #pragma unroll
@@ -230,9 +315,12 @@ static hipError_t prep_operand(const float *src, int64_t o, int64_t inc_r, int64
   if (inc_k == 1 || inc_k == -1)
     hipLaunchKernelGGL(absmax_rows_wave, dim3((unsigned)((R + 3) / 4)), dim3(256), 0, s, src, o, inc_r, inc_k, R, K,
                        scale);
-  else
-    hipLaunchKernelGGL(absmax_rows_thread, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, src, o, inc_r, inc_k,
-                       R, K, scale);
+  else {
+    const int64_t nr = (R + 63) / 64, nk = (K + 63) / 64;
+    hipLaunchKernelGGL(zero_u32, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, (unsigned *)scale, R);
+    hipLaunchKernelGGL(absmax_tile, dim3((unsigned)(nr * nk)), dim3(256), 0, s, src, o, inc_r, inc_k, R, K,
+                       (unsigned *)scale, nk);
+  }
   hipLaunchKernelGGL(make_scales, dim3((unsigned)((Rp + 255) / 256)), dim3(256), 0, s, scale, R, Rp, fp8 ? 8 : 15);
   const int64_t ntr = (Rp + 63) / 64, ntk = (Kp + 63) / 64;
   if (fp8)
@@ -249,7 +337,11 @@ hipError_t launch_mfma_f32(const MmPlan &p, const float *a, const float *b, floa
   // only where a 128x128 tile is mostly useful
   if (p.nbatch != 1 || p.H < 64 || p.W < 64 || p.T < 32) return hipErrorNotSupported;
   const bool fp8 = (mode == PDLHIP_MM_FP8);
-  const int64_t Hp = (p.H + MB - 1) / MB * MB, Wp = (p.W + NB - 1) / NB * NB, Kp = (p.T + KB - 1) / KB * KB;
+  // 256-row blocks for split16 measured no faster than 128-row ones (the
+  // larger tile leaves one wave per SIMD); kept selectable here
+  const bool wide = false;
+  const int64_t BMr = wide ? 256 : MB;
+  const int64_t Hp = (p.H + BMr - 1) / BMr * BMr, Wp = (p.W + NB - 1) / NB * NB, Kp = (p.T + KB - 1) / KB * KB;
   const size_t esz = fp8 ? 1 : 2, ncopy = fp8 ? 1 : 2;
   char *buf = (char *)scratch_alloc((size_t)(Hp + Wp) * Kp * esz * ncopy + (size_t)(Hp + Wp) * sizeof(float) + 256);
   if (!buf) return hipErrorOutOfMemory;
@@ -270,13 +362,17 @@ hipError_t launch_mfma_f32(const MmPlan &p, const float *a, const float *b, floa
   // A rows = h (stride a_h), k = t (stride a_t); B "rows" = w (b_w), k = t (b_t)
   hipError_t e = prep_operand(a, o[0], p.a_h, p.a_t, p.H, p.T, Hp, Kp, sA, Ahi, Alo, A8, fp8, s);
   if (e == hipSuccess) e = prep_operand(b, o[1], p.b_w, p.b_t, p.W, p.T, Wp, Kp, sB, Bhi, Blo, B8, fp8, s);
-  const int64_t ntm = Hp / MB, ntn = Wp / NB;
+  const int64_t ntm = Hp / BMr, ntn = Wp / NB;
   if (e == hipSuccess) {
     if (fp8)
-      hipLaunchKernelGGL(fp8_gemm, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, A8, B8, sA, sB, c, p, Kp, ntn);
+      hipLaunchKernelGGL(fp8_gemm, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, A8, B8, sA, sB, c, p, Kp, ntm,
+                         ntn);
+    else if (wide)
+      hipLaunchKernelGGL(split16_gemm<8>, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, Ahi, Alo, Bhi, Blo, sA, sB, c,
+                         p, Kp, ntm, ntn);
     else
-      hipLaunchKernelGGL(split16_gemm, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, Ahi, Alo, Bhi, Blo, sA, sB, c, p,
-                         Kp, ntn);
+      hipLaunchKernelGGL(split16_gemm<4>, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, Ahi, Alo, Bhi, Blo, sA, sB, c,
+                         p, Kp, ntm, ntn);
     e = hipGetLastError();
   }
   scratch_free(buf);

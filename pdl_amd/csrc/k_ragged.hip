@@ -6,8 +6,8 @@
 // Reductions with a fused chain: each element's value is computed in
 // registers by a small program of Ops.pd binops (each node rounding in the
 // chain's trans type, exactly as PDL's eager temporaries would, with no FMA
-// contraction), then accumulated with the partial rules of Ufunc.pd
-// (reduce_common.hpp).  Items are cut into segments of at most RAG_SEG
+// contraction), optionally stored, then accumulated with the partial rules of
+// Ufunc.pd (reduce_common.hpp) unless the final op is "none".  Items are cut into segments of at most RAG_SEG
 // elements, one wave per segment; items of several segments are finished by a
 // combine kernel.  A left-deep chain (each node takes the previous result
 // and one input slot) runs in a kernel that works on U elements per lane per
@@ -154,6 +154,25 @@ __device__ inline void chain_lin(T *acc, const RagProg &pg, const RagItem &it, i
   }
 }
 
+// the chain's values at elements base + j*64 into the item's value output
+template <class T, int U, bool TAIL>
+__device__ inline void store_u(const T *v, char *p, int64_t inc, int64_t base, int64_t end) {
+  T *q = (T *)p;
+  if (inc == 1) {
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int64_t e = base + (int64_t)j * 64;
+      if (!TAIL || e < end) __builtin_nontemporal_store(v[j], q + e);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int64_t e = base + (int64_t)j * 64;
+      if (!TAIL || e < end) q[e * inc] = v[j];
+    }
+  }
+}
+
 template <int OP, class T, class Tout, class Tacc>
 __device__ inline void seg_finish(const Part<T, Tacc> &p, const RagProg &pg, const RagItem &it, int32_t item,
                                   int32_t *empty) {
@@ -175,33 +194,46 @@ __global__ __launch_bounds__(256) void rag_reduce(RagProg pg, const RagItem *__r
   for (int64_t s = (int64_t)blockIdx.x * 4 + wv; s < nseg; s += nw) {
     const RagSeg sg = segs[s];
     const RagItem &it = items[sg.item];
+    constexpr bool RED = OP != RAG_NOREDUCE;
     Part<T, Tacc> p;
-    part_init<OP, T, Tacc>(p);
+    if constexpr (RED) part_init<OP, T, Tacc>(p);
     if constexpr (LIN) {
       constexpr int64_t STEP = 64 * U;
       int64_t b = sg.start;
       for (; b + STEP <= sg.end; b += STEP) {
         T v[U];
         chain_lin<T, U, false>(v, pg, it, b + lane, sg.end);
+        if (pg.store) store_u<T, U, false>(v, it.val, it.val_inc, b + lane, sg.end);
+        if constexpr (RED) {
 #pragma unroll
-        for (int j = 0; j < U; j++) part_push<OP, T, Tacc>(p, v[j], b + lane + (int64_t)j * 64);
+          for (int j = 0; j < U; j++) part_push<OP, T, Tacc>(p, v[j], b + lane + (int64_t)j * 64);
+        }
       }
       if (b < sg.end) {
         T v[U];
         chain_lin<T, U, true>(v, pg, it, b + lane, sg.end);
+        if (pg.store) store_u<T, U, true>(v, it.val, it.val_inc, b + lane, sg.end);
+        if constexpr (RED) {
 #pragma unroll
-        for (int j = 0; j < U; j++) {
-          const int64_t e = b + lane + (int64_t)j * 64;
-          if (e < sg.end) part_push<OP, T, Tacc>(p, v[j], e);
+          for (int j = 0; j < U; j++) {
+            const int64_t e = b + lane + (int64_t)j * 64;
+            if (e < sg.end) part_push<OP, T, Tacc>(p, v[j], e);
+          }
         }
       }
     } else {
-      for (int64_t i = sg.start + lane; i < sg.end; i += 64) part_push<OP, T, Tacc>(p, chain_value<T>(pg, it, i), i);
+      for (int64_t i = sg.start + lane; i < sg.end; i += 64) {
+        const T x = chain_value<T>(pg, it, i);
+        if (pg.store) ((T *)it.val)[i * it.val_inc] = x;
+        if constexpr (RED) part_push<OP, T, Tacc>(p, x, i);
+      }
     }
-    wave_reduce<OP, T, Tacc>(p);
-    if (lane == 0) {
-      if (sg.single) seg_finish<OP, T, Tout, Tacc>(p, pg, it, sg.item, empty);
-      else partials[s] = p;
+    if constexpr (RED) {
+      wave_reduce<OP, T, Tacc>(p);
+      if (lane == 0) {
+        if (sg.single) seg_finish<OP, T, Tout, Tacc>(p, pg, it, sg.item, empty);
+        else partials[s] = p;
+      }
     }
   }
 }
@@ -341,9 +373,11 @@ static hipError_t rag_red(const RagLaunch &L, hipStream_t s) {
   else
     hipLaunchKernelGGL((rag_reduce<T, OP, Tout, Tacc, 1, false>), dim3((unsigned)blocks), dim3(256), 0, s, L.prog,
                        L.items, L.segs, L.nseg, part, L.empty);
-  if (L.nmulti > 0)
-    hipLaunchKernelGGL((rag_combine<T, OP, Tout, Tacc>), dim3((unsigned)((L.nmulti + 3) / 4)), dim3(256), 0, s,
-                       L.prog, L.items, L.multi, L.nmulti, (const Part<T, Tacc> *)part, L.empty);
+  if constexpr (OP != RAG_NOREDUCE) {
+    if (L.nmulti > 0)
+      hipLaunchKernelGGL((rag_combine<T, OP, Tout, Tacc>), dim3((unsigned)((L.nmulti + 3) / 4)), dim3(256), 0, s,
+                         L.prog, L.items, L.multi, L.nmulti, (const Part<T, Tacc> *)part, L.empty);
+  }
   return hipGetLastError();
 }
 
@@ -351,6 +385,7 @@ template <class T>
 static hipError_t rag_red_T(int op, const RagLaunch &L, hipStream_t s) {
   typedef typename IPlus<T>::O TI;
   switch (op) {
+  case RAG_NOREDUCE: return rag_red<T, RAG_NOREDUCE, T, T>(L, s);
   case R_SUM: return rag_red<T, R_SUM, TI, TI>(L, s);
   case R_DSUM: return rag_red<T, R_DSUM, double, double>(L, s);
   case R_AVERAGE: return rag_red<T, R_AVERAGE, TI, TI>(L, s);

@@ -335,17 +335,31 @@ def test_matmult_pdl_order_bit_exact(t):
         C.set_matmult_precision("split16")
 
 
-@pytest.mark.parametrize("mode", ["fp32", "split16"])
-def test_matmult_float_modes_accuracy(mode):
-    rng = np.random.default_rng(7)
+# relative Frobenius error against the FP64 product, per row of C (rows of A
+# are scaled over many decades, so a whole-matrix norm would hide small rows)
+MM_TOL = {"fp32": 4e-6, "split16": 4e-6, "fp8": 0.08}
+
+
+@pytest.mark.parametrize("mode", ["fp32", "split16", "fp8"])
+@pytest.mark.parametrize("shape", [(300, 257, 190), (600, 100, 130), (1030, 64, 333)],
+                         ids=["128-row tiles", "256-row tiles", "256-row tiles ragged"])
+@pytest.mark.parametrize("layout", ["dense", "a transposed"])
+def test_matmult_float_modes_accuracy(mode, shape, layout):
+    h, k, w = shape
+    rng = np.random.default_rng(7 + h)
     C.set_matmult_precision(mode)
     try:
-        a = rng.standard_normal((300, 257)).astype(np.float32)
-        b = rng.standard_normal((257, 190)).astype(np.float32)
-        d = C.matmult(C.ndarray.from_numpy(a), C.ndarray.from_numpy(b)).numpy().astype(np.float64)
+        a = rng.standard_normal((h, k)).astype(np.float32)
+        a *= np.exp(3.0 * rng.standard_normal((h, 1))).astype(np.float32)     # rows over ~25 decades
+        b = rng.standard_normal((k, w)).astype(np.float32)
+        if layout == "dense":
+            da = C.ndarray.from_numpy(a)
+        else:   # a(t,h) as the transpose of a dense (t,h) array: its k index is strided
+            da = C.ndarray.from_numpy(np.ascontiguousarray(a.T)).xchg(0, 1)
+        d = C.matmult(da, C.ndarray.from_numpy(b)).numpy().astype(np.float64)
         truth = a.astype(np.float64) @ b.astype(np.float64)
-        err = np.linalg.norm(d - truth) / np.linalg.norm(truth)
-        assert err < 2e-6, err
+        err = np.linalg.norm(d - truth, axis=1) / np.linalg.norm(truth, axis=1)
+        assert err.max() < MM_TOL[mode], (mode, shape, layout, err.max())
     finally:
         C.set_matmult_precision("split16")
 

@@ -248,3 +248,82 @@ def test_graph_replays_equal_eager_runs():
     g.destroy()
     rs.free()
     rc.free()
+
+
+# ------------------------------------------------------------------ dense chains
+def test_chain_c_equals_a_times_b_plus_a_with_sumover_in_one_pass():
+    """c = a*b + a and s = sumover(c) (double, 2048 x 300): c bit for bit as
+    PDL's two eager ops give it, s within the compensated-sum bound"""
+    rng = np.random.default_rng(51)
+    a = rng.uniform(-1, 1, (300, 2048))
+    b = rng.uniform(-1, 1, (300, 2048))
+    ch = C.Chain([C.ndarray.from_numpy(a, O.D), C.ndarray.from_numpy(b, O.D)],
+                 [("mult", 0, 1), ("plus", ("node", 0), 0)], "sumover", value=True)
+    ch.run()
+    oa, ob = O.OPdl.from_numpy(a, O.D), O.OPdl.from_numpy(b, O.D)
+    oc = eager_chain(oracle_run, [oa, ob], [("mult", 0, 1), ("plus", ("node", 0), 0)], O.OPdl.null)
+    assert ch.value.dims == [2048, 300] and ch.result.dims == [300]
+    assert ch.value.numpy().tobytes() == oc.gather().tobytes()
+    x = oc.gather().astype(np.longdouble)
+    truth = x.sum(axis=-1).astype(np.float64)
+    bound = sum_bound(truth, np.abs(x).sum(axis=-1).astype(np.float64), O.D)
+    assert np.all(np.abs(ch.result.numpy() - truth) <= bound)
+
+
+@pytest.mark.parametrize("final", ["minimum", "maximum", "none"])
+def test_chain_broadcasts_like_the_eager_ops(final):
+    """a (PDL dims 64,50,3, an xchg view) - b (64,1,3: a size-1 dim) times a
+    0-dim Perl integer 2: values and min/max bit for bit with the oracle's
+    eager ops"""
+    rng = np.random.default_rng(52)
+    a = rng.standard_normal((3, 64, 50)).astype(np.float32)      # PDL dims (50,64,3) -> xchg(0,1) -> (64,50,3)
+    b = rng.standard_normal((3, 1, 64)).astype(np.float32)       # PDL dims (64,1,3)
+    a[1, 5, :] = np.nan                                          # a NaN column of the view
+    w = np.array(2, np.int8)
+    chain = [("minus", 0, 1), ("mult", ("node", 0), 2)]
+    da = C.ndarray.from_numpy(a, O.F).xchg(0, 1)
+    db = C.ndarray.from_numpy(b, O.F)
+    dw = C.topdl(2)
+    ch = C.Chain([da, db, dw], chain, final, value=True)
+    ch.run()
+    flag = ch.badflags()
+    oa = O.OPdl.from_numpy(a, O.F).xchg(0, 1)
+    ob = O.OPdl.from_numpy(b, O.F)
+    ow = O.OPdl.from_numpy(w, O.SB)
+    ov = eager_chain(oracle_run, [oa, ob, ow], chain, O.OPdl.null)
+    assert ch.value.dims == list(ov.dims) and ch.value.dtype == ov.dtype
+    np.testing.assert_array_equal(ch.value.numpy(), ov.gather())
+    if final != "none":
+        out = O.OPdl.null()
+        O.run(final, ov, out)
+        np.testing.assert_array_equal(ch.result.numpy(), out.gather())
+        assert flag == bool(out.state & O.BADVAL)
+
+
+def test_chain_integer_scan_rows():
+    rng = np.random.default_rng(53)
+    x = rng.integers(-2**31, 2**31 - 1, size=(37, 5000), dtype=np.int32)
+    ch = C.Chain([C.ndarray.from_numpy(x, O.L)], [], "cumusumover")
+    ch.run()
+    out = O.OPdl.null()
+    O.run("cumusumover", O.OPdl.from_numpy(x, O.L), out)
+    np.testing.assert_array_equal(ch.result.numpy(), out.gather())
+
+
+def test_chain_graph_replays_equal_eager_runs():
+    rng = np.random.default_rng(54)
+    a = C.ndarray.from_numpy(rng.uniform(-1, 1, (64, 3000)), O.D)
+    b = C.ndarray.from_numpy(rng.uniform(-1, 1, (64, 3000)), O.D)
+    ch = C.Chain([a, b], [("mult", 0, 1), ("plus", ("node", 0), 0)], "sumover", value=True)
+    g = C.Graph(ch.run)
+    for rep in range(4):
+        C._run("plus", [a, C.topdl(0.5), a])
+        C.synchronize()
+        g.launch()
+        C.synchronize()
+        v, s = ch.value.numpy().copy(), ch.result.numpy().copy()
+        ch.run()
+        C.synchronize()
+        assert v.tobytes() == ch.value.numpy().tobytes(), rep
+        assert s.tobytes() == ch.result.numpy().tobytes(), rep
+    g.destroy()

@@ -67,3 +67,20 @@ def test_refusals(plan_only):
     expect("wrong shape", [[f4]], [], "sumover", outputs=[C.shell(O.F, [2])])
     expect("node 0 operand", [[f4]], [("plus", 0, 3)], "sumover")
     expect("node 0 operand", [[f4]], [("plus", 0, ("node", 0))], "sumover")
+
+
+def test_chain_shapes_follow_broadcasting(plan_only):
+    a = C.shell(O.F, [64, 50, 3])
+    b = C.shell(O.F, [64, 1, 3])
+    w = C.shell(O.SB, [])
+    ch = C.Chain([a, b, w], [("minus", 0, 1), ("mult", ("node", 0), 2)], "sumover", value=True)
+    assert ch.value.dims == [64, 50, 3] and ch.value.dtype == O.F
+    assert ch.result.dims == [50, 3] and ch.result.dtype == O.F
+    ch = C.Chain([C.shell(O.L, [7, 4])], [], "cumusumover")
+    assert ch.result.dims == [7, 4] and ch.result.dtype == O.L
+    with pytest.raises(PDLError) as e:
+        C.Chain([C.shell(O.F, [4, 5]), C.shell(O.F, [4, 6])], [("plus", 0, 1)], "sumover")
+    assert "mismatched broadcast dim 1: size 5 vs 6" in str(e.value)
+    with pytest.raises(PDLError) as e:
+        C.Chain([C.shell(O.F, [4])], [], "none")
+    assert "needs a value output" in str(e.value)
