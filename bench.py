@@ -208,7 +208,8 @@ def run_small(reps=20):
     double (512,512) x (512,512) drawn next from the same stream.  Device ms
     per op (HIP events) next to PDL's CPU path (the oracle, autopthread on
     every host core); c must match bit for bit, s and m are scored by their
-    relative error against a long-double truth, device and CPU alike."""
+    relative error against a long-double truth, device and CPU alike.  The
+    chain is also run fused (c and s from one pass over a and b)."""
     from pdl_amd import core as C
     from oracle import pdlcpu as O
 
@@ -224,6 +225,10 @@ def run_small(reps=20):
     ops = (("a*b+a", lambda: (C._run("mult", [a, b, t]), C._run("plus", [t, a, c]))),
            ("sumover", lambda: C._run("sumover", [c, s])),
            ("matmult", lambda: C._run("matmult", [ma, mb, m])))
+    # the same c and s from one fused pass (pdlhip_chain_prepare): reads a, b once
+    fc, fs = C.ndarray.new_dense(D, [N, N]), C.ndarray.new_dense(D, [N])
+    ch = C.Chain([a, b], [("mult", 0, 1), ("plus", ("node", 0), 0)], "sumover", value=fc, result=fs)
+    ops = ops + (("fused a*b+a -> c, sumover(c) -> s", ch.run),)
     gpu = {}
     for name, f in ops:
         f()
@@ -258,6 +263,8 @@ def run_small(reps=20):
     return {"workload": "a,b double (2048,2048) U[-1,1] seed 0: c=a*b+a, s=sumover(c); matmult double 512^2",
             "gpu_ms": gpu, "cpu_ms": cpu, "cpu_threads": nthr,
             "c_bit_exact": bool(np.array_equal(c.numpy().view(np.uint64), cg.view(np.uint64))),
+            "fused_c_bit_exact": bool(np.array_equal(fc.numpy().view(np.uint64), cg.view(np.uint64))),
+            "fused_s_rel_err": rel(fs.numpy(), s_truth),
             "s_rel_err": {"gpu": rel(s.numpy(), s_truth), "pdl_cpu": rel(os_.gather(), s_truth)},
             "m_rel_err": {"gpu": rel(m.numpy(), m_truth), "pdl_cpu": rel(om.gather(), m_truth)}}
 

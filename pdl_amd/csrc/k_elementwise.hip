@@ -168,56 +168,79 @@ __global__ __launch_bounds__(EW_THREADS) void ew_tile(const T *__restrict__ a, c
     }
   }
   __syncthreads();
-  // compute + store: lanes along i, V elements each
+  // compute + store: lanes along i, V elements each.  Pointers advance by a
+  // fixed step per row of the tile; a directly read input that does not vary
+  // along j (a dummy dim, increment 0) is loaded once.
   constexpr int LI = TI / V;
   constexpr int LC = EW_THREADS / LI;
   const int ii = (threadIdx.x % LI) * V;
   const int jj0 = threadIdx.x / LI;
   const int64_t gi = i0 + ii;
+  if (V > 1 && full) {
+    const int64_t cstep = LC * p.inc[2][q];
+    T *cp = c + o[2] + gi + (j0 + jj0) * p.inc[2][q];
+    const T *dp[2] = {nullptr, nullptr};
+    int64_t dstep[2] = {0, 0};
+    T dv[2][V];
 #pragma unroll
-  for (int jj = jj0; jj < TJ; jj += LC) {
-    const int64_t gj = j0 + jj;
-    T x[V], y[V], r[V];
+    for (int k = 0; k < 2; k++) {
+      if (TMASK & (1 << k)) continue;
+      dp[k] = src[k] + o[k] + gi * p.inc[k][0] + (j0 + jj0) * p.inc[k][q];
+      dstep[k] = LC * p.inc[k][q];
+      if (dstep[k] == 0) {
+        if (p.inc[k][0] == 0) {
 #pragma unroll
-    for (int e = 0; e < V; e++) {
-      if constexpr (TMASK & 1) x[e] = lds[0][ii + e][jj];
-      if constexpr (TMASK & 2) y[e] = lds[(TMASK & 1) ? 1 : 0][ii + e][jj];
-    }
-    if (V > 1 && full) {
-      if constexpr (!(TMASK & 1)) {
-        const T *ap = a + o[0] + gi * p.inc[0][0] + gj * p.inc[0][q];
-        if (p.inc[0][0] == 0) {
-#pragma unroll
-          for (int e = 0; e < V; e++) x[e] = ap[0];
+          for (int e = 0; e < V; e++) dv[k][e] = dp[k][0];
         } else {
-          Vec<T, V> v = *reinterpret_cast<const Vec<T, V> *>(ap);
+          const Vec<T, V> v = *reinterpret_cast<const Vec<T, V> *>(dp[k]);
 #pragma unroll
-          for (int e = 0; e < V; e++) x[e] = v.v[e];
+          for (int e = 0; e < V; e++) dv[k][e] = v.v[e];
         }
       }
-      if constexpr (!(TMASK & 2)) {
-        const T *bp = b + o[1] + gi * p.inc[1][0] + gj * p.inc[1][q];
-        if (p.inc[1][0] == 0) {
+    }
 #pragma unroll
-          for (int e = 0; e < V; e++) y[e] = bp[0];
+    for (int jj = jj0; jj < TJ; jj += LC) {
+      T x[V], y[V];
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        T *dst = k == 0 ? x : y;
+        if (TMASK & (1 << k)) {
+          const int slot = (k == 1 && (TMASK & 1)) ? 1 : 0;
+#pragma unroll
+          for (int e = 0; e < V; e++) dst[e] = lds[slot][ii + e][jj];
+        } else if (dstep[k] == 0) {
+#pragma unroll
+          for (int e = 0; e < V; e++) dst[e] = dv[k][e];
+        } else if (p.inc[k][0] == 0) {
+#pragma unroll
+          for (int e = 0; e < V; e++) dst[e] = dp[k][0];
+          dp[k] += dstep[k];
         } else {
-          Vec<T, V> v = *reinterpret_cast<const Vec<T, V> *>(bp);
+          const Vec<T, V> v = *reinterpret_cast<const Vec<T, V> *>(dp[k]);
 #pragma unroll
-          for (int e = 0; e < V; e++) y[e] = v.v[e];
+          for (int e = 0; e < V; e++) dst[e] = v.v[e];
+          dp[k] += dstep[k];
         }
       }
       Vec<T, V> out;
 #pragma unroll
       for (int e = 0; e < V; e++) out.v[e] = binop<OP, T>(x[e], y[e]);
-      store_nt<T, V>(c + o[2] + gi + gj * p.inc[2][q], out);
-    } else {
+      store_nt<T, V>(cp, out);
+      cp += cstep;
+    }
+  } else {
+#pragma unroll
+    for (int jj = jj0; jj < TJ; jj += LC) {
+      const int64_t gj = j0 + jj;
+      T x[V], y[V];
 #pragma unroll
       for (int e = 0; e < V; e++) {
         if (gi + e >= ni || gj >= nj) continue;
-        if constexpr (!(TMASK & 1)) x[e] = a[o[0] + (gi + e) * p.inc[0][0] + gj * p.inc[0][q]];
-        if constexpr (!(TMASK & 2)) y[e] = b[o[1] + (gi + e) * p.inc[1][0] + gj * p.inc[1][q]];
-        r[e] = binop<OP, T>(x[e], y[e]);
-        c[o[2] + (gi + e) * p.inc[2][0] + gj * p.inc[2][q]] = r[e];
+        if constexpr (TMASK & 1) x[e] = lds[0][ii + e][jj];
+        else x[e] = a[o[0] + (gi + e) * p.inc[0][0] + gj * p.inc[0][q]];
+        if constexpr (TMASK & 2) y[e] = lds[(TMASK & 1) ? 1 : 0][ii + e][jj];
+        else y[e] = b[o[1] + (gi + e) * p.inc[1][0] + gj * p.inc[1][q]];
+        c[o[2] + (gi + e) * p.inc[2][0] + gj * p.inc[2][q]] = binop<OP, T>(x[e], y[e]);
       }
     }
   }
