@@ -269,16 +269,19 @@ def run_small(reps=20):
             "m_rel_err": {"gpu": rel(m.numpy(), m_truth), "pdl_cpu": rel(om.gather(), m_truth)}}
 
 
-def run_ragged_matmult(count=4096, lo=8, hi=256, seed=0):
+def run_ragged_matmult(count=4096, lo=8, hi=256, seed=0, reps=5):
     """4096 independent square products, n ~ U{8..256}, float32 and float64
-    entries uniform[-1,1]: one persistent work-queue launch vs one launch per
-    product; plus a broadcast (64,64)x(64,64) matmult over 65536."""
+    entries uniform[-1,1]: the list is prepared once (MatmultList: planning,
+    item table, autotuned tile), then timed as one persistent work-queue
+    launch vs one launch per product; plus a broadcast (64,64)x(64,64)
+    matmult over 65536."""
     from pdl_amd import core as C
 
     rng = np.random.default_rng(seed)
     ns = rng.integers(lo, hi, size=count, endpoint=True)
     flops = float(np.sum(2.0 * ns.astype(np.float64) ** 3))
-    out = {"count": count, "n_range": [lo, hi], "flop": flops}
+    out = {"count": count, "n_range": [lo, hi], "flop": flops,
+           "bytes_f32": int(np.sum(3 * ns.astype(np.int64) ** 2) * 4)}
     tot = int(np.sum(ns.astype(np.int64) ** 2))
     for t, name in ((C.L.F, "f32"), (C.L.D, "f64")):
         A = C.random_uniform(tot, dtype=t, seed=41, lo=-1, hi=1)
@@ -290,17 +293,33 @@ def run_ragged_matmult(count=4096, lo=8, hi=256, seed=0):
             As.append(C.view_into(A, [n, n], off)); Bs.append(C.view_into(B, [n, n], off))
             Cs.append(C.view_into(Cb, [n, n], off))
             off += n * n
-        res = {}
-        for mode in ("queue", "each"):
-            C.matmult_list(As, Bs, Cs, mode=mode)          # warm (and autotune)
+        C.synchronize()
+        t0 = time.perf_counter()
+        ml = C.MatmultList(As, Bs, Cs)
+        C.synchronize()
+        res = {"prepare_ms": round((time.perf_counter() - t0) * 1e3, 1)}
+        got = {}
+        for mode, r in (("queue", reps), ("each", 2)):
+            ml.launch(mode)
             C.synchronize()
             e0, e1 = C.Event(), C.Event()
             e0.record()
-            C.matmult_list(As, Bs, Cs, mode=mode)
+            for _ in range(r):
+                ml.launch(mode)
             e1.record()
-            ms = e0.elapsed_ms(e1)
+            ms = e0.elapsed_ms(e1) / r
+            got[mode] = Cb.numpy()
             res[mode] = {"ms": round(ms, 3), "TFLOP/s": round(flops / (ms * 1e-3) / 1e12, 2)}
         res["queue_vs_each"] = round(res["each"]["ms"] / res["queue"]["ms"], 2)
+        res["queue_equals_each"] = bool(np.array_equal(got["queue"], got["each"]))
+        # the one-call form (planning included), as a PDL loop would call it
+        C.matmult_list(As, Bs, Cs, mode="queue")
+        C.synchronize()
+        t0 = time.perf_counter()
+        C.matmult_list(As, Bs, Cs, mode="queue")
+        C.synchronize()
+        res["unprepared_call_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+        ml.free()
         out[name] = res
     # broadcast matmult (64,64) x (64,64) over a broadcast dim of 65536
     a = C.random_uniform(64, 64, 65536, dtype=C.L.F, seed=51, lo=-1, hi=1)

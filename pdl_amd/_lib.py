@@ -115,6 +115,10 @@ EXPORTS["pdlhip_graph_begin"] = ([], Error)
 EXPORTS["pdlhip_graph_end"] = ([ctypes.POINTER(ctypes.c_void_p)], Error)
 EXPORTS["pdlhip_graph_launch"] = ([ctypes.c_void_p], Error)
 EXPORTS["pdlhip_graph_destroy"] = ([ctypes.c_void_p], Error)
+EXPORTS["pdlhip_matmult_list_prepare"] = ([ctypes.c_int, ctypes.POINTER(PP), ctypes.POINTER(PP), ctypes.POINTER(PP),
+                                           ctypes.POINTER(ctypes.c_void_p)], Error)
+EXPORTS["pdlhip_matmult_list_launch"] = ([ctypes.c_void_p, ctypes.c_int], Error)
+EXPORTS["pdlhip_matmult_list_free"] = ([ctypes.c_void_p], Error)
 EXPORTS["pdlhip_run_matmult_list"] = ([ctypes.c_int, ctypes.POINTER(PP), ctypes.POINTER(PP), ctypes.POINTER(PP),
                                       ctypes.c_int], Error)
 

@@ -777,6 +777,49 @@ class Graph:
             pass
 
 
+class MatmultList:
+    """A prepared list of products A_i x B_i (uneven sizes): planned once
+    (outputs created as matmult creates them), then each launch() runs the
+    whole list in one work-queue launch ("queue") or one launch per product
+    ("each") without allocating, so launches can be captured in a Graph."""
+
+    def __init__(self, As, Bs, Cs=None):
+        _need()
+        n = len(As)
+        if len(Bs) != n:
+            raise PDLError(L.EUSERERROR, "MatmultList: %d left and %d right operands" % (n, len(Bs)))
+        self.outputs = [ndarray() for _ in range(n)] if Cs is None else list(Cs)
+        objs = [topdl(a) for a in As] + [topdl(b) for b in Bs]
+        ca = [o._to_c() for o in objs[:n]]
+        cb = [o._to_c() for o in objs[n:]]
+        cc = [c._to_c() for c in self.outputs]
+        created = [c.isnull() for c in self.outputs]
+        pa = (L.PP * max(1, n))(*[ctypes.pointer(x) for x in ca])
+        pb = (L.PP * max(1, n))(*[ctypes.pointer(x) for x in cb])
+        pc = (L.PP * max(1, n))(*[ctypes.pointer(x) for x in cc])
+        h = ctypes.c_void_p()
+        L.check(L.lib().pdlhip_matmult_list_prepare(n, pa, pb, pc, ctypes.byref(h)))
+        self._h = h
+        self._keep = objs
+        for o, s, cr in zip(self.outputs, cc, created):
+            o._from_c(s, cr)
+
+    def launch(self, mode="queue"):
+        L.check(L.lib().pdlhip_matmult_list_launch(self._h, {"queue": 0, "each": 1}[mode]))
+        return self
+
+    def free(self):
+        if self._h:
+            L.check(L.lib().pdlhip_matmult_list_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def view_into(parent, dims, offs):
     """a dense-strided view of `dims` starting at element `offs` of parent's
     root buffer (for packing many small ndarrays into one allocation)"""

@@ -66,6 +66,7 @@ struct Deferred {
   std::vector<MmItem> items;
   int T = -1;
   bool fused = true;
+  int missed = 0;      // list items that had to run at once (bad values, conversions, another type)
 };
 
 struct Global {
@@ -83,6 +84,15 @@ static pdlhip_error ensure_init() {
   if (g.plan_only) return ok();
   if (g.device >= 0) return ok();
   return make_err(PDLHIP_EFATAL, "pdlhip: not initialised (call pdlhip_init)");
+}
+
+// host -> device copy ordered on the library stream (a block from the cache
+// may still be read by work queued before it was released), complete on
+// return so the host buffer may go away
+static hipError_t upload_now(void *dst, const void *src, size_t bytes) {
+  if (!bytes) return hipSuccess;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g.stream);
+  return e == hipSuccess ? hipStreamSynchronize(g.stream) : e;
 }
 
 // ------------------------------------------------------------------ allocator
@@ -696,6 +706,7 @@ static pdlhip_error readdata(const Sig &s, int T, bool bad, const Brc &b, pdlhip
       }
       return ok();
     }
+    if (g.defer) g.defer->missed++;
     e = matmult_dispatch(T, p, pdls[0]->data, pdls[1]->data, pdls[2]->data, bad, bi, g.mm_mode, st);
   }
   if (e != hipSuccess) return hip_err(e, s.name.c_str());
@@ -855,7 +866,10 @@ static pdlhip_error run(const Sig &s, pdlhip_pdl **user, int npdls, const int64_
   Deferred *defer_saved = g.defer;
   bool any_conv = false;
   for (int i = 0; i < npdls; i++) any_conv = any_conv || converted[i];
-  if (any_conv) g.defer = nullptr;
+  if (any_conv && g.defer) {
+    g.defer->missed++;
+    g.defer = nullptr;
+  }
   if (!err.error) err = readdata(s, T, inputs_bad, b, pdls, other);
   g.defer = defer_saved;
   for (int i = s.nparents; i < npdls && !err.error; i++) {
@@ -886,79 +900,117 @@ static int ilog2(int64_t v) {
   return r;
 }
 
-static pdlhip_error run_list(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlhip_pdl **c, int mode) {
-  const Sig *s = find_sig("matmult");
+// A planned matmult list: the items (one product per broadcast position of
+// every list entry) and both tile decompositions in HBM, the tile picked by
+// the autotuner.  A launch allocates nothing.
+struct ListH {
   Deferred d;
-  g.defer = &d;
+  std::vector<int3> work32, work64;
+  MmItem *d_items = nullptr;
+  unsigned int *d_counter = nullptr;
+  int3 *d_w32 = nullptr, *d_w64 = nullptr;
+  int tile = 64;
+};
+
+static void list_release(ListH &h) {
+  dev_free(h.d_items);
+  dev_free(h.d_counter);
+  dev_free(h.d_w32);
+  dev_free(h.d_w64);
+  h.d_items = nullptr; h.d_counter = nullptr; h.d_w32 = nullptr; h.d_w64 = nullptr;
+}
+
+static hipError_t list_launch(const ListH &h, int tile, bool each) {
+  const std::vector<int3> &w = tile == 32 ? h.work32 : h.work64;
+  const int nblocks = (int)std::max<size_t>(std::min<size_t>(w.size(), 2048), 1);
+  return launch_matmult_list(h.d.T, h.d_items, h.d.items.data(), (int)h.d.items.size(), tile == 32 ? h.d_w32 : h.d_w64,
+                             (unsigned)w.size(), h.d_counter, tile, h.d.fused, each, nblocks, g.stream);
+}
+
+// plan every entry through pdl_run_matmult's checks (outputs created as
+// PDL would), then build the work lists; `strict`: refuse entries that would
+// have to run at once instead of joining the list
+static pdlhip_error list_build(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlhip_pdl **c, bool strict, ListH &h) {
+  const Sig *s = find_sig("matmult");
+  g.defer = &h.d;
   pdlhip_error err = ok();
+  int odd = 0;   // entries that cannot join (found from their types and flags, also when only planning)
   for (int i = 0; i < n && !err.error; i++) {
     pdlhip_pdl *v[3] = {a[i], b[i], c[i]};
     err = run(*s, v, 3, nullptr);
+    if (!err.error && (c[i]->datatype != c[0]->datatype || a[i]->datatype != c[i]->datatype ||
+                       b[i]->datatype != c[i]->datatype || ((a[i]->state | b[i]->state) & PDLHIP_BADVAL)))
+      odd++;
   }
   g.defer = nullptr;
-  if (err.error || d.items.empty() || g.plan_only) return err;
-  const int nit = (int)d.items.size();
+  if (err.error) return err;
+  h.d.missed = std::max(h.d.missed, odd);
+  if (strict && h.d.missed)
+    return make_err(PDLHIP_EUSERERROR, "pdlhip_matmult_list_prepare: %d entries need bad values, a type conversion "
+                    "or another type than the first; convert them or run them one by one", h.d.missed);
+  if (h.d.items.empty() || g.plan_only) return ok();
+  const int nit = (int)h.d.items.size();
   int64_t maxdim = 1;
-  for (auto &it : d.items) maxdim = std::max(maxdim, std::max(it.H, it.W));
+  for (auto &it : h.d.items) maxdim = std::max(maxdim, std::max(it.H, it.W));
   // work items, largest inner dimension first
   std::vector<int> order(nit);
   for (int i = 0; i < nit; i++) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return d.items[x].T > d.items[y].T; });
-  MmItem *d_items = (MmItem *)dev_alloc(sizeof(MmItem) * nit);
-  unsigned int *d_counter = (unsigned int *)dev_alloc(sizeof(unsigned int));
-  if (!d_items || !d_counter) return make_err(PDLHIP_EFATAL, "pdlhip: out of device memory");
-  hipError_t e = hipMemcpyAsync(d_items, d.items.data(), sizeof(MmItem) * nit, hipMemcpyHostToDevice, g.stream);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return h.d.items[x].T > h.d.items[y].T; });
   auto build_work = [&](int tile, std::vector<int3> &work) {
     work.clear();
     for (int i : order) {
-      const MmItem &it = d.items[i];
+      const MmItem &it = h.d.items[i];
       int64_t ntm = (it.H + tile - 1) / tile, ntn = (it.W + tile - 1) / tile;
       for (int64_t tm = 0; tm < ntm; tm++)
         for (int64_t tn = 0; tn < ntn; tn++) work.push_back(make_int3(i, (int)tm, (int)tn));
     }
   };
-  std::vector<int3> work32, work64;
-  build_work(32, work32);
-  build_work(64, work64);
-  int3 *d_w32 = (int3 *)dev_alloc(sizeof(int3) * std::max<size_t>(work32.size(), 1));
-  int3 *d_w64 = (int3 *)dev_alloc(sizeof(int3) * std::max<size_t>(work64.size(), 1));
-  if (!d_w32 || !d_w64) return make_err(PDLHIP_EFATAL, "pdlhip: out of device memory");
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(d_w32, work32.data(), sizeof(int3) * work32.size(), hipMemcpyHostToDevice, g.stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(d_w64, work64.data(), sizeof(int3) * work64.size(), hipMemcpyHostToDevice, g.stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(g.stream);   // host vectors may go out of scope after this
-  const bool each = (mode == PDLHIP_LIST_EACH);
-  auto launch = [&](int tile) {
-    const std::vector<int3> &w = tile == 32 ? work32 : work64;
-    int nblocks = (int)std::min<size_t>(w.size(), 2048);
-    return launch_matmult_list(d.T, d_items, d.items.data(), nit, tile == 32 ? d_w32 : d_w64, (unsigned)w.size(),
-                               d_counter, tile, d.fused, each, std::max(nblocks, 1), g.stream);
-  };
-  TuneKey key{d.T, d.fused ? 1 : 0, ilog2(maxdim), ilog2(nit)};
-  int tile = 64;
+  build_work(32, h.work32);
+  build_work(64, h.work64);
+  h.d_items = (MmItem *)dev_alloc(sizeof(MmItem) * nit);
+  h.d_counter = (unsigned int *)dev_alloc(sizeof(unsigned int));
+  h.d_w32 = (int3 *)dev_alloc(sizeof(int3) * std::max<size_t>(h.work32.size(), 1));
+  h.d_w64 = (int3 *)dev_alloc(sizeof(int3) * std::max<size_t>(h.work64.size(), 1));
+  if (!h.d_items || !h.d_counter || !h.d_w32 || !h.d_w64) {
+    list_release(h);
+    return make_err(PDLHIP_EFATAL, "pdlhip: out of device memory");
+  }
+  hipError_t e = upload_now(h.d_items, h.d.items.data(), sizeof(MmItem) * nit);
+  if (e == hipSuccess) e = upload_now(h.d_w32, h.work32.data(), sizeof(int3) * h.work32.size());
+  if (e == hipSuccess) e = upload_now(h.d_w64, h.work64.data(), sizeof(int3) * h.work64.size());
+  // tile by the autotuner: both candidates timed once per shape class
+  TuneKey key{h.d.T, h.d.fused ? 1 : 0, ilog2(maxdim), ilog2(nit)};
   auto it = g_tune.find(key);
   if (it != g_tune.end()) {
-    tile = it->second;
-  } else if (e == hipSuccess && !each) {
+    h.tile = it->second;
+  } else if (e == hipSuccess) {
     float best = 1e30f;
     for (int cand : {32, 64}) {
       (void)hipEventRecord(g.ev0, g.stream);
-      e = launch(cand);
+      e = list_launch(h, cand, false);
       (void)hipEventRecord(g.ev1, g.stream);
       if (e != hipSuccess) break;
       float ms = 0;
       (void)hipEventSynchronize(g.ev1);
       (void)hipEventElapsedTime(&ms, g.ev0, g.ev1);
-      if (ms < best) { best = ms; tile = cand; }
+      if (ms < best) { best = ms; h.tile = cand; }
     }
-    g_tune[key] = tile;
+    if (e == hipSuccess) g_tune[key] = h.tile;
   }
-  if (e == hipSuccess) e = launch(tile);
-  dev_free(d_items);
-  dev_free(d_counter);
-  dev_free(d_w32);
-  dev_free(d_w64);
+  if (e != hipSuccess) {
+    list_release(h);
+    return hip_err(e, "matmult list");
+  }
+  return ok();
+}
+
+static pdlhip_error run_list(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlhip_pdl **c, int mode) {
+  ListH h;
+  RETURN_IF(list_build(n, a, b, c, false, h));
+  if (h.d.items.empty() || g.plan_only) return ok();
+  hipError_t e = list_launch(h, h.tile, mode == PDLHIP_LIST_EACH);
+  // the host item table of a per-product launch is read at launch time only
+  list_release(h);
   return e == hipSuccess ? ok() : hip_err(e, "matmult list");
 }
 
@@ -1150,7 +1202,7 @@ static pdlhip_error ragged_finish(const char *who, RaggedH &h, std::vector<RagIt
   auto up = [&](const void *src, size_t bytes, void **dst) -> bool {
     *dst = dev_alloc(std::max<size_t>(bytes, 1));
     if (!*dst) return false;
-    return !bytes || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    return upload_now(*dst, src, bytes) == hipSuccess;
   };
   bool okk = up(items.data(), sizeof(RagItem) * items.size(), &h.d_items) &&
              up(segs.data(), sizeof(RagSeg) * segs.size(), &h.d_segs) &&
@@ -1511,6 +1563,40 @@ pdlhip_error pdlhip_run_matmult_list(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlh
   return run_list(n, a, b, c, mode);
 }
 
+struct pdlhip_mmlist { pdlhip::ListH h; };
+
+pdlhip_error pdlhip_matmult_list_prepare(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlhip_pdl **c,
+                                         pdlhip_mmlist **handle) {
+  if (!handle || n < 0 || (n > 0 && (!a || !b || !c)))
+    return make_err(PDLHIP_EUSERERROR, "pdlhip_matmult_list_prepare: bad arguments");
+  RETURN_IF(ensure_init());
+  pdlhip_mmlist *l = new pdlhip_mmlist;
+  pdlhip_error e = list_build(n, a, b, c, true, l->h);
+  if (e.error) {
+    delete l;
+    return e;
+  }
+  *handle = l;
+  return ok();
+}
+
+pdlhip_error pdlhip_matmult_list_launch(pdlhip_mmlist *handle, int mode) {
+  if (!handle) return make_err(PDLHIP_EUSERERROR, "pdlhip_matmult_list_launch: no handle");
+  if (mode != PDLHIP_LIST_QUEUE && mode != PDLHIP_LIST_EACH)
+    return make_err(PDLHIP_EUSERERROR, "pdlhip_matmult_list_launch: unknown mode %d", mode);
+  RETURN_IF(ensure_init());
+  if (g.plan_only || handle->h.d.items.empty()) return ok();
+  hipError_t e = list_launch(handle->h, handle->h.tile, mode == PDLHIP_LIST_EACH);
+  return e == hipSuccess ? ok() : hip_err(e, "matmult list");
+}
+
+pdlhip_error pdlhip_matmult_list_free(pdlhip_mmlist *handle) {
+  if (!handle) return ok();
+  list_release(handle->h);
+  delete handle;
+  return ok();
+}
+
 struct pdlhip_ragged { pdlhip::RaggedH h; };
 
 pdlhip_error pdlhip_ragged_prepare(int nitems, int ninputs, pdlhip_pdl **inputs, int nnodes, const pdlhip_node *nodes,
@@ -1558,7 +1644,8 @@ pdlhip_error pdlhip_ragged_badflags(pdlhip_ragged *handle, int32_t *flags) {
   }
   if (g.plan_only || h.n == 0) return ok();
   hipError_t e = hipStreamSynchronize(g.stream);
-  if (e == hipSuccess) e = hipMemcpy(flags, h.d_empty, sizeof(int32_t) * h.n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpyAsync(flags, h.d_empty, sizeof(int32_t) * h.n, hipMemcpyDeviceToHost, g.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
   return e == hipSuccess ? ok() : hip_err(e, "ragged badflags");
 }
 

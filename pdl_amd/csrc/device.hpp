@@ -10,6 +10,15 @@
 
 namespace pdlhip {
 
+// A pointer read from a table in device memory is generic to the compiler,
+// which then emits flat loads/stores and waits for all of them before any
+// use; these casts state that the data is in global memory (it always is
+// here), so the accesses become global_* instructions with counted waits.
+template <class T> using gptr = T __attribute__((address_space(1))) *;
+template <class T> __device__ inline gptr<const T> as_global(const T *p) { return (gptr<const T>)p; }
+template <class T> __device__ inline gptr<T> as_global(T *p) { return (gptr<T>)p; }
+
+
 template <class T> struct Unsigned { typedef T U; };
 template <> struct Unsigned<int8_t> { typedef uint8_t U; };
 template <> struct Unsigned<int16_t> { typedef uint16_t U; };

@@ -34,7 +34,7 @@ template <class D, class S> __device__ inline D conv_to(S v) {
 
 template <class T> __device__ inline T load_as(const char *p, int type, int64_t idx) {
   switch (type) {
-#define X(TC) case TC: return conv_to<T>(((const CType<TC>::T *)p)[idx]);
+#define X(TC) case TC: return conv_to<T>(as_global((const CType<TC>::T *)p)[idx]);
     PDLHIP_FOR_EACH_TYPE(X)
 #undef X
   default: return T(0);
@@ -44,8 +44,8 @@ template <class T> __device__ inline T load_as(const char *p, int type, int64_t 
 // U elements of one slot: element e = base + j*64 (+ the lane's offset in
 // base); all lanes read the same address for a 0-dim slot (inc 0)
 template <class T, class S, int U, bool TAIL>
-__device__ inline void load_u(T *v, const char *p, int64_t inc, int64_t base, int64_t end) {
-  const S *q = (const S *)p;
+__device__ __forceinline__ void load_u(T *v, const char *p, int64_t inc, int64_t base, int64_t end) {
+  const auto q = as_global((const S *)p);
   if (inc == 0) {
     const T x = conv_to<T>(q[0]);
 #pragma unroll
@@ -66,7 +66,7 @@ __device__ inline void load_u(T *v, const char *p, int64_t inc, int64_t base, in
 }
 
 template <class T, int U, bool TAIL>
-__device__ inline void load_slot(T *v, int type, const char *p, int64_t inc, int64_t base, int64_t end) {
+__device__ __forceinline__ void load_slot(T *v, int type, const char *p, int64_t inc, int64_t base, int64_t end) {
   switch (type) {
 #define X(TC) case TC: load_u<T, CType<TC>::T, U, TAIL>(v, p, inc, base, end); break;
     PDLHIP_FOR_EACH_TYPE(X)
@@ -75,7 +75,7 @@ __device__ inline void load_slot(T *v, int type, const char *p, int64_t inc, int
   }
 }
 
-template <int OP, class T, int U> __device__ inline void apply_u(T *acc, const T *x, bool swap) {
+template <int OP, class T, int U> __device__ __forceinline__ void apply_u(T *acc, const T *x, bool swap) {
   if (swap) {
 #pragma unroll
     for (int j = 0; j < U; j++) acc[j] = binop<OP, T>(x[j], acc[j]);
@@ -85,7 +85,7 @@ template <int OP, class T, int U> __device__ inline void apply_u(T *acc, const T
   }
 }
 
-template <class T, int U> __device__ inline void run_op_u(int op, T *acc, const T *x, bool swap) {
+template <class T, int U> __device__ __forceinline__ void run_op_u(int op, T *acc, const T *x, bool swap) {
   switch (op) {
   case OP_PLUS: apply_u<OP_PLUS, T, U>(acc, x, swap); break;
   case OP_MULT: apply_u<OP_MULT, T, U>(acc, x, swap); break;
@@ -142,9 +142,11 @@ template <class T> __device__ inline T chain_value(const RagProg &pg, const RagI
   return v[pg.nnodes - 1];
 }
 
-// U chain values of a left-deep program at elements base + j*64
+// U chain values of a left-deep program at elements base + j*64: each node
+// loads its slot and applies its op (loading every slot up front doubled the
+// registers and made the kernel slower)
 template <class T, int U, bool TAIL>
-__device__ inline void chain_lin(T *acc, const RagProg &pg, const RagItem &it, int64_t base, int64_t end) {
+__device__ __forceinline__ void chain_lin(T *acc, const RagProg &pg, const RagItem &it, int64_t base, int64_t end) {
   load_slot<T, U, TAIL>(acc, pg.type[pg.first], it.ptr[pg.first], it.inc[pg.first], base, end);
   for (int n = 0; n < pg.nnodes; n++) {
     T x[U];
@@ -156,8 +158,8 @@ __device__ inline void chain_lin(T *acc, const RagProg &pg, const RagItem &it, i
 
 // the chain's values at elements base + j*64 into the item's value output
 template <class T, int U, bool TAIL>
-__device__ inline void store_u(const T *v, char *p, int64_t inc, int64_t base, int64_t end) {
-  T *q = (T *)p;
+__device__ __forceinline__ void store_u(const T *v, char *p, int64_t inc, int64_t base, int64_t end) {
+  const auto q = as_global((T *)p);
   if (inc == 1) {
 #pragma unroll
     for (int j = 0; j < U; j++) {
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(256) void rag_reduce(RagProg pg, const RagItem *__r
     } else {
       for (int64_t i = sg.start + lane; i < sg.end; i += 64) {
         const T x = chain_value<T>(pg, it, i);
-        if (pg.store) ((T *)it.val)[i * it.val_inc] = x;
+        if (pg.store) as_global((T *)it.val)[i * it.val_inc] = x;
         if constexpr (RED) part_push<OP, T, Tacc>(p, x, i);
       }
     }
@@ -261,7 +263,7 @@ __device__ inline int rs_pad(int i) { return i + (i >> 4); }
 // tid + 256*k in pf[k]; ident past the item's end
 template <class Tin, class Tout, int IT>
 __device__ inline void scan_fetch(Tout *pf, const RagItem &it, int64_t c0, int tid, Tout ident) {
-  const Tin *ap = (const Tin *)it.ptr[0];
+  const auto ap = as_global((const Tin *)it.ptr[0]);
   const int64_t inc = it.inc[0];
   const int n = (int)(it.len - c0 < RAG_SEG ? it.len - c0 : RAG_SEG);
   if (inc == 1 && n == RAG_SEG) {
@@ -298,7 +300,7 @@ __global__ __launch_bounds__(256) void rag_scan(const RagItem *__restrict__ item
   Tout carry = ident;
   while (q < qend) {
     const RagItem &it = items[order[q]];
-    Tout *bp = (Tout *)it.out;
+    const auto bp = as_global((Tout *)it.out);
     const int64_t binc = it.out_inc;
     const int n = (int)(it.len - c0 < RAG_SEG ? it.len - c0 : RAG_SEG);
     if (c0 == 0) carry = ident;

@@ -182,7 +182,7 @@ struct Outs {
   pdlhip_value bv[4];
 };
 
-template <class T> __device__ inline void put(void *base, int64_t off, T v) { ((T *)base)[off] = v; }
+template <class T> __device__ inline void put(void *base, int64_t off, T v) { as_global((T *)base)[off] = v; }
 
 template <class Tout> __device__ inline Tout avg_div(Tout tmp, int64_t cnt) {
   if constexpr (IsFloat<Tout>::v) return tmp / (Tout)cnt;

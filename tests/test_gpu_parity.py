@@ -400,6 +400,44 @@ def test_matmult_list_matches_single_products(t):
         C.set_matmult_precision("split16")
 
 
+def test_prepared_matmult_list_launches_and_graph_replays():
+    """a prepared list: queue and per-product launches equal the oracle's
+    products (PDL order, bit-exact); a HIP graph of one launch, replayed with
+    the inputs changed in between, equals an eager launch"""
+    rng = np.random.default_rng(13)
+    C.set_matmult_precision("pdlorder")
+    try:
+        sizes = [(int(rng.integers(1, 140)), int(rng.integers(1, 140)), int(rng.integers(1, 140))) for _ in range(30)]
+        As, Bs, ah, bh = [], [], [], []
+        for h, tt, w in sizes:
+            a = rand(O.F, (h, tt), rng)
+            b = rand(O.F, (tt, w), rng)
+            ah.append(a); bh.append(b)
+            As.append(C.ndarray.from_numpy(a, O.F)); Bs.append(C.ndarray.from_numpy(b, O.F))
+        ml = C.MatmultList(As, Bs)
+        for mode in ("queue", "each"):
+            ml.launch(mode)
+            for k in range(len(sizes)):
+                oc = O.OPdl.null()
+                O.run("matmult", O.OPdl.from_numpy(ah[k], O.F), O.OPdl.from_numpy(bh[k], O.F), oc)
+                same(oc, ml.outputs[k], "prepared %s item %d %s" % (mode, k, sizes[k]))
+        g = C.Graph(lambda: ml.launch("queue"))
+        for rep in range(3):
+            C._run("plus", [As[rep], C.topdl(1), As[rep]])
+            C.synchronize()
+            g.launch()
+            C.synchronize()
+            got = [o.numpy().copy() for o in ml.outputs]
+            ml.launch("queue")
+            C.synchronize()
+            for k in range(len(sizes)):
+                assert got[k].tobytes() == ml.outputs[k].numpy().tobytes(), (rep, k)
+        g.destroy()
+        ml.free()
+    finally:
+        C.set_matmult_precision("split16")
+
+
 # ------------------------------------------------------------------ readdata entry
 def test_readdata_entry_runs_a_planned_broadcast():
     """pdlhip_readdata over a broadcast struct built as PDL's redodims would"""

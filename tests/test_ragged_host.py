@@ -84,3 +84,11 @@ def test_chain_shapes_follow_broadcasting(plan_only):
     with pytest.raises(PDLError) as e:
         C.Chain([C.shell(O.F, [4])], [], "none")
     assert "needs a value output" in str(e.value)
+
+
+def test_prepared_matmult_list_refuses_entries_that_cannot_join(plan_only):
+    ok = C.MatmultList([C.shell(O.F, [3, 4]), C.shell(O.F, [5, 2])], [C.shell(O.F, [6, 3]), C.shell(O.F, [7, 5])])
+    assert [o.dims for o in ok.outputs] == [[6, 4], [7, 2]]
+    with pytest.raises(PDLError) as e:
+        C.MatmultList([C.shell(O.F, [3, 4]), C.shell(O.D, [5, 2])], [C.shell(O.F, [6, 3]), C.shell(O.D, [7, 5])])
+    assert "another type than the first" in str(e.value)
