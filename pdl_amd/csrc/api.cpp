@@ -1155,6 +1155,12 @@ static pdlhip_error output_for(const char *who, const char *what, int64_t idx, p
 static pdlhip_error ragged_finish(const char *who, RaggedH &h, std::vector<RagItem> &items, RaggedH **hout) {
   const int n = (int)items.size();
   h.n = n;
+  // streaming access when the chain's inputs are too large to stay cached
+  size_t bytes = 0;
+  for (const RagItem &it : items)
+    for (int k = 0; k < h.L.prog.nin; k++)
+      if (it.inc[k]) bytes += (size_t)it.len * type_size(h.L.prog.type[k]);
+  h.L.prog.stream = streaming_read(bytes) ? 1 : 0;
   std::vector<RagSeg> segs;
   std::vector<int32_t> multi;
   for (int i = 0; i < n; i++) {

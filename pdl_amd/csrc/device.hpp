@@ -147,6 +147,19 @@ template <class T, int VEC> __device__ inline void store_nt(T *p, const Vec<T, V
   }
 }
 
+// 16-byte streaming load
+template <class T, int VEC> __device__ inline Vec<T, VEC> load_nt(const T *p) {
+  Vec<T, VEC> v;
+  if constexpr (sizeof(T) * VEC == 16) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 w = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+    __builtin_memcpy(&v, &w, 16);
+  } else {
+    v = *reinterpret_cast<const Vec<T, VEC> *>(p);
+  }
+  return v;
+}
+
 // offsets of element (i0, row) where `row` enumerates dims 1..nd-1
 struct RowOffsets {
   int64_t o[3];

@@ -22,7 +22,7 @@ constexpr int EW_THREADS = 256;
 // ---------------------------------------------------------------- flat
 template <class T, int OP, int VEC>
 __global__ __launch_bounds__(EW_THREADS) void ew_flat(const T *__restrict__ a, const T *__restrict__ b,
-                                                     T *__restrict__ c, int64_t n, int sa, int sb) {
+                                                     T *__restrict__ c, int64_t n, int sa, int sb, bool stream) {
   const int64_t nvec = n / VEC;
   const int64_t stride = (int64_t)gridDim.x * EW_THREADS;
   const T a0 = sa ? a[0] : T(0), b0 = sb ? b[0] : T(0);
@@ -32,17 +32,18 @@ __global__ __launch_bounds__(EW_THREADS) void ew_flat(const T *__restrict__ a, c
 #pragma unroll
       for (int k = 0; k < VEC; k++) av.v[k] = a0;
     } else {
-      av = reinterpret_cast<const Vec<T, VEC> *>(a)[v];
+      av = stream ? load_nt<T, VEC>(a + v * VEC) : reinterpret_cast<const Vec<T, VEC> *>(a)[v];
     }
     if (sb) {
 #pragma unroll
       for (int k = 0; k < VEC; k++) bv.v[k] = b0;
     } else {
-      bv = reinterpret_cast<const Vec<T, VEC> *>(b)[v];
+      bv = stream ? load_nt<T, VEC>(b + v * VEC) : reinterpret_cast<const Vec<T, VEC> *>(b)[v];
     }
 #pragma unroll
     for (int k = 0; k < VEC; k++) cv.v[k] = binop<OP, T>(av.v[k], bv.v[k]);
-    store_nt<T, VEC>(c + v * VEC, cv);
+    if (stream) store_nt<T, VEC>(c + v * VEC, cv);
+    else reinterpret_cast<Vec<T, VEC> *>(c)[v] = cv;
   }
   // tail
   int64_t t = nvec * VEC + (int64_t)blockIdx.x * EW_THREADS + threadIdx.x;
@@ -119,7 +120,7 @@ template <> struct TileShape<uint32_t> { static constexpr int I = 32, J = 128; }
 template <class T, int OP, int TMASK, int V, int TI = TileShape<T>::I, int TJ = TileShape<T>::J>
 __global__ __launch_bounds__(EW_THREADS) void ew_tile(const T *__restrict__ a, const T *__restrict__ b,
                                                      T *__restrict__ c, EwPlan p, int q, int64_t ntile_i,
-                                                     int64_t ntile_j) {
+                                                     int64_t ntile_j, bool stream) {
   constexpr int NL = ((TMASK & 1) ? 1 : 0) + ((TMASK & 2) ? 1 : 0);
   __shared__ T lds[NL][TI][TJ + 1];
   int64_t blk = blockIdx.x;
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(EW_THREADS) void ew_tile(const T *__restrict__ a, c
         const int64_t gi = i0 + ii, gj = j0 + jj;
         const T *sp = src[k] + o[k] + gi * inc_i + gj;
         if (V > 1 && full) {
-          Vec<T, V> v = *reinterpret_cast<const Vec<T, V> *>(sp);
+          Vec<T, V> v = stream ? load_nt<T, V>(sp) : *reinterpret_cast<const Vec<T, V> *>(sp);
 #pragma unroll
           for (int e = 0; e < V; e++) lds[slot][ii][jj + e] = v.v[e];
         } else {
@@ -225,7 +226,8 @@ __global__ __launch_bounds__(EW_THREADS) void ew_tile(const T *__restrict__ a, c
       Vec<T, V> out;
 #pragma unroll
       for (int e = 0; e < V; e++) out.v[e] = binop<OP, T>(x[e], y[e]);
-      store_nt<T, V>(cp, out);
+      if (stream) store_nt<T, V>(cp, out);
+      else *reinterpret_cast<Vec<T, V> *>(cp) = out;
       cp += cstep;
     }
   } else {
@@ -274,12 +276,13 @@ static hipError_t binop_T_OP(const EwPlan &p, const void *va, const void *vb, vo
       int64_t nthreads_needed = p.n / (aligned ? VEC : 1);
       int grid = grid_for(nthreads_needed, EW_THREADS * 4);
       if (grid < 256 * 4 && nthreads_needed > EW_THREADS * 256 * 4) grid = 1024;
+      const bool stream = streaming_read((size_t)p.n * sizeof(T) * 3);
       if (aligned)
         hipLaunchKernelGGL((ew_flat<T, OP, VEC>), dim3(grid), dim3(EW_THREADS), 0, s, pa, pb, pc, p.n,
-                           p.inc[0][0] == 0, p.inc[1][0] == 0);
+                           p.inc[0][0] == 0, p.inc[1][0] == 0, stream);
       else
         hipLaunchKernelGGL((ew_flat<T, OP, 1>), dim3(grid), dim3(EW_THREADS), 0, s, pa, pb, pc, p.n,
-                           p.inc[0][0] == 0, p.inc[1][0] == 0);
+                           p.inc[0][0] == 0, p.inc[1][0] == 0, stream);
       return hipGetLastError();
     }
     // tile: output unit stride on dim 0; a transposed input has its unit stride on dim q
@@ -309,8 +312,9 @@ static hipError_t binop_T_OP(const EwPlan &p, const void *va, const void *vb, vo
           }
         }
         if (nblk < (int64_t)1 << 31) {
+          const bool stream = streaming_read((size_t)p.n * sizeof(T) * 2);
 #define TILE_LAUNCH(TM, VV) hipLaunchKernelGGL((ew_tile<T, OP, TM, VV>), dim3((unsigned)nblk), dim3(EW_THREADS), 0, s, \
-                                               a, b, c, p, q, nti, ntj)
+                                               a, b, c, p, q, nti, ntj, stream)
           if (vec) {
             if (tmask == 1) TILE_LAUNCH(1, V4); else if (tmask == 2) TILE_LAUNCH(2, V4); else TILE_LAUNCH(3, V4);
           } else {

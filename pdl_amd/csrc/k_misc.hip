@@ -34,12 +34,15 @@ __global__ __launch_bounds__(256) void k_convert(const S *__restrict__ src, D *_
 }
 
 // contiguous same-type copy (the common assgn of dense data); 16-byte
-// vectors with streaming stores when both ends are 16-byte aligned
+// vectors when both ends are 16-byte aligned, streaming loads and stores for
+// copies too large for the infinity cache
 template <class T, int VEC>
-__global__ __launch_bounds__(256) void k_copy_flat(const T *__restrict__ src, T *__restrict__ dst, int64_t n) {
+__global__ __launch_bounds__(256) void k_copy_flat(const T *__restrict__ src, T *__restrict__ dst, int64_t n,
+                                                   bool stream) {
   const int64_t nv = n / VEC, stride = (int64_t)gridDim.x * 256;
   for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < nv; v += stride)
-    store_nt<T, VEC>(dst + v * VEC, reinterpret_cast<const Vec<T, VEC> *>(src)[v]);
+    if (stream) store_nt<T, VEC>(dst + v * VEC, load_nt<T, VEC>(src + v * VEC));
+    else reinterpret_cast<Vec<T, VEC> *>(dst)[v] = reinterpret_cast<const Vec<T, VEC> *>(src)[v];
   for (int64_t e = nv * VEC + (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += stride) dst[e] = src[e];
 }
 
@@ -58,9 +61,10 @@ static hipError_t convert_DS(const EwPlan &p, const void *src, void *dst, bool b
     if ((uintptr_t)sp % 16 == 0 && (uintptr_t)dp % 16 == 0) {
       int64_t vb = (p.n / VEC + 255) / 256;
       vb = vb < 1 ? 1 : (vb > 65536 ? 65536 : vb);
-      hipLaunchKernelGGL((k_copy_flat<D, VEC>), dim3((unsigned)vb), dim3(256), 0, s, sp, dp, p.n);
+      hipLaunchKernelGGL((k_copy_flat<D, VEC>), dim3((unsigned)vb), dim3(256), 0, s, sp, dp, p.n,
+                         streaming_read((size_t)p.n * sizeof(D) * 2));
     } else {
-      hipLaunchKernelGGL((k_copy_flat<D, 1>), dim3((unsigned)blocks), dim3(256), 0, s, sp, dp, p.n);
+      hipLaunchKernelGGL((k_copy_flat<D, 1>), dim3((unsigned)blocks), dim3(256), 0, s, sp, dp, p.n, false);
     }
     return hipGetLastError();
   }

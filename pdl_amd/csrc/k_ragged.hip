@@ -44,13 +44,13 @@ template <class T> __device__ inline T load_as(const char *p, int type, int64_t 
 // U elements of one slot: element e = base + j*64 (+ the lane's offset in
 // base); all lanes read the same address for a 0-dim slot (inc 0)
 template <class T, class S, int U, bool TAIL>
-__device__ __forceinline__ void load_u(T *v, const char *p, int64_t inc, int64_t base, int64_t end) {
+__device__ __forceinline__ void load_u(T *v, const char *p, int64_t inc, int64_t base, int64_t end, bool stream) {
   const auto q = as_global((const S *)p);
   if (inc == 0) {
     const T x = conv_to<T>(q[0]);
 #pragma unroll
     for (int j = 0; j < U; j++) v[j] = x;
-  } else if (inc == 1) {
+  } else if (inc == 1 && stream) {
 #pragma unroll
     for (int j = 0; j < U; j++) {
       const int64_t e = base + (int64_t)j * 64;
@@ -66,9 +66,10 @@ __device__ __forceinline__ void load_u(T *v, const char *p, int64_t inc, int64_t
 }
 
 template <class T, int U, bool TAIL>
-__device__ __forceinline__ void load_slot(T *v, int type, const char *p, int64_t inc, int64_t base, int64_t end) {
+__device__ __forceinline__ void load_slot(T *v, int type, const char *p, int64_t inc, int64_t base, int64_t end,
+                                          bool stream) {
   switch (type) {
-#define X(TC) case TC: load_u<T, CType<TC>::T, U, TAIL>(v, p, inc, base, end); break;
+#define X(TC) case TC: load_u<T, CType<TC>::T, U, TAIL>(v, p, inc, base, end, stream); break;
     PDLHIP_FOR_EACH_TYPE(X)
 #undef X
   default: break;
@@ -147,20 +148,20 @@ template <class T> __device__ inline T chain_value(const RagProg &pg, const RagI
 // registers and made the kernel slower)
 template <class T, int U, bool TAIL>
 __device__ __forceinline__ void chain_lin(T *acc, const RagProg &pg, const RagItem &it, int64_t base, int64_t end) {
-  load_slot<T, U, TAIL>(acc, pg.type[pg.first], it.ptr[pg.first], it.inc[pg.first], base, end);
+  load_slot<T, U, TAIL>(acc, pg.type[pg.first], it.ptr[pg.first], it.inc[pg.first], base, end, pg.stream != 0);
   for (int n = 0; n < pg.nnodes; n++) {
     T x[U];
     const int s = pg.lslot[n];
-    load_slot<T, U, TAIL>(x, pg.type[s], it.ptr[s], it.inc[s], base, end);
+    load_slot<T, U, TAIL>(x, pg.type[s], it.ptr[s], it.inc[s], base, end, pg.stream != 0);
     run_op_u<T, U>(pg.node[n].op, acc, x, pg.lswap[n] != 0);
   }
 }
 
 // the chain's values at elements base + j*64 into the item's value output
 template <class T, int U, bool TAIL>
-__device__ __forceinline__ void store_u(const T *v, char *p, int64_t inc, int64_t base, int64_t end) {
+__device__ __forceinline__ void store_u(const T *v, char *p, int64_t inc, int64_t base, int64_t end, bool stream) {
   const auto q = as_global((T *)p);
-  if (inc == 1) {
+  if (inc == 1 && stream) {
 #pragma unroll
     for (int j = 0; j < U; j++) {
       const int64_t e = base + (int64_t)j * 64;
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(256) void rag_reduce(RagProg pg, const RagItem *__r
       for (; b + STEP <= sg.end; b += STEP) {
         T v[U];
         chain_lin<T, U, false>(v, pg, it, b + lane, sg.end);
-        if (pg.store) store_u<T, U, false>(v, it.val, it.val_inc, b + lane, sg.end);
+        if (pg.store) store_u<T, U, false>(v, it.val, it.val_inc, b + lane, sg.end, pg.stream != 0);
         if constexpr (RED) {
 #pragma unroll
           for (int j = 0; j < U; j++) part_push<OP, T, Tacc>(p, v[j], b + lane + (int64_t)j * 64);
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(256) void rag_reduce(RagProg pg, const RagItem *__r
       if (b < sg.end) {
         T v[U];
         chain_lin<T, U, true>(v, pg, it, b + lane, sg.end);
-        if (pg.store) store_u<T, U, true>(v, it.val, it.val_inc, b + lane, sg.end);
+        if (pg.store) store_u<T, U, true>(v, it.val, it.val_inc, b + lane, sg.end, pg.stream != 0);
         if constexpr (RED) {
 #pragma unroll
           for (int j = 0; j < U; j++) {

@@ -49,15 +49,21 @@ constexpr int RT = 256;  // threads per block (4 waves)
 
 template <int OP, class Tin, class Tout, class Tacc, bool BAD, int VEC>
 __device__ inline void row_segment(Part<Tin, Tacc> &pt, const Tin *__restrict__ a, int64_t o, int64_t begin,
-                                   int64_t end, int64_t inc, int lane, const BadInfo &ba) {
+                                   int64_t end, int64_t inc, int lane, const BadInfo &ba, bool stream) {
   const Tin abv = bv_as<Tin>(ba.bv);
   int64_t i = begin + (int64_t)lane * VEC;
   if constexpr (VEC > 1) {
     // begin is a multiple of VEC and the row start is 16-byte aligned; two
     // vectors in flight per lane, consumed in increasing index order
     for (; i + 64 * VEC + VEC <= end; i += 128 * VEC) {
-      Vec<Tin, VEC> x0 = *reinterpret_cast<const Vec<Tin, VEC> *>(a + o + i);
-      Vec<Tin, VEC> x1 = *reinterpret_cast<const Vec<Tin, VEC> *>(a + o + i + 64 * VEC);
+      Vec<Tin, VEC> x0, x1;
+      if (stream) {
+        x0 = load_nt<Tin, VEC>(a + o + i);
+        x1 = load_nt<Tin, VEC>(a + o + i + 64 * VEC);
+      } else {
+        x0 = *reinterpret_cast<const Vec<Tin, VEC> *>(a + o + i);
+        x1 = *reinterpret_cast<const Vec<Tin, VEC> *>(a + o + i + 64 * VEC);
+      }
 #pragma unroll
       for (int k = 0; k < VEC; k++) {
         if constexpr (BAD) { if (isbad(x0.v[k], abv, ba.isnan)) continue; }
@@ -70,7 +76,7 @@ __device__ inline void row_segment(Part<Tin, Tacc> &pt, const Tin *__restrict__ 
       }
     }
     for (; i + VEC <= end; i += 64 * VEC) {
-      Vec<Tin, VEC> x = *reinterpret_cast<const Vec<Tin, VEC> *>(a + o + i);
+      Vec<Tin, VEC> x = stream ? load_nt<Tin, VEC>(a + o + i) : *reinterpret_cast<const Vec<Tin, VEC> *>(a + o + i);
 #pragma unroll
       for (int k = 0; k < VEC; k++) {
         if constexpr (BAD) { if (isbad(x.v[k], abv, ba.isnan)) continue; }
@@ -93,7 +99,8 @@ __device__ inline void row_segment(Part<Tin, Tacc> &pt, const Tin *__restrict__ 
 }
 
 template <int OP, class Tin, class Tout, class Tacc, bool BAD, int VEC>
-__global__ __launch_bounds__(RT) void red_wave(const Tin *__restrict__ a, RedPlan p, Outs o, BadInfo ba) {
+__global__ __launch_bounds__(RT) void red_wave(const Tin *__restrict__ a, RedPlan p, Outs o, BadInfo ba,
+                                               bool stream) {
   const int lane = threadIdx.x & 63;
   const int64_t nw = (int64_t)gridDim.x * (RT / 64);
   for (int64_t e = (int64_t)blockIdx.x * (RT / 64) + (threadIdx.x >> 6); e < p.nout; e += nw) {
@@ -101,7 +108,7 @@ __global__ __launch_bounds__(RT) void red_wave(const Tin *__restrict__ a, RedPla
     out_offsets(p, e, off);
     Part<Tin, Tacc> pt;
     part_init<OP, Tin, Tacc>(pt);
-    row_segment<OP, Tin, Tout, Tacc, BAD, VEC>(pt, a, off[0], 0, p.n, p.inc_n[0], lane, ba);
+    row_segment<OP, Tin, Tout, Tacc, BAD, VEC>(pt, a, off[0], 0, p.n, p.inc_n[0], lane, ba, stream);
     wave_reduce<OP, Tin, Tacc>(pt);
     if (lane == 0) part_store<OP, Tin, Tout, Tacc>(pt, o, off, BAD);
   }
@@ -111,7 +118,7 @@ __global__ __launch_bounds__(RT) void red_wave(const Tin *__restrict__ a, RedPla
 // stage 1: block (e, k) reduces chunk k of row e into parts[e * nchunk + k]
 template <int OP, class Tin, class Tout, class Tacc, bool BAD, int VEC>
 __global__ __launch_bounds__(RT) void red_split1(const Tin *__restrict__ a, RedPlan p, int64_t nchunk, int64_t clen,
-                                                 Part<Tin, Tacc> *parts, BadInfo ba) {
+                                                 Part<Tin, Tacc> *parts, BadInfo ba, bool stream) {
   __shared__ Part<Tin, Tacc> sh[RT / 64];
   const int64_t e = blockIdx.x / nchunk, k = blockIdx.x % nchunk;
   int64_t off[MAXP];
@@ -124,7 +131,7 @@ __global__ __launch_bounds__(RT) void red_split1(const Tin *__restrict__ a, RedP
   const int64_t sub = ((ce - cb) + 3) / 4;
   const int64_t subr = ((sub + VEC - 1) / VEC) * VEC;
   const int64_t sb = cb + w * subr, se = (sb + subr < ce) ? sb + subr : ce;
-  if (sb < se) row_segment<OP, Tin, Tout, Tacc, BAD, VEC>(pt, a, off[0], sb, se, p.inc_n[0], lane, ba);
+  if (sb < se) row_segment<OP, Tin, Tout, Tacc, BAD, VEC>(pt, a, off[0], sb, se, p.inc_n[0], lane, ba, stream);
   wave_reduce<OP, Tin, Tacc>(pt);
   if (lane == 0) sh[w] = pt;
   __syncthreads();
@@ -324,6 +331,7 @@ static hipError_t red_launch(const RedPlan &p, const Tin *a, const Outs &o, cons
       constexpr int VEC = 16 / sizeof(Tin);
       bool aligned = ((uintptr_t)(a + p.off[0]) % 16 == 0);
       for (int d = 0; d < p.nd; d++) aligned = aligned && (p.inc[0][d] % VEC == 0);
+      const bool stream = streaming_read((size_t)p.n * (size_t)p.nout * sizeof(Tin));
       if (p.nout < 1024 && p.n >= 65536) {
         // long rows, few outputs: split each row into chunks
         int64_t want = (2048 + p.nout - 1) / p.nout;
@@ -335,10 +343,10 @@ static hipError_t red_launch(const RedPlan &p, const Tin *a, const Outs &o, cons
         if (!parts) return hipErrorOutOfMemory;
         if (aligned)
           hipLaunchKernelGGL((red_split1<OP, Tin, Tout, Tacc, BAD, VEC>), dim3((unsigned)(p.nout * nchunk)), dim3(RT),
-                             0, s, a, p, nchunk, clen, parts, ba);
+                             0, s, a, p, nchunk, clen, parts, ba, stream);
         else
           hipLaunchKernelGGL((red_split1<OP, Tin, Tout, Tacc, BAD, 1>), dim3((unsigned)(p.nout * nchunk)), dim3(RT),
-                             0, s, a, p, nchunk, clen, parts, ba);
+                             0, s, a, p, nchunk, clen, parts, ba, stream);
         hipLaunchKernelGGL((red_split2<OP, Tin, Tout, Tacc, BAD>), dim3((unsigned)((p.nout + 3) / 4)), dim3(RT), 0, s,
                            p, nchunk, parts, o);
         hipError_t err = hipGetLastError();
@@ -349,10 +357,10 @@ static hipError_t red_launch(const RedPlan &p, const Tin *a, const Outs &o, cons
       if (blocks > 65536) blocks = 65536;
       if (aligned)
         hipLaunchKernelGGL((red_wave<OP, Tin, Tout, Tacc, BAD, VEC>), dim3((unsigned)blocks), dim3(RT), 0, s, a, p, o,
-                           ba);
+                           ba, stream);
       else
         hipLaunchKernelGGL((red_wave<OP, Tin, Tout, Tacc, BAD, 1>), dim3((unsigned)blocks), dim3(RT), 0, s, a, p, o,
-                           ba);
+                           ba, stream);
       return hipGetLastError();
     }
   }
