@@ -465,6 +465,16 @@ def run_single(args):
     if args.only == "matmult":
         print(json.dumps({"matmult": run_matmult(), "ragged_matmult": run_ragged_matmult()}))
         return
+    if args.only in ("mm16", "mm8"):
+        C.set_matmult_precision("split16" if args.only == "mm16" else "fp8")
+        a = C.random_normal(8192, 8192, dtype=C.L.F, seed=21)
+        bm = C.random_normal(8192, 8192, dtype=C.L.F, seed=22)
+        cm = C.ndarray.new_dense(C.L.F, [8192, 8192])
+        for _ in range(args.steps):
+            C._run("matmult", [a, bm, cm])
+        C.synchronize()
+        print(json.dumps({"only": args.only, "steps": args.steps}))
+        return
     if args.only == "copy":
         src = C.random_uniform(256, 1024, 1024, dtype=C.L.F, seed=1)      # 1 GiB
         dst = C.ndarray.new_dense(C.L.F, src.dims)

@@ -152,103 +152,119 @@ __device__ inline void tile_of(int64_t b, int64_t ntm, int64_t ntn, int64_t &tm,
   tn = r / rows;
 }
 
-// The k loop is software-pipelined over two LDS stages: while the matrix
-// cores work on stage k from LDS, the operands of step k+1 (fetched into
-// registers during step k-1) are written to the other stage and those of
-// step k+2 are fetched, so each step has one barrier and global latency is
-// hidden behind a whole step.  WM = 16-row A fragments per wave: a wave
-// computes (16*WM) x 64 of C, the block (32*WM) x 128.
-template <int WM>
-__global__ __launch_bounds__(256) void split16_gemm(const half_t *__restrict__ Ahi, const half_t *__restrict__ Alo,
-                                                   const half_t *__restrict__ Bhi, const half_t *__restrict__ Blo,
-                                                   const float *__restrict__ sA, const float *__restrict__ sB,
-                                                   float *__restrict__ c, MmPlan p, int64_t Kp, int64_t ntm,
-                                                   int64_t ntn) {
-  constexpr int BM = 32 * WM, AF = BM / 64;   // block rows; A half8 per thread per array per k-step
-  __shared__ half_t lA[2][2][BM][KPAD];       // [stage][hi/lo]
-  __shared__ half_t lB[2][2][NB][KPAD];
+// The operands stream into LDS with direct global->LDS loads
+// (global_load_lds_dwordx4: lane i of a wave writes 16 bytes at the wave's
+// LDS address + 16*i), no registers in between, over two stages: while the
+// matrix cores work on stage k, stage k+1 is being filled.  A wave computes
+// 64x64 of C as 2x2 tiles of v_mfma_f32_32x32x16_f16 (twice the work of the
+// 16x16x32 form per issue slot); the block 128x128.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <class T> __device__ __forceinline__ void dma16(const T *g, T *l) {
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)g,
+                                   (__attribute__((address_space(3))) void *)l, 16, 0, 0);
+}
+
+// BM x BN block tile, NW = WR x WC waves, each wave (BM/WR) x (BN/WC) of C as
+// TI x TJ tiles of 32x32.  128x128 (4 waves, 2x2 tiles each) for smaller
+// products; 256x256 (8 waves, 2x4 tiles each) halves the bytes staged per
+// MFMA and gives each step enough work to cover the loads of the next.
+template <int BM, int BN, int WR, int WC>
+__global__ __launch_bounds__(WR * WC * 64) void split16_gemm(const half_t *__restrict__ Ahi,
+                                                             const half_t *__restrict__ Alo,
+                                                             const half_t *__restrict__ Bhi,
+                                                             const half_t *__restrict__ Blo,
+                                                             const float *__restrict__ sA,
+                                                             const float *__restrict__ sB, float *__restrict__ c,
+                                                             MmPlan p, int64_t Kp, int64_t ntm, int64_t ntn) {
+  constexpr int NW = WR * WC, TI = BM / WR / 32, TJ = BN / WC / 32;
+  constexpr int RA = BM / NW, RB = BN / NW;   // rows of A / B each wave stages
+  static_assert(RA % 16 == 0 && RB % 16 == 0, "a wave stages whole 16-row loads");
+  // [stage][hi/lo][row][k]: rows of 32 halves (four 16-byte segments) with
+  // segment s of row r stored at s ^ ((r >> 1) & 3).  LDS serves 128 bytes
+  // (32 banks) per cycle, i.e. 8 lanes of a 16-byte access: the fragment
+  // reads (8 rows, one segment) then fall on distinct banks
+  __shared__ half_t lA[2][2][BM][KB];
+  __shared__ half_t lB[2][2][BN][KB];
+  auto sw = [](int row, int seg) { return ((seg ^ ((row >> 1) & 3)) * 8); };
   int64_t tm, tn;
   tile_of(blockIdx.x, ntm, ntn, tm, tn);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wr = w >> 1, wc = w & 1;
+  const int wr = w / WC, wc = w % WC;
   const half_t *ah = Ahi + tm * BM * Kp, *al = Alo + tm * BM * Kp;
-  const half_t *bh = Bhi + tn * NB * Kp, *bl = Blo + tn * NB * Kp;
-  f32x4 acc[WM][4];
+  const half_t *bh = Bhi + tn * BN * Kp, *bl = Blo + tn * BN * Kp;
+  f32x16 acc[TI][TJ];
 #pragma unroll
-  for (int i = 0; i < WM; i++)
+  for (int i = 0; i < TI; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // rows x 32 halves = 4 half8 per row
-  half8 ra[AF][2], rb[2][2];
-  auto fetch = [&](int64_t k0) {
+    for (int j = 0; j < TJ; j++)
 #pragma unroll
-    for (int s = 0; s < AF; s++) {
-      const int idx = tid + 256 * s;
-      const int64_t off = (int64_t)(idx >> 2) * Kp + k0 + (idx & 3) * 8;
-      ra[s][0] = *reinterpret_cast<const half8 *>(ah + off);
-      ra[s][1] = *reinterpret_cast<const half8 *>(al + off);
+      for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
+  // lane i of a 16-row load lands on row i/4, physical segment i%4, so it
+  // fetches that row's logical segment
+  const int drow = lane >> 2;
+  auto fill = [&](int st, int64_t k0) {
+#pragma unroll
+    for (int s = 0; s < RA / 16; s++) {
+      const int row = RA * w + 16 * s + drow;
+      const int64_t off = (int64_t)row * Kp + k0 + ((lane & 3) ^ ((row >> 1) & 3)) * 8;
+      dma16(ah + off, &lA[st][0][RA * w + 16 * s][0]);
+      dma16(al + off, &lA[st][1][RA * w + 16 * s][0]);
     }
 #pragma unroll
-    for (int s = 0; s < 2; s++) {
-      const int idx = tid + 256 * s;
-      const int64_t off = (int64_t)(idx >> 2) * Kp + k0 + (idx & 3) * 8;
-      rb[s][0] = *reinterpret_cast<const half8 *>(bh + off);
-      rb[s][1] = *reinterpret_cast<const half8 *>(bl + off);
-    }
-  };
-  auto stage = [&](int st) {
-#pragma unroll
-    for (int s = 0; s < AF; s++) {
-      const int idx = tid + 256 * s;
-      *reinterpret_cast<half8 *>(&lA[st][0][idx >> 2][(idx & 3) * 8]) = ra[s][0];
-      *reinterpret_cast<half8 *>(&lA[st][1][idx >> 2][(idx & 3) * 8]) = ra[s][1];
-    }
-#pragma unroll
-    for (int s = 0; s < 2; s++) {
-      const int idx = tid + 256 * s;
-      *reinterpret_cast<half8 *>(&lB[st][0][idx >> 2][(idx & 3) * 8]) = rb[s][0];
-      *reinterpret_cast<half8 *>(&lB[st][1][idx >> 2][(idx & 3) * 8]) = rb[s][1];
+    for (int s = 0; s < RB / 16; s++) {
+      const int row = RB * w + 16 * s + drow;
+      const int64_t off = (int64_t)row * Kp + k0 + ((lane & 3) ^ ((row >> 1) & 3)) * 8;
+      dma16(bh + off, &lB[st][0][RB * w + 16 * s][0]);
+      dma16(bl + off, &lB[st][1][RB * w + 16 * s][0]);
     }
   };
-  fetch(0);
-  stage(0);
-  if (KB < Kp) fetch(KB);
+  fill(0, 0);
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   int cur = 0;
+  // 32x32x16 operand layout: lane l holds row (l % 32), k (l / 32) * 8 .. +7
+  const int lr = lane & 31, lseg = lane >> 5;
   for (int64_t k0 = 0; k0 < Kp; k0 += KB) {
-    const int lr = lane & 15, lk = (lane >> 4) * 8;
-    half8 fbh[4], fbl[4];
+    // the other stage was last read in the previous step, before the barrier
+    if (k0 + KB < Kp) fill(cur ^ 1, k0 + KB);
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      fbh[j] = *reinterpret_cast<const half8 *>(&lB[cur][0][wc * 64 + j * 16 + lr][lk]);
-      fbl[j] = *reinterpret_cast<const half8 *>(&lB[cur][1][wc * 64 + j * 16 + lr][lk]);
-    }
+    for (int ks = 0; ks < KB / 16; ks++) {
+      half8 fbh[TJ], fbl[TJ];
 #pragma unroll
-    for (int i = 0; i < WM; i++) {
-      const half8 fah = *reinterpret_cast<const half8 *>(&lA[cur][0][wr * 16 * WM + i * 16 + lr][lk]);
-      const half8 fal = *reinterpret_cast<const half8 *>(&lA[cur][1][wr * 16 * WM + i * 16 + lr][lk]);
+      for (int j = 0; j < TJ; j++) {
+        const int br = wc * 32 * TJ + j * 32 + lr;
+        fbh[j] = *reinterpret_cast<const half8 *>(&lB[cur][0][br][sw(br, ks * 2 + lseg)]);
+        fbl[j] = *reinterpret_cast<const half8 *>(&lB[cur][1][br][sw(br, ks * 2 + lseg)]);
+      }
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fal, fbh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah, fbl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah, fbh[j], acc[i][j], 0, 0, 0);
+      for (int i = 0; i < TI; i++) {
+        const int ar = wr * 32 * TI + i * 32 + lr;
+        const half8 fah = *reinterpret_cast<const half8 *>(&lA[cur][0][ar][sw(ar, ks * 2 + lseg)]);
+        const half8 fal = *reinterpret_cast<const half8 *>(&lA[cur][1][ar][sw(ar, ks * 2 + lseg)]);
+        // product by product over the row's TJ accumulators
+#pragma unroll
+        for (int j = 0; j < TJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fal, fbh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < TJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fah, fbl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < TJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fah, fbh[j], acc[i][j], 0, 0, 0);
       }
     }
-    // the other stage was last read in the previous step, before the barrier
-    if (k0 + KB < Kp) stage(cur ^ 1);
-    if (k0 + 2 * KB < Kp) fetch(k0 + 2 * KB);
+    // the next stage's loads have landed and every wave is done with this one
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     cur ^= 1;
   }
-  // D layout of a 16x16 tile: lane l, reg r -> row 4*(l/16)+r, col l%16
+  // D layout of a 32x32 tile: lane l, reg r -> row 8*(r/4) + 4*(l/32) + r%4, col l%32
 #pragma unroll
-  for (int i = 0; i < WM; i++)
+  for (int i = 0; i < TI; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++)
+    for (int j = 0; j < TJ; j++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int64_t h = tm * BM + wr * 16 * WM + i * 16 + 4 * (lane >> 4) + r;
-        const int64_t wcol = tn * NB + wc * 64 + j * 16 + (lane & 15);
+      for (int r = 0; r < 16; r++) {
+        const int64_t h = tm * BM + wr * 32 * TI + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r & 3);
+        const int64_t wcol = tn * BN + wc * 32 * TJ + j * 32 + (lane & 31);
         if (h < p.H && wcol < p.W)
           c[p.off[2] + wcol * p.c_w + h * p.c_h] = (acc[i][j][r] * (1.0f / sA[h])) * (1.0f / sB[wcol]);
       }
@@ -337,11 +353,10 @@ hipError_t launch_mfma_f32(const MmPlan &p, const float *a, const float *b, floa
   // only where a 128x128 tile is mostly useful
   if (p.nbatch != 1 || p.H < 64 || p.W < 64 || p.T < 32) return hipErrorNotSupported;
   const bool fp8 = (mode == PDLHIP_MM_FP8);
-  // 256-row blocks for split16 measured no faster than 128-row ones (the
-  // larger tile leaves one wave per SIMD); kept selectable here
-  const bool wide = false;
-  const int64_t BMr = wide ? 256 : MB;
-  const int64_t Hp = (p.H + BMr - 1) / BMr * BMr, Wp = (p.W + NB - 1) / NB * NB, Kp = (p.T + KB - 1) / KB * KB;
+  // split16 takes 256x256 blocks once both sides fill them
+  const bool big = !fp8 && p.H >= 1024 && p.W >= 1024;
+  const int64_t BMr = big ? 256 : MB, BNr = big ? 256 : NB;
+  const int64_t Hp = (p.H + BMr - 1) / BMr * BMr, Wp = (p.W + BNr - 1) / BNr * BNr, Kp = (p.T + KB - 1) / KB * KB;
   const size_t esz = fp8 ? 1 : 2, ncopy = fp8 ? 1 : 2;
   char *buf = (char *)scratch_alloc((size_t)(Hp + Wp) * Kp * esz * ncopy + (size_t)(Hp + Wp) * sizeof(float) + 256);
   if (!buf) return hipErrorOutOfMemory;
@@ -362,17 +377,17 @@ hipError_t launch_mfma_f32(const MmPlan &p, const float *a, const float *b, floa
   // A rows = h (stride a_h), k = t (stride a_t); B "rows" = w (b_w), k = t (b_t)
   hipError_t e = prep_operand(a, o[0], p.a_h, p.a_t, p.H, p.T, Hp, Kp, sA, Ahi, Alo, A8, fp8, s);
   if (e == hipSuccess) e = prep_operand(b, o[1], p.b_w, p.b_t, p.W, p.T, Wp, Kp, sB, Bhi, Blo, B8, fp8, s);
-  const int64_t ntm = Hp / BMr, ntn = Wp / NB;
+  const int64_t ntm = Hp / BMr, ntn = Wp / BNr;
   if (e == hipSuccess) {
     if (fp8)
       hipLaunchKernelGGL(fp8_gemm, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, A8, B8, sA, sB, c, p, Kp, ntm,
                          ntn);
-    else if (wide)
-      hipLaunchKernelGGL(split16_gemm<8>, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, Ahi, Alo, Bhi, Blo, sA, sB, c,
-                         p, Kp, ntm, ntn);
+    else if (big)
+      hipLaunchKernelGGL((split16_gemm<256, 256, 4, 2>), dim3((unsigned)(ntm * ntn)), dim3(512), 0, s, Ahi, Alo, Bhi,
+                         Blo, sA, sB, c, p, Kp, ntm, ntn);
     else
-      hipLaunchKernelGGL(split16_gemm<4>, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, Ahi, Alo, Bhi, Blo, sA, sB, c,
-                         p, Kp, ntm, ntn);
+      hipLaunchKernelGGL((split16_gemm<128, 128, 2, 2>), dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, Ahi, Alo, Bhi,
+                         Blo, sA, sB, c, p, Kp, ntm, ntn);
     e = hipGetLastError();
   }
   scratch_free(buf);

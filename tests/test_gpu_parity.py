@@ -341,8 +341,8 @@ MM_TOL = {"fp32": 4e-6, "split16": 4e-6, "fp8": 0.08}
 
 
 @pytest.mark.parametrize("mode", ["fp32", "split16", "fp8"])
-@pytest.mark.parametrize("shape", [(300, 257, 190), (600, 100, 130), (1030, 64, 333)],
-                         ids=["128-row tiles", "256-row tiles", "256-row tiles ragged"])
+@pytest.mark.parametrize("shape", [(300, 257, 190), (600, 100, 130), (1030, 64, 333), (1100, 100, 1030)],
+                         ids=["small", "tall", "tall ragged", "256x256 tiles ragged"])
 @pytest.mark.parametrize("layout", ["dense", "a transposed"])
 def test_matmult_float_modes_accuracy(mode, shape, layout):
     h, k, w = shape
