@@ -56,6 +56,24 @@ __global__ __launch_bounds__(256) void absmax_rows_wave(const float *__restrict_
   if (lane == 0) smax[r] = m;
 }
 
+// the same with 16-byte loads, for rows that start 16-byte aligned
+__global__ __launch_bounds__(256) void absmax_rows_vec(const float *__restrict__ src, int64_t o, int64_t inc_r,
+                                                       int64_t R, int64_t K, float *smax) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= R) return;
+  const float *row = src + o + r * inc_r;
+  float m = 0.0f;
+  const int64_t K4 = K / 4;
+  for (int64_t k = lane; k < K4; k += 64) {
+    const Vec<float, 4> v = *reinterpret_cast<const Vec<float, 4> *>(row + 4 * k);
+    m = fmaxf(m, fmaxf(fmaxf(fabsf(v.v[0]), fabsf(v.v[1])), fmaxf(fabsf(v.v[2]), fabsf(v.v[3]))));
+  }
+  for (int64_t k = 4 * K4 + lane; k < K; k += 64) m = fmaxf(m, fabsf(row[k]));
+  for (int s = 32; s >= 1; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
+  if (lane == 0) smax[r] = m;
+}
+
 // max |x| per row of one 64x64 tile (read as pack_tile reads it, coalesced
 // along whichever index is unit-stride), folded into smax with an atomic max
 // on the bit pattern (non-negative floats order like their bits); smax starts
@@ -113,17 +131,78 @@ __global__ __launch_bounds__(256) void pack_tile(const float *__restrict__ src, 
     t[rr][kk] = (gr < R && gk < K) ? src[o + gr * inc_r + gk * inc_k] : 0.0f;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-    const int kk = e % 64, rr = e / 64;
-    const int64_t gr = r0 + rr, gk = k0 + kk;
+  // a thread writes 8 consecutive k of one row (Kp and the tile's k0 are
+  // multiples of 32, so a group is all in or all out)
+  for (int e = threadIdx.x; e < 64 * 8; e += 256) {
+    const int rr = e / 8, kg = (e % 8) * 8;
+    const int64_t gr = r0 + rr, gk = k0 + kg;
     if (gk >= Kp) continue;
-    const float x = t[rr][kk] * scale[gr];
+    const float sc = scale[gr];
     if constexpr (FP8) {
-      q8[gr * Kp + gk] = (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false) & 0xff);
+      uint32_t w[2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        uint32_t lo01 = __builtin_amdgcn_cvt_pk_fp8_f32(t[rr][kg + 4 * h] * sc, t[rr][kg + 4 * h + 1] * sc, 0, false);
+        w[h] = __builtin_amdgcn_cvt_pk_fp8_f32(t[rr][kg + 4 * h + 2] * sc, t[rr][kg + 4 * h + 3] * sc, lo01, true);
+      }
+      *reinterpret_cast<uint2 *>(q8 + gr * Kp + gk) = make_uint2(w[0], w[1]);
     } else {
-      const half_t h = (half_t)x;
-      hi[gr * Kp + gk] = h;
-      lo[gr * Kp + gk] = (half_t)(x - (float)h);
+      half8 vh, vl;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const float y = t[rr][kg + i] * sc;
+        const half_t h = (half_t)y;
+        vh[i] = h;
+        vl[i] = (half_t)(y - (float)h);
+      }
+      *reinterpret_cast<half8 *>(hi + gr * Kp + gk) = vh;
+      *reinterpret_cast<half8 *>(lo + gr * Kp + gk) = vl;
+    }
+  }
+}
+
+// pack an operand whose k index is unit-stride (rows 16-byte aligned): a
+// thread converts 8 consecutive k of one row (two 16-byte loads) into one
+// 16-byte hi and one lo store (8 bytes for FP8); zeros past R and K
+template <bool FP8>
+__global__ __launch_bounds__(256) void pack_rows(const float *__restrict__ src, int64_t o, int64_t inc_r, int64_t R,
+                                                 int64_t K, int64_t Rp, int64_t Kp, const float *__restrict__ scale,
+                                                 half_t *__restrict__ hi, half_t *__restrict__ lo,
+                                                 uint8_t *__restrict__ q8) {
+  const int64_t ng = Kp / 8, total = Rp * ng;
+  for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < total; g += (int64_t)gridDim.x * 256) {
+    const int64_t row = g / ng, k0 = (g % ng) * 8;
+    float x[8];
+    if (row < R && k0 + 8 <= K) {
+      const float *p = src + o + row * inc_r + k0;
+      const Vec<float, 4> a = *reinterpret_cast<const Vec<float, 4> *>(p);
+      const Vec<float, 4> b = *reinterpret_cast<const Vec<float, 4> *>(p + 4);
+#pragma unroll
+      for (int i = 0; i < 4; i++) { x[i] = a.v[i]; x[4 + i] = b.v[i]; }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; i++) x[i] = (row < R && k0 + i < K) ? src[o + row * inc_r + k0 + i] : 0.0f;
+    }
+    const float sc = scale[row];
+    if constexpr (FP8) {
+      uint32_t w[2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        uint32_t lo01 = __builtin_amdgcn_cvt_pk_fp8_f32(x[4 * h] * sc, x[4 * h + 1] * sc, 0, false);
+        w[h] = __builtin_amdgcn_cvt_pk_fp8_f32(x[4 * h + 2] * sc, x[4 * h + 3] * sc, lo01, true);
+      }
+      *reinterpret_cast<uint2 *>(q8 + row * Kp + k0) = make_uint2(w[0], w[1]);
+    } else {
+      half8 vh, vl;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const float y = x[i] * sc;
+        const half_t h = (half_t)y;
+        vh[i] = h;
+        vl[i] = (half_t)(y - (float)h);
+      }
+      *reinterpret_cast<half8 *>(hi + row * Kp + k0) = vh;
+      *reinterpret_cast<half8 *>(lo + row * Kp + k0) = vl;
     }
   }
 }
@@ -328,7 +407,11 @@ __global__ __launch_bounds__(256) void fp8_gemm(const uint8_t *__restrict__ A8, 
 static hipError_t prep_operand(const float *src, int64_t o, int64_t inc_r, int64_t inc_k, int64_t R, int64_t K,
                                int64_t Rp, int64_t Kp, float *scale, half_t *hi, half_t *lo, uint8_t *q8, bool fp8,
                                hipStream_t s) {
-  if (inc_k == 1 || inc_k == -1)
+  // rows along unit-stride k that start 16-byte aligned take the vector kernels
+  const bool rows16 = inc_k == 1 && ((uintptr_t)(src + o) % 16 == 0) && inc_r % 4 == 0;
+  if (rows16)
+    hipLaunchKernelGGL(absmax_rows_vec, dim3((unsigned)((R + 3) / 4)), dim3(256), 0, s, src, o, inc_r, R, K, scale);
+  else if (inc_k == 1 || inc_k == -1)
     hipLaunchKernelGGL(absmax_rows_wave, dim3((unsigned)((R + 3) / 4)), dim3(256), 0, s, src, o, inc_r, inc_k, R, K,
                        scale);
   else {
@@ -338,6 +421,17 @@ static hipError_t prep_operand(const float *src, int64_t o, int64_t inc_r, int64
                        (unsigned *)scale, nk);
   }
   hipLaunchKernelGGL(make_scales, dim3((unsigned)((Rp + 255) / 256)), dim3(256), 0, s, scale, R, Rp, fp8 ? 8 : 15);
+  if (rows16) {
+    int64_t blocks = (Rp * (Kp / 8) + 255) / 256;
+    blocks = blocks > 65536 ? 65536 : blocks;
+    if (fp8)
+      hipLaunchKernelGGL((pack_rows<true>), dim3((unsigned)blocks), dim3(256), 0, s, src, o, inc_r, R, K, Rp, Kp, scale,
+                         hi, lo, q8);
+    else
+      hipLaunchKernelGGL((pack_rows<false>), dim3((unsigned)blocks), dim3(256), 0, s, src, o, inc_r, R, K, Rp, Kp,
+                         scale, hi, lo, q8);
+    return hipGetLastError();
+  }
   const int64_t ntr = (Rp + 63) / 64, ntk = (Kp + 63) / 64;
   if (fp8)
     hipLaunchKernelGGL((pack_tile<true>), dim3((unsigned)(ntr * ntk)), dim3(256), 0, s, src, o, inc_r, inc_k, R, K, Kp,
