@@ -327,3 +327,26 @@ def test_chain_graph_replays_equal_eager_runs():
         assert v.tobytes() == ch.value.numpy().tobytes(), rep
         assert s.tobytes() == ch.result.numpy().tobytes(), rep
     g.destroy()
+
+
+def test_general_chain_program_matches_eager_ops():
+    """a program that is not left-deep ((a-b) + (a*w), then times the mask)
+    runs in the general per-element kernel: values bit for bit with the
+    oracle's eager ops, minimum bit-exact, sums within the bound"""
+    rng = np.random.default_rng(55)
+    lens = lengths(rng, 40)
+    items_o, items_d = make_items(rng, O.F, lens, packed=False)
+    chain = [("minus", 0, 1), ("mult", 0, 3), ("plus", ("node", 0), ("node", 1)), ("mult", ("node", 2), 4)]
+    rv = C.Ragged(items_d, chain, "minimum", values=True)
+    rv.run()
+    rs = C.Ragged(items_d, chain, "sumover")
+    rs.run()
+    for i, so in enumerate(items_o):
+        v = eager_chain(oracle_run, so, chain, O.OPdl.null)
+        np.testing.assert_array_equal(rv.values[i].numpy(), v.gather(), err_msg="item %d" % i)
+        out = O.OPdl.null()
+        O.run("minimum", v, out)
+        assert out.gather().tobytes() == rv.outputs[i].numpy().tobytes(), i
+        x = v.gather().astype(np.longdouble)
+        truth = float(x.sum())
+        assert abs(float(rs.outputs[i].numpy()) - truth) <= sum_bound(truth, float(np.abs(x).sum()), O.F), i
