@@ -404,6 +404,98 @@ __global__ __launch_bounds__(256) void fp8_gemm(const uint8_t *__restrict__ A8, 
       }
 }
 
+// FP8 on v_mfma_scale_f32_32x32x64_f8f6f4 (e4m3 operands, unit block scales:
+// rows and columns are pre-scaled by powers of two as for split16), the same
+// DMA two-stage pipeline over 64-byte k-steps; 256x256 blocks of 8 waves,
+// each 64x128 of C as 2x4 tiles of 32x32.
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+constexpr int KB8 = 64;
+
+template <int BM, int BN, int WR, int WC>
+__global__ __launch_bounds__(WR * WC * 64) void fp8_gemm_big(const uint8_t *__restrict__ A8,
+                                                             const uint8_t *__restrict__ B8,
+                                                             const float *__restrict__ sA,
+                                                             const float *__restrict__ sB, float *__restrict__ c,
+                                                             MmPlan p, int64_t Kp, int64_t ntm, int64_t ntn) {
+  constexpr int NW = WR * WC, TI = BM / WR / 32, TJ = BN / WC / 32;
+  constexpr int RA = BM / NW, RB = BN / NW;
+  static_assert(RA % 16 == 0 && RB % 16 == 0, "a wave stages whole 16-row loads");
+  // [stage][row][64 bytes], 16-byte segment s of row r at s ^ ((r >> 1) & 3)
+  __shared__ uint8_t lA[2][BM][KB8];
+  __shared__ uint8_t lB[2][BN][KB8];
+  auto sw = [](int row, int seg) { return ((seg ^ ((row >> 1) & 3)) * 16); };
+  int64_t tm, tn;
+  tile_of(blockIdx.x, ntm, ntn, tm, tn);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w / WC, wc = w % WC;
+  const uint8_t *a8 = A8 + tm * BM * Kp, *b8 = B8 + tn * BN * Kp;
+  f32x16 acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; i++)
+#pragma unroll
+    for (int j = 0; j < TJ; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
+  const int drow = lane >> 2;
+  auto fill = [&](int st, int64_t k0) {
+#pragma unroll
+    for (int s = 0; s < RA / 16; s++) {
+      const int row = RA * w + 16 * s + drow;
+      dma16(a8 + (int64_t)row * Kp + k0 + ((lane & 3) ^ ((row >> 1) & 3)) * 16, &lA[st][RA * w + 16 * s][0]);
+    }
+#pragma unroll
+    for (int s = 0; s < RB / 16; s++) {
+      const int row = RB * w + 16 * s + drow;
+      dma16(b8 + (int64_t)row * Kp + k0 + ((lane & 3) ^ ((row >> 1) & 3)) * 16, &lB[st][RB * w + 16 * s][0]);
+    }
+  };
+  fill(0, 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  int cur = 0;
+  // 32x32x64 fp8 operand layout: lane l holds row (l % 32), k bytes (l / 32) * 32 .. +31
+  const int lr = lane & 31, sg = (lane >> 5) * 2;
+  auto frag = [&](const uint8_t *rowp, int row) {
+    const uint4 x = *reinterpret_cast<const uint4 *>(rowp + sw(row, sg));
+    const uint4 y = *reinterpret_cast<const uint4 *>(rowp + sw(row, sg + 1));
+    i32x8 v;
+    v[0] = (int)x.x; v[1] = (int)x.y; v[2] = (int)x.z; v[3] = (int)x.w;
+    v[4] = (int)y.x; v[5] = (int)y.y; v[6] = (int)y.z; v[7] = (int)y.w;
+    return v;
+  };
+  for (int64_t k0 = 0; k0 < Kp; k0 += KB8) {
+    if (k0 + KB8 < Kp) fill(cur ^ 1, k0 + KB8);
+    i32x8 fb[TJ];
+#pragma unroll
+    for (int j = 0; j < TJ; j++) {
+      const int br = wc * 32 * TJ + j * 32 + lr;
+      fb[j] = frag(&lB[cur][br][0], br);
+    }
+#pragma unroll
+    for (int i = 0; i < TI; i++) {
+      const int ar = wr * 32 * TI + i * 32 + lr;
+      const i32x8 fa = frag(&lA[cur][ar][0], ar);
+#pragma unroll
+      for (int j = 0; j < TJ; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa, fb[j], acc[i][j], 0, 0, 0, 127, 0, 127);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int i = 0; i < TI; i++)
+#pragma unroll
+    for (int j = 0; j < TJ; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int64_t h = tm * BM + wr * 32 * TI + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r & 3);
+        const int64_t wcol = tn * BN + wc * 32 * TJ + j * 32 + (lane & 31);
+        if (h < p.H && wcol < p.W)
+          c[p.off[2] + wcol * p.c_w + h * p.c_h] = (acc[i][j][r] * (1.0f / sA[h])) * (1.0f / sB[wcol]);
+      }
+}
+
 static hipError_t prep_operand(const float *src, int64_t o, int64_t inc_r, int64_t inc_k, int64_t R, int64_t K,
                                int64_t Rp, int64_t Kp, float *scale, half_t *hi, half_t *lo, uint8_t *q8, bool fp8,
                                hipStream_t s) {
@@ -447,10 +539,10 @@ hipError_t launch_mfma_f32(const MmPlan &p, const float *a, const float *b, floa
   // only where a 128x128 tile is mostly useful
   if (p.nbatch != 1 || p.H < 64 || p.W < 64 || p.T < 32) return hipErrorNotSupported;
   const bool fp8 = (mode == PDLHIP_MM_FP8);
-  // split16 takes 256x256 blocks once both sides fill them
-  const bool big = !fp8 && p.H >= 1024 && p.W >= 1024;
-  const int64_t BMr = big ? 256 : MB, BNr = big ? 256 : NB;
-  const int64_t Hp = (p.H + BMr - 1) / BMr * BMr, Wp = (p.W + BNr - 1) / BNr * BNr, Kp = (p.T + KB - 1) / KB * KB;
+  // 256x256 blocks once both sides fill them (FP8 then steps k by 64)
+  const bool big = p.H >= 1024 && p.W >= 1024;
+  const int64_t BMr = big ? 256 : MB, BNr = big ? 256 : NB, KS = (big && fp8) ? KB8 : KB;
+  const int64_t Hp = (p.H + BMr - 1) / BMr * BMr, Wp = (p.W + BNr - 1) / BNr * BNr, Kp = (p.T + KS - 1) / KS * KS;
   const size_t esz = fp8 ? 1 : 2, ncopy = fp8 ? 1 : 2;
   char *buf = (char *)scratch_alloc((size_t)(Hp + Wp) * Kp * esz * ncopy + (size_t)(Hp + Wp) * sizeof(float) + 256);
   if (!buf) return hipErrorOutOfMemory;
@@ -473,7 +565,10 @@ hipError_t launch_mfma_f32(const MmPlan &p, const float *a, const float *b, floa
   if (e == hipSuccess) e = prep_operand(b, o[1], p.b_w, p.b_t, p.W, p.T, Wp, Kp, sB, Bhi, Blo, B8, fp8, s);
   const int64_t ntm = Hp / BMr, ntn = Wp / BNr;
   if (e == hipSuccess) {
-    if (fp8)
+    if (fp8 && big)
+      hipLaunchKernelGGL((fp8_gemm_big<256, 256, 4, 2>), dim3((unsigned)(ntm * ntn)), dim3(512), 0, s, A8, B8, sA, sB,
+                         c, p, Kp, ntm, ntn);
+    else if (fp8)
       hipLaunchKernelGGL(fp8_gemm, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, A8, B8, sA, sB, c, p, Kp, ntm,
                          ntn);
     else if (big)
