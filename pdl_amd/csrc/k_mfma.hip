@@ -74,6 +74,32 @@ __global__ __launch_bounds__(256) void absmax_rows_vec(const float *__restrict__
   if (lane == 0) smax[r] = m;
 }
 
+// load a 64x64 (rows x k) tile of src into t, coalesced along whichever
+// index is unit-stride; rows unit-stride and 16-byte aligned with the whole
+// tile in range take 16-byte loads.  |x| when ABS.  Zeros outside R x K.
+template <bool ABS>
+__device__ inline void load_tile64(float (&t)[64][65], const float *__restrict__ src, int64_t o, int64_t inc_r,
+                                   int64_t inc_k, int64_t R, int64_t K, int64_t r0, int64_t k0) {
+  const bool kfast = (inc_k == 1 || inc_k == -1 || inc_r != 1);
+  const bool rvec = !kfast && r0 + 64 <= R && k0 + 64 <= K && inc_k % 4 == 0 && ((uintptr_t)(src + o + r0) % 16 == 0);
+  if (rvec) {
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      const int rr = (e % 16) * 4, kk = e / 16;
+      const Vec<float, 4> v = *reinterpret_cast<const Vec<float, 4> *>(src + o + (r0 + rr) + (k0 + kk) * inc_k);
+#pragma unroll
+      for (int i = 0; i < 4; i++) t[rr + i][kk] = ABS ? fabsf(v.v[i]) : v.v[i];
+    }
+    return;
+  }
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    int rr, kk;
+    if (kfast) { kk = e % 64; rr = e / 64; } else { rr = e % 64; kk = e / 64; }
+    const int64_t gr = r0 + rr, gk = k0 + kk;
+    const float x = (gr < R && gk < K) ? src[o + gr * inc_r + gk * inc_k] : 0.0f;
+    t[rr][kk] = ABS ? fabsf(x) : x;
+  }
+}
+
 // max |x| per row of one 64x64 tile (read as pack_tile reads it, coalesced
 // along whichever index is unit-stride), folded into smax with an atomic max
 // on the bit pattern (non-negative floats order like their bits); smax starts
@@ -84,13 +110,7 @@ __global__ __launch_bounds__(256) void absmax_tile(const float *__restrict__ src
                                                    int64_t ntk) {
   __shared__ float t[64][65];
   const int64_t r0 = (blockIdx.x / ntk) * 64, k0 = (blockIdx.x % ntk) * 64;
-  const bool kfast = (inc_k == 1 || inc_k == -1 || inc_r != 1);
-  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-    int rr, kk;
-    if (kfast) { kk = e % 64; rr = e / 64; } else { rr = e % 64; kk = e / 64; }
-    const int64_t gr = r0 + rr, gk = k0 + kk;
-    t[rr][kk] = (gr < R && gk < K) ? fabsf(src[o + gr * inc_r + gk * inc_k]) : 0.0f;
-  }
+  load_tile64<true>(t, src, o, inc_r, inc_k, R, K, r0, k0);
   __syncthreads();
   const int rr = threadIdx.x >> 2, part = threadIdx.x & 3;
   float m = 0.0f;
@@ -123,13 +143,7 @@ __global__ __launch_bounds__(256) void pack_tile(const float *__restrict__ src, 
                                                  int64_t ntk) {
   __shared__ float t[64][65];
   const int64_t r0 = (blockIdx.x / ntk) * 64, k0 = (blockIdx.x % ntk) * 64;
-  const bool kfast = (inc_k == 1 || inc_k == -1 || inc_r != 1);
-  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-    int rr, kk;
-    if (kfast) { kk = e % 64; rr = e / 64; } else { rr = e % 64; kk = e / 64; }
-    const int64_t gr = r0 + rr, gk = k0 + kk;
-    t[rr][kk] = (gr < R && gk < K) ? src[o + gr * inc_r + gk * inc_k] : 0.0f;
-  }
+  load_tile64<false>(t, src, o, inc_r, inc_k, R, K, r0, k0);
   __syncthreads();
   // a thread writes 8 consecutive k of one row (Kp and the tile's k0 are
   // multiples of 32, so a group is all in or all out)
