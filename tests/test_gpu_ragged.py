@@ -210,6 +210,42 @@ def test_ragged_integer_scans_bit_exact(t, final):
     r.free()
 
 
+@pytest.mark.parametrize("t,final", [(O.LL, "cumusumover"), (O.B, "cumusumover"), (O.L, "cumuprodover")])
+def test_ragged_scans_long_items_span_many_tiles(t, final):
+    """items of up to 300000 elements (74 chunks of 4096 carried through one
+    block) between empty, one-element and chunk-edge items; unit-stride, misaligned (slice from 1) and strided (step 3)
+    layouts; wrapping results bit-exact with the oracle"""
+    rng = np.random.default_rng(900 + t)
+    lens = [300000, 0, 1, 4096, 4097, 270001, 8191, 12288, 5, 266240, 300000, 3]
+    dt = np.dtype(O.NP_TYPES[t])
+    info = np.iinfo(dt)
+    lo, hi = (info.min, info.max) if final == "cumusumover" else (-3, 3)
+    items_o, items_d, how = [], [], []
+    for i, n in enumerate(lens):
+        kind = ("unit", "misaligned", "strided")[i % 3]
+        if kind == "unit":
+            base = rng.integers(lo, hi, size=n, endpoint=True, dtype=dt)
+            so, sd = O.OPdl.from_numpy(base, t), C.ndarray.from_numpy(base, t)
+        else:
+            step, start = (1, 1) if kind == "misaligned" else (3, 0)
+            base = rng.integers(lo, hi, size=start + step * max(n, 1), endpoint=True, dtype=dt)
+            sl = (start, start + step * (n - 1), step)
+            so = O.OPdl.from_numpy(base, t).slice(sl) if n else O.OPdl.from_numpy(base[:0], t)
+            sd = C.ndarray.from_numpy(base, t).slice(sl) if n else C.ndarray.from_numpy(base[:0], t)
+        items_o.append([so])
+        items_d.append([sd])
+        how.append(kind)
+    r = C.Ragged(items_d, [], final)
+    for rep in range(2):
+        r.run()
+        for i, so in enumerate(items_o):
+            out = O.OPdl.null()
+            O.run(final, so[0], out)
+            np.testing.assert_array_equal(out.gather(), r.outputs[i].numpy(), err_msg="%s %s item %d len %d %s rep %d" %
+                                          (final, NAMES[t], i, lens[i], how[i], rep))
+    r.free()
+
+
 def test_graph_replays_equal_eager_runs():
     """a fused ragged sumover and an int64 ragged cumusumover captured in one
     HIP graph; the inputs change between replays (eager device ops and
