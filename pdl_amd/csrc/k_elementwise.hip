@@ -112,22 +112,28 @@ __global__ __launch_bounds__(EW_THREADS) void ew_elems(const T *__restrict__ a, 
 // transposed input is read 4-wide along j, the output written 4-wide along i.
 // 4-byte types use 32 (i) x 128 (j) tiles: 512-byte runs on the transposed
 // read, 128-byte runs on the streaming write; others 64 x 64.
-template <class T> struct TileShape { static constexpr int I = 64, J = 64; };
-template <> struct TileShape<float> { static constexpr int I = 32, J = 128; };
-template <> struct TileShape<int32_t> { static constexpr int I = 32, J = 128; };
-template <> struct TileShape<uint32_t> { static constexpr int I = 32, J = 128; };
+// tile of the transposing kernel: TI along the output's unit stride, TJ along
+// the transposed input's; about 33 KB of LDS per block, so four blocks share a CU
+template <class T, int TMASK> struct TileShape {
+  static constexpr int I = (sizeof(T) == 4 && TMASK == 3) ? 32 : 64;
+  static constexpr int J = sizeof(T) == 4 ? 128 : 64;
+};
+constexpr int TILE_GROUP = 8;   // tile-rows (along i) visited together, column after column
 
-template <class T, int OP, int TMASK, int V, int TI = TileShape<T>::I, int TJ = TileShape<T>::J>
+template <class T, int OP, int TMASK, int V, int TI = TileShape<T, TMASK>::I, int TJ = TileShape<T, TMASK>::J>
 __global__ __launch_bounds__(EW_THREADS) void ew_tile(const T *__restrict__ a, const T *__restrict__ b,
                                                      T *__restrict__ c, EwPlan p, int q, int64_t ntile_i,
                                                      int64_t ntile_j, bool stream) {
   constexpr int NL = ((TMASK & 1) ? 1 : 0) + ((TMASK & 2) ? 1 : 0);
   __shared__ T lds[NL][TI][TJ + 1];
-  int64_t blk = blockIdx.x;
-  const int64_t ti = blk % ntile_i;
-  blk /= ntile_i;
-  const int64_t tj = blk % ntile_j;
-  int64_t rest = blk / ntile_j;
+  // blocks walk groups of TILE_GROUP tile-rows column by column: the blocks
+  // in flight read and write a compact band of both operands
+  const int64_t per = ntile_i * ntile_j;
+  int64_t rest = (int64_t)blockIdx.x / per, r = (int64_t)blockIdx.x % per;
+  const int64_t grp = r / (TILE_GROUP * ntile_j);
+  const int64_t gsz = ntile_i - grp * TILE_GROUP < TILE_GROUP ? ntile_i - grp * TILE_GROUP : TILE_GROUP;
+  r -= grp * TILE_GROUP * ntile_j;
+  const int64_t ti = grp * TILE_GROUP + r % gsz, tj = r / gsz;
   int64_t o[3] = {p.off[0], p.off[1], p.off[2]};
   for (int d = 1; d < p.nd; d++) {
     if (d == q) continue;
@@ -294,7 +300,9 @@ static hipError_t binop_T_OP(const EwPlan &p, const void *va, const void *vb, vo
           if (p.inc[k][d] == 1 && (q < 0 || q == d)) { q = d; tmask |= 1 << k; }
       }
       if (q > 0 && tmask) {
-        constexpr int TI = TileShape<T>::I, TJ = TileShape<T>::J;
+        // tile counts of the shape the launched kernel uses
+        const int TI = tmask == 3 ? TileShape<T, 3>::I : TileShape<T, 1>::I;
+        const int TJ = tmask == 3 ? TileShape<T, 3>::J : TileShape<T, 1>::J;
         int64_t nti = (p.dims[0] + TI - 1) / TI, ntj = (p.dims[q] + TJ - 1) / TJ;
         int64_t nrest = p.n / (p.dims[0] * p.dims[q]);
         int64_t nblk = nti * ntj * nrest;

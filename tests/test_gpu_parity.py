@@ -92,12 +92,17 @@ LAYOUTS = {
     "scalar": ([1000], [], []),
     "bcast": ([33, 1, 5], [33, 6, 1], []),
     "xchg": ([70, 130, 3], [130, 1, 3], [("xchg", 0, 1)]),
+    # 11 tile-rows of 64 (one whole group of 8 and a part group), 3 ragged tile-columns
+    "xchg_groups": ([300, 700, 2], [700, 1, 2], [("xchg", 0, 1)]),
+    # both inputs transposed against the output
+    "xchg_both": ([300, 700, 2], [300, 700, 2], [("xchg", 0, 1)], [("xchg", 0, 1)]),
     "slice": ([40, 30], [20, 30], [("slice", (0, 39, 2), None)]),
 }
 
 
 def binop_inputs(op, t, layout, rng):
-    sa, sb, va = LAYOUTS[layout]
+    sa, sb, va = LAYOUTS[layout][:3]
+    vb = LAYOUTS[layout][3] if len(LAYOUTS[layout]) > 3 else []
     a = rand(t, sa[::-1], rng)
     b = rand(t, sb[::-1] if sb else (), rng)
     dt = np.dtype(O.NP_TYPES[t])
@@ -114,6 +119,7 @@ def binop_inputs(op, t, layout, rng):
         b = np.round(b).astype(dt)
     A, Bp = pair(a, t), pair(b, t)
     A = (view(A[0], va), view(A[1], va))
+    Bp = (view(Bp[0], vb), view(Bp[1], vb))
     return [A, Bp]
 
 
