@@ -52,28 +52,25 @@ __device__ inline void row_segment(Part<Tin, Tacc> &pt, const Tin *__restrict__ 
                                    int64_t end, int64_t inc, int lane, const BadInfo &ba, bool stream) {
   const Tin abv = bv_as<Tin>(ba.bv);
   int64_t i = begin + (int64_t)lane * VEC;
+  // 16-byte loads in flight per lane: 4 for 4- and 8-byte elements, 2 for
+  // the narrower ones (eight or sixteen elements per load already)
+  constexpr int RED_U = sizeof(Tin) >= 4 ? 4 : 2;
   if constexpr (VEC > 1) {
-    // begin is a multiple of VEC and the row start is 16-byte aligned; two
+    // begin is a multiple of VEC and the row start is 16-byte aligned; RED_U
     // vectors in flight per lane, consumed in increasing index order
-    for (; i + 64 * VEC + VEC <= end; i += 128 * VEC) {
-      Vec<Tin, VEC> x0, x1;
-      if (stream) {
-        x0 = load_nt<Tin, VEC>(a + o + i);
-        x1 = load_nt<Tin, VEC>(a + o + i + 64 * VEC);
-      } else {
-        x0 = *reinterpret_cast<const Vec<Tin, VEC> *>(a + o + i);
-        x1 = *reinterpret_cast<const Vec<Tin, VEC> *>(a + o + i + 64 * VEC);
-      }
+    for (; i + (RED_U - 1) * 64 * VEC + VEC <= end; i += RED_U * 64 * VEC) {
+      Vec<Tin, VEC> x[RED_U];
 #pragma unroll
-      for (int k = 0; k < VEC; k++) {
-        if constexpr (BAD) { if (isbad(x0.v[k], abv, ba.isnan)) continue; }
-        part_push<OP, Tin, Tacc>(pt, x0.v[k], i + k);
-      }
+      for (int u = 0; u < RED_U; u++)
+        x[u] = stream ? load_nt<Tin, VEC>(a + o + i + u * 64 * VEC)
+                      : *reinterpret_cast<const Vec<Tin, VEC> *>(a + o + i + u * 64 * VEC);
 #pragma unroll
-      for (int k = 0; k < VEC; k++) {
-        if constexpr (BAD) { if (isbad(x1.v[k], abv, ba.isnan)) continue; }
-        part_push<OP, Tin, Tacc>(pt, x1.v[k], i + 64 * VEC + k);
-      }
+      for (int u = 0; u < RED_U; u++)
+#pragma unroll
+        for (int k = 0; k < VEC; k++) {
+          if constexpr (BAD) { if (isbad(x[u].v[k], abv, ba.isnan)) continue; }
+          part_push<OP, Tin, Tacc>(pt, x[u].v[k], i + u * 64 * VEC + k);
+        }
     }
     for (; i + VEC <= end; i += 64 * VEC) {
       Vec<Tin, VEC> x = stream ? load_nt<Tin, VEC>(a + o + i) : *reinterpret_cast<const Vec<Tin, VEC> *>(a + o + i);
