@@ -181,15 +181,21 @@ __global__ __launch_bounds__(RT) void scan_rows(const Tin *__restrict__ a, RedPl
     Tout carry = ident;
     for (int64_t c0 = 0; c0 < p.n; c0 += SC_CH) {
       const int len = (p.n - c0 < SC_CH) ? (int)(p.n - c0) : SC_CH;
+      // every load of the chunk is issued before the first LDS write
+      Tin xv[SC_IT];
+#pragma unroll
+      for (int k = 0; k < SC_IT; k++) {
+        const int idx = tid + RT * k;
+        xv[k] = idx < len ? ap[c0 + idx] : Tin(0);
+      }
 #pragma unroll
       for (int k = 0; k < SC_IT; k++) {
         const int idx = tid + RT * k;
         Tout v = ident;
         if (idx < len) {
-          Tin x = ap[c0 + idx];
           bool isb = false;
-          if constexpr (BAD) { isb = isbad(x, abv, ba.isnan); badm[idx] = isb; }
-          if (!isb) v = (Tout)x;
+          if constexpr (BAD) { isb = isbad(xv[k], abv, ba.isnan); badm[idx] = isb; }
+          if (!isb) v = (Tout)xv[k];
         }
         sh[sc_pad(idx)] = v;
       }
