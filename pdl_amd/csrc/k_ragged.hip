@@ -157,6 +157,99 @@ __device__ __forceinline__ void chain_lin(T *acc, const RagProg &pg, const RagIt
   }
 }
 
+// Preloaded form of chain_lin for chains of at most RAG_PRE slots whose
+// operand types are all at most 4 bytes: every slot's U elements are loaded
+// as raw 32-bit words first, so a step costs one memory round trip instead
+// of one per node, and converted only when their node runs.
+constexpr int RAG_PRE = 5;
+
+// raw word of an element as its load instruction leaves it (sign- or
+// zero-extended), so nothing has to wait for the data before conversion
+template <class S> __device__ __forceinline__ uint32_t raw_bits(S v) {
+  if constexpr (IsFloat<S>::v) {
+    uint32_t u;
+    __builtin_memcpy(&u, &v, 4);
+    return u;
+  } else if constexpr ((S)-1 < (S)0) {
+    return (uint32_t)(int32_t)v;
+  } else {
+    return (uint32_t)v;
+  }
+}
+
+template <class S, int U, bool TAIL>
+__device__ __forceinline__ void load_raw_u(uint32_t *raw, const char *p, int64_t inc, int64_t base, int64_t end,
+                                           bool stream) {
+  const auto q = as_global((const S *)p);
+  if (inc == 0) {
+    raw[0] = raw_bits(q[0]);   // broadcast when converted
+  } else if (inc == 1 && stream) {
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int64_t e = base + (int64_t)j * 64;
+      raw[j] = (!TAIL || e < end) ? raw_bits(__builtin_nontemporal_load(q + e)) : 0u;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+      const int64_t e = base + (int64_t)j * 64;
+      raw[j] = (!TAIL || e < end) ? raw_bits(q[e * inc]) : 0u;
+    }
+  }
+}
+
+template <int U, bool TAIL>
+__device__ __forceinline__ void load_raw(uint32_t *raw, int type, const char *p, int64_t inc, int64_t base,
+                                         int64_t end, bool stream) {
+  switch (type) {
+#define X(TC)                                                                       \
+  case TC:                                                                          \
+    if constexpr (sizeof(CType<TC>::T) <= 4) load_raw_u<CType<TC>::T, U, TAIL>(raw, p, inc, base, end, stream); \
+    break;
+    PDLHIP_FOR_EACH_TYPE(X)
+#undef X
+  default: break;
+  }
+}
+
+template <class T, int U> __device__ __forceinline__ void conv_raw(T *v, int type, int64_t inc, const uint32_t *raw) {
+  switch (type) {
+#define X(TC)                                                      \
+  case TC:                                                         \
+    if constexpr (sizeof(CType<TC>::T) <= 4) {                     \
+      _Pragma("unroll") for (int j = 0; j < U; j++) {              \
+        CType<TC>::T s;                                            \
+        __builtin_memcpy(&s, &raw[inc == 0 ? 0 : j], sizeof(s));   \
+        v[j] = conv_to<T>(s);                                      \
+      }                                                            \
+    }                                                              \
+    break;
+    PDLHIP_FOR_EACH_TYPE(X)
+#undef X
+  default: break;
+  }
+}
+
+template <class T, int U, bool TAIL>
+__device__ __forceinline__ void chain_lin_pre(T *acc, const RagProg &pg, const RagItem &it, int64_t base,
+                                              int64_t end) {
+  uint32_t raw[RAG_PRE][U];
+#pragma unroll
+  for (int s = 0; s < RAG_PRE; s++) {
+    if (s > pg.nnodes) break;
+    const int slot = s == 0 ? pg.first : pg.lslot[s - 1];
+    load_raw<U, TAIL>(raw[s], pg.type[slot], it.ptr[slot], it.inc[slot], base, end, pg.stream != 0);
+  }
+  conv_raw<T, U>(acc, pg.type[pg.first], it.inc[pg.first], raw[0]);
+#pragma unroll
+  for (int n = 0; n < RAG_PRE - 1; n++) {
+    if (n >= pg.nnodes) break;
+    T x[U];
+    conv_raw<T, U>(x, pg.type[pg.lslot[n]], it.inc[pg.lslot[n]], raw[n + 1]);
+    run_op_u<T, U>(pg.node[n].op, acc, x, pg.lswap[n] != 0);
+  }
+}
+
 // the chain's values at elements base + j*64 into the item's value output
 template <class T, int U, bool TAIL>
 __device__ __forceinline__ void store_u(const T *v, char *p, int64_t inc, int64_t base, int64_t end, bool stream) {
@@ -187,7 +280,7 @@ __device__ inline void seg_finish(const Part<T, Tacc> &p, const RagProg &pg, con
   if constexpr (is_minmax(OP)) empty[item] = p.i0 < 0 ? 1 : 0;
 }
 
-template <class T, int OP, class Tout, class Tacc, int U, bool LIN>
+template <class T, int OP, class Tout, class Tacc, int U, bool LIN, bool PRE = false>
 __global__ __launch_bounds__(256) void rag_reduce(RagProg pg, const RagItem *__restrict__ items,
                                                   const RagSeg *__restrict__ segs, int64_t nseg,
                                                   Part<T, Tacc> *partials, int32_t *empty) {
@@ -205,7 +298,8 @@ __global__ __launch_bounds__(256) void rag_reduce(RagProg pg, const RagItem *__r
       int64_t b = sg.start;
       for (; b + STEP <= sg.end; b += STEP) {
         T v[U];
-        chain_lin<T, U, false>(v, pg, it, b + lane, sg.end);
+        if constexpr (PRE) chain_lin_pre<T, U, false>(v, pg, it, b + lane, sg.end);
+        else chain_lin<T, U, false>(v, pg, it, b + lane, sg.end);
         if (pg.store) store_u<T, U, false>(v, it.val, it.val_inc, b + lane, sg.end, pg.stream != 0);
         if constexpr (RED) {
 #pragma unroll
@@ -214,7 +308,8 @@ __global__ __launch_bounds__(256) void rag_reduce(RagProg pg, const RagItem *__r
       }
       if (b < sg.end) {
         T v[U];
-        chain_lin<T, U, true>(v, pg, it, b + lane, sg.end);
+        if constexpr (PRE) chain_lin_pre<T, U, true>(v, pg, it, b + lane, sg.end);
+        else chain_lin<T, U, true>(v, pg, it, b + lane, sg.end);
         if (pg.store) store_u<T, U, true>(v, it.val, it.val_inc, b + lane, sg.end, pg.stream != 0);
         if constexpr (RED) {
 #pragma unroll
@@ -370,7 +465,12 @@ static hipError_t rag_red(const RagLaunch &L, hipStream_t s) {
   if (blocks < 1) blocks = 1;
   Part<T, Tacc> *part = (Part<T, Tacc> *)L.scratch;
   constexpr int U = sizeof(T) == 4 ? 8 : 4;
-  if (L.prog.lin)
+  bool pre = L.prog.lin && L.prog.nnodes + 1 <= RAG_PRE;
+  for (int k = 0; k < L.prog.nin && pre; k++) pre = type_size(L.prog.type[k]) <= 4;
+  if (pre)
+    hipLaunchKernelGGL((rag_reduce<T, OP, Tout, Tacc, U, true, true>), dim3((unsigned)blocks), dim3(256), 0, s,
+                       L.prog, L.items, L.segs, L.nseg, part, L.empty);
+  else if (L.prog.lin)
     hipLaunchKernelGGL((rag_reduce<T, OP, Tout, Tacc, U, true>), dim3((unsigned)blocks), dim3(256), 0, s, L.prog,
                        L.items, L.segs, L.nseg, part, L.empty);
   else
