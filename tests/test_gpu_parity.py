@@ -214,6 +214,8 @@ def red_layouts(rng, t):
         "short": (rand(t, (300, 9), rng), []),                # thread kernel
         "xchg": (rand(t, (400, 70), rng), [("xchg", 0, 1)]),  # strided rows
         "long": (rand(t, (3, 200000), rng), []),              # split kernel
+        "rows_long": (rand(t, (40, 3000), rng), []),          # wave kernel, several load groups per lane
+        "rows_unaligned": (rand(t, (20, 2051), rng), [("slice", (1, 2050, 1), None)]),
         "empty": (rand(t, (4, 0), rng), []),
     }
 
@@ -283,10 +285,11 @@ def test_reductions(t):
             same(o, d, "minmaximum %d %s %s" % (k, NAMES[t], lname))
 
 
+@pytest.mark.parametrize("shape", [(50, 300), (20, 3000)], ids=["short_rows", "long_rows"])
 @pytest.mark.parametrize("t", ALL_T, ids=[NAMES[t] for t in ALL_T])
-def test_reductions_bad_values(t):
+def test_reductions_bad_values(t, shape):
     rng = np.random.default_rng(31 + t)
-    a = rand(t, (50, 300), rng)
+    a = rand(t, shape, rng)
     bv = O.default_badval(t)
     a.reshape(-1)[::3] = bv
     a[7, :] = bv                       # an all-bad row
