@@ -548,11 +548,151 @@ static hipError_t prep_operand(const float *src, int64_t o, int64_t inc_r, int64
   return hipGetLastError();
 }
 
+// Small float products (16 <= H, W, T <= 64), typically broadcast batches:
+// persistent blocks of 4 waves, one product at a time, nothing staged
+// outside the kernel.  A
+// (rows h, k = t) and B (rows w, k = t, transposed on the way in) are read
+// once in FP32 into LDS; each row of A and each column of B gets the same
+// power-of-two scale the large kernels use; every wave then owns one 32x32
+// quarter of C and, per 16-k step, reads its fragments from LDS, scales and
+// splits them into FP16 hi/lo in registers and issues the three
+// v_mfma_f32_32x32x16_f16 products.  Rows of 68 floats keep the 16-byte
+// fragment reads aligned and on distinct banks.
+constexpr int SMM = 64, SMP = 68;
+
+__global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict__ a, const float *__restrict__ b,
+                                                        float *__restrict__ c, MmPlan p) {
+  __shared__ __align__(16) float fA[SMM][SMP];
+  __shared__ __align__(16) float fB[SMM][SMP];
+  __shared__ float scA[SMM], scB[SMM];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int H = (int)p.H, W = (int)p.W, T = (int)p.T, Tp = (T + 15) & ~15;
+  // staging: thread (r0, q) holds rows r0 + 4j of A(h = r, t = q) and of
+  // B(t = r, w = q), loaded from indices clamped into the operand (zeroed
+  // when staged) so that no load sits behind a branch
+  const int q = tid & 63, r0 = tid >> 6;
+  const int qa = q < T ? q : T - 1, qb = q < W ? q : W - 1;
+  // a product's own extent fits 32-bit offsets (checked by the launcher)
+  const int a_h = (int)p.a_h, b_t = (int)p.b_t;
+  const int qoa = qa * (int)p.a_t, qob = qb * (int)p.b_w;
+  float va[16], vb[16];
+  auto fetch = [&](int64_t bi) {
+    int64_t o[3];
+    mm_batch_offsets_m(p, bi, o);
+    const float *pa = a + o[0], *pb = b + o[1];
+    // the row index re-read per product keeps the compiler from hoisting
+    // (and holding) 32 per-load offsets across the loop
+    int rr = r0;
+    asm volatile("" : "+v"(rr));
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const int r = rr + 4 * j;
+      const int ra = r < H ? r : H - 1, rb = r < T ? r : T - 1;
+      va[j] = pa[ra * a_h + qoa];
+      vb[j] = pb[rb * b_t + qob];
+    }
+  };
+  const int wr = w >> 1, wc = w & 1;
+  const int ar = 32 * wr + (lane & 31), br = 32 * wc + (lane & 31), kq = (lane >> 5) * 8;
+  // persistent blocks over the batch; the next product's operands are in
+  // flight while this one is scaled, multiplied and stored
+  int64_t bi = blockIdx.x;
+  if (bi < p.nbatch) fetch(bi);
+  for (; bi < p.nbatch; bi += gridDim.x) {
+    // per-thread LDS and store addresses are recomputed each product rather
+    // than held in registers across the loop
+    int tl = tid;
+    asm volatile("" : "+v"(tl));
+    const int ql = tl & 63, rl = tl >> 6, lanel = tl & 63;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const int r = rl + 4 * j;
+      fA[r][ql] = (r < H && ql < T) ? va[j] : 0.0f;
+      fB[ql][r] = (ql < W && r < T) ? vb[j] : 0.0f;
+    }
+    __syncthreads();
+    if (bi + gridDim.x < p.nbatch) fetch(bi + gridDim.x);
+    // row scales: four threads per row of fA and of fB, 16 values each
+    // (16-byte LDS reads), combined within the quad
+    {
+      const int r = tl >> 2, k0 = (tl & 3) * 16;
+      float ma = 0.0f, mb = 0.0f;
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        const Vec<float, 4> xa = *reinterpret_cast<const Vec<float, 4> *>(&fA[r][k0 + 4 * g]);
+        const Vec<float, 4> xb = *reinterpret_cast<const Vec<float, 4> *>(&fB[r][k0 + 4 * g]);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          ma = fmaxf(ma, fabsf(xa.v[e]));
+          mb = fmaxf(mb, fabsf(xb.v[e]));
+        }
+      }
+      ma = fmaxf(ma, __shfl_xor(ma, 1, 64));
+      mb = fmaxf(mb, __shfl_xor(mb, 1, 64));
+      ma = fmaxf(ma, __shfl_xor(ma, 2, 64));
+      mb = fmaxf(mb, __shfl_xor(mb, 2, 64));
+      if ((tl & 3) == 0) {
+        scA[r] = pow2_scale(ma, 15);
+        scB[r] = pow2_scale(mb, 15);
+      }
+    }
+    __syncthreads();
+    const float sa = scA[ar], sb = scB[br];
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = 0.f;
+    for (int k0 = 0; k0 < Tp; k0 += 16) {
+      half8 ah, al, bh, bl;
+      const Vec<float, 4> a0 = *reinterpret_cast<const Vec<float, 4> *>(&fA[ar][k0 + kq]);
+      const Vec<float, 4> a1 = *reinterpret_cast<const Vec<float, 4> *>(&fA[ar][k0 + kq + 4]);
+      const Vec<float, 4> b0 = *reinterpret_cast<const Vec<float, 4> *>(&fB[br][k0 + kq]);
+      const Vec<float, 4> b1 = *reinterpret_cast<const Vec<float, 4> *>(&fB[br][k0 + kq + 4]);
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const float ya = (i < 4 ? a0.v[i] : a1.v[i - 4]) * sa;
+        const float yb = (i < 4 ? b0.v[i] : b1.v[i - 4]) * sb;
+        const half_t ha = (half_t)ya, hb = (half_t)yb;
+        ah[i] = ha;
+        al[i] = (half_t)(ya - (float)ha);
+        bh[i] = hb;
+        bl[i] = (half_t)(yb - (float)hb);
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc, 0, 0, 0);
+    }
+    int64_t o[3];
+    mm_batch_offsets_m(p, bi, o);
+    // D layout of a 32x32 tile: lane l, reg r -> row 8*(r/4) + 4*(l/32) + r%4, col l%32
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int h = 32 * wr + 8 * (r / 4) + 4 * (lanel >> 5) + (r & 3);
+      const int wcol = 32 * wc + (lanel & 31);
+      if (h < H && wcol < W)
+        c[o[2] + wcol * p.c_w + h * p.c_h] = (acc[r] * (1.0f / scA[h])) * (1.0f / scB[wcol]);
+    }
+    __syncthreads();   // fA, fB and the scales are refilled next
+  }
+}
+
 hipError_t launch_mfma_f32(const MmPlan &p, const float *a, const float *b, float *c, int mode, hipStream_t s) {
-  // one matrix product per call (broadcast batches use the SIMT kernel), and
-  // only where a 128x128 tile is mostly useful
-  if (p.nbatch != 1 || p.H < 64 || p.W < 64 || p.T < 32) return hipErrorNotSupported;
   const bool fp8 = (mode == PDLHIP_MM_FP8);
+  // small products (broadcast batches of them in one launch)
+  auto fits32 = [](int64_t n1, int64_t i1, int64_t n2, int64_t i2) {
+    const int64_t lim = (int64_t)1 << 30;
+    return i1 < lim && i1 > -lim && i2 < lim && i2 > -lim && (n1 - 1) * (i1 < 0 ? -i1 : i1) + (n2 - 1) * (i2 < 0 ? -i2 : i2) < lim;
+  };
+  if (!fp8 && p.H >= 16 && p.W >= 16 && p.T >= 16 && p.H <= SMM && p.W <= SMM && p.T <= SMM &&
+      fits32(p.H, p.a_h, p.T, p.a_t) && fits32(p.W, p.b_w, p.T, p.b_t)) {
+    // a block per product (the block loop then runs once); more products
+    // than 2^31 - 1 loop
+    const int64_t blocks = p.nbatch < ((int64_t)1 << 31) - 1 ? p.nbatch : ((int64_t)1 << 31) - 1;
+    hipLaunchKernelGGL(split16_small, dim3((unsigned)blocks), dim3(256), 0, s, a, b, c, p);
+    return hipGetLastError();
+  }
+  // otherwise one matrix product per call (broadcast batches use the SIMT
+  // kernel), and only where a 128x128 tile is mostly useful
+  if (p.nbatch != 1 || p.H < 64 || p.W < 64 || p.T < 32) return hipErrorNotSupported;
   // 256x256 blocks once both sides fill them (FP8 then steps k by 64)
   const bool big = p.H >= 1024 && p.W >= 1024;
   const int64_t BMr = big ? 256 : MB, BNr = big ? 256 : NB, KS = (big && fp8) ? KB8 : KB;

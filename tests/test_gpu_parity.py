@@ -373,6 +373,30 @@ def test_matmult_float_modes_accuracy(mode, shape, layout):
         C.set_matmult_precision("split16")
 
 
+@pytest.mark.parametrize("shape", [(64, 64, 64), (17, 33, 50), (48, 16, 64), (64, 20, 31), (16, 16, 16), (15, 40, 40)],
+                         ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("layout", ["dense", "a transposed", "b broadcast"])
+def test_matmult_small_batches_split16(shape, layout):
+    """broadcast batches of small products (16 <= h, k, w <= 64 run one block
+    per product on the matrix cores; 15 stays on the FP32 path): per-row
+    relative error against the float64 product, rows of A over ~25 decades"""
+    h, k, w = shape
+    nb = 37
+    rng = np.random.default_rng(100 + h + 3 * k + 7 * w)
+    a = rng.standard_normal((nb, h, k)).astype(np.float32)
+    a *= np.exp(3.0 * rng.standard_normal((nb, h, 1))).astype(np.float32)
+    b = rng.standard_normal((k, w) if layout == "b broadcast" else (nb, k, w)).astype(np.float32)
+    if layout == "a transposed":   # a(t,h,n) as the transpose of a dense (h,t,n) array
+        da = C.ndarray.from_numpy(np.ascontiguousarray(a.transpose(0, 2, 1))).xchg(0, 1)
+    else:
+        da = C.ndarray.from_numpy(a)
+    d = C.matmult(da, C.ndarray.from_numpy(b)).numpy().astype(np.float64)
+    truth = np.matmul(a.astype(np.float64), b.astype(np.float64))
+    err = np.linalg.norm(d - truth, axis=2) / np.linalg.norm(truth, axis=2)
+    assert d.shape == truth.shape
+    assert err.max() < MM_TOL["split16"], (shape, layout, err.max())
+
+
 def test_matmult_bad_values_exact():
     rng = np.random.default_rng(8)
     for t in (O.F, O.D, O.L):
