@@ -549,8 +549,10 @@ static hipError_t prep_operand(const float *src, int64_t o, int64_t inc_r, int64
 }
 
 // Small float products (16 <= H, W, T <= 64), typically broadcast batches:
-// persistent blocks of 4 waves, one product at a time, nothing staged
-// outside the kernel.  A
+// a block of 4 waves per product (a grid capped at 2^31 - 1 blocks loops,
+// fetching its next product's operands while it multiplies the current
+// one; a persistent grid of resident blocks measured slower), nothing
+// staged outside the kernel.  A
 // (rows h, k = t) and B (rows w, k = t, transposed on the way in) are read
 // once in FP32 into LDS; each row of A and each column of B gets the same
 // power-of-two scale the large kernels use; every wave then owns one 32x32
@@ -594,8 +596,8 @@ __global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict_
   };
   const int wr = w >> 1, wc = w & 1;
   const int ar = 32 * wr + (lane & 31), br = 32 * wc + (lane & 31), kq = (lane >> 5) * 8;
-  // persistent blocks over the batch; the next product's operands are in
-  // flight while this one is scaled, multiplied and stored
+  // products bi, bi + gridDim.x, ...; the next one's operands are in flight
+  // while this one is scaled, multiplied and stored
   int64_t bi = blockIdx.x;
   if (bi < p.nbatch) fetch(bi);
   for (; bi < p.nbatch; bi += gridDim.x) {
