@@ -558,15 +558,17 @@ static hipError_t prep_operand(const float *src, int64_t o, int64_t inc_r, int64
 // power-of-two scale the large kernels use; every wave then owns one 32x32
 // quarter of C and, per 16-k step, reads its fragments from LDS, scales and
 // splits them into FP16 hi/lo in registers and issues the three
-// v_mfma_f32_32x32x16_f16 products.  Rows of 68 floats keep the 16-byte
-// fragment reads aligned and on distinct banks.
-constexpr int SMM = 64, SMP = 68;
+// v_mfma_f32_32x32x16_f16 products.  A's rows of 68 floats keep its 16-byte
+// fragment reads aligned and on distinct banks; B's rows of 65 floats keep
+// its transposing column writes on distinct banks.  C is unscaled by the
+// scales' exact reciprocals.
+constexpr int SMM = 64, SMP = 68, SMPB = 65;
 
 __global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict__ a, const float *__restrict__ b,
                                                         float *__restrict__ c, MmPlan p) {
   __shared__ __align__(16) float fA[SMM][SMP];
-  __shared__ __align__(16) float fB[SMM][SMP];
-  __shared__ float scA[SMM], scB[SMM];
+  __shared__ __align__(16) float fB[SMM][SMPB];
+  __shared__ float scA[SMM], scB[SMM], inA[SMM], inB[SMM];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int H = (int)p.H, W = (int)p.W, T = (int)p.T, Tp = (T + 15) & ~15;
   // staging: thread (r0, q) holds rows r0 + 4j of A(h = r, t = q) and of
@@ -622,11 +624,10 @@ __global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict_
 #pragma unroll
       for (int g = 0; g < 4; g++) {
         const Vec<float, 4> xa = *reinterpret_cast<const Vec<float, 4> *>(&fA[r][k0 + 4 * g]);
-        const Vec<float, 4> xb = *reinterpret_cast<const Vec<float, 4> *>(&fB[r][k0 + 4 * g]);
 #pragma unroll
         for (int e = 0; e < 4; e++) {
           ma = fmaxf(ma, fabsf(xa.v[e]));
-          mb = fmaxf(mb, fabsf(xb.v[e]));
+          mb = fmaxf(mb, fabsf(fB[r][k0 + 4 * g + e]));
         }
       }
       ma = fmaxf(ma, __shfl_xor(ma, 1, 64));
@@ -634,8 +635,11 @@ __global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict_
       ma = fmaxf(ma, __shfl_xor(ma, 2, 64));
       mb = fmaxf(mb, __shfl_xor(mb, 2, 64));
       if ((tl & 3) == 0) {
-        scA[r] = pow2_scale(ma, 15);
-        scB[r] = pow2_scale(mb, 15);
+        const float s1 = pow2_scale(ma, 15), s2 = pow2_scale(mb, 15);
+        scA[r] = s1;
+        scB[r] = s2;
+        inA[r] = 1.0f / s1;   // powers of two: exact
+        inB[r] = 1.0f / s2;
       }
     }
     __syncthreads();
@@ -647,12 +651,10 @@ __global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict_
       half8 ah, al, bh, bl;
       const Vec<float, 4> a0 = *reinterpret_cast<const Vec<float, 4> *>(&fA[ar][k0 + kq]);
       const Vec<float, 4> a1 = *reinterpret_cast<const Vec<float, 4> *>(&fA[ar][k0 + kq + 4]);
-      const Vec<float, 4> b0 = *reinterpret_cast<const Vec<float, 4> *>(&fB[br][k0 + kq]);
-      const Vec<float, 4> b1 = *reinterpret_cast<const Vec<float, 4> *>(&fB[br][k0 + kq + 4]);
 #pragma unroll
       for (int i = 0; i < 8; i++) {
         const float ya = (i < 4 ? a0.v[i] : a1.v[i - 4]) * sa;
-        const float yb = (i < 4 ? b0.v[i] : b1.v[i - 4]) * sb;
+        const float yb = fB[br][k0 + kq + i] * sb;
         const half_t ha = (half_t)ya, hb = (half_t)yb;
         ah[i] = ha;
         al[i] = (half_t)(ya - (float)ha);
@@ -671,7 +673,7 @@ __global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict_
       const int h = 32 * wr + 8 * (r / 4) + 4 * (lanel >> 5) + (r & 3);
       const int wcol = 32 * wc + (lanel & 31);
       if (h < H && wcol < W)
-        c[o[2] + wcol * p.c_w + h * p.c_h] = (acc[r] * (1.0f / scA[h])) * (1.0f / scB[wcol]);
+        c[o[2] + wcol * p.c_w + h * p.c_h] = (acc[r] * inA[h]) * inB[wcol];
     }
     __syncthreads();   // fA, fB and the scales are refilled next
   }
