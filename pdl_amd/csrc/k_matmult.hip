@@ -21,6 +21,7 @@
 namespace pdlhip {
 
 hipError_t launch_mfma_f32(const MmPlan &p, const float *a, const float *b, float *c, int mode, hipStream_t s);
+hipError_t launch_mfma_f64(const MmPlan &p, const double *a, const double *b, double *c, hipStream_t s);
 
 __device__ inline void mm_batch_offsets(const MmPlan &p, int64_t bidx, int64_t *o) {
   o[0] = p.off[0]; o[1] = p.off[1]; o[2] = p.off[2];
@@ -149,6 +150,13 @@ static hipError_t matmult_T(const MmPlan &p, const void *va, const void *vb, voi
   if constexpr (sizeof(T) == 4 && IsFloat<T>::v) {
     if (mode == PDLHIP_MM_SPLIT16 || mode == PDLHIP_MM_FP8) {
       hipError_t e = launch_mfma_f32(p, (const float *)a, (const float *)b, (float *)c, mode, s);
+      if (e != hipErrorNotSupported) return e;
+    }
+  }
+  if constexpr (sizeof(T) == 8 && IsFloat<T>::v) {
+    // the default mode runs double products on the FP64 matrix cores
+    if (mode == PDLHIP_MM_SPLIT16 || mode == PDLHIP_MM_FP8) {
+      hipError_t e = launch_mfma_f64(p, (const double *)a, (const double *)b, (double *)c, s);
       if (e != hipErrorNotSupported) return e;
     }
   }

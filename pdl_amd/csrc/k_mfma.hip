@@ -679,6 +679,119 @@ __global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict_
   }
 }
 
+// ---------------------------------------------------------------- FP64
+// Double matmult on v_mfma_f64_16x16x4_f64 (FP64 products and sums; lane l
+// holds A[l%16][l/16] and B[l/16][l%16]; result register r of lane l is
+// D[4r + l/16][l%16]).  128x128 blocks of 4 waves, each wave 64x64 as 4x4
+// tiles; stages of 16 k go global -> registers -> LDS as [k][row] for A
+// (transposed on the way in) and [k][col] for B, so that every fragment
+// read is 16 consecutive doubles; the next stage's loads are in flight
+// while the matrix cores work on the current one.
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int DKB = 16;
+
+template <int BM, int BN, int WR, int WC>
+__global__ __launch_bounds__(WR * WC * 64, 2) void dgemm_mfma(const double *__restrict__ a,
+                                                              const double *__restrict__ b, double *__restrict__ c,
+                                                              MmPlan p, int64_t ntm, int64_t ntn) {
+  constexpr int NT = WR * WC * 64, TI = BM / WR / 16, TJ = BN / WC / 16;
+  constexpr int PA = BM * DKB / (2 * NT), PB = BN * DKB / (2 * NT);   // passes of 2 doubles per thread
+  __shared__ __align__(16) double sA[2][DKB][BM];
+  __shared__ __align__(16) double sB[2][DKB][BN];
+  int64_t tm, tn;
+  tile_of(blockIdx.x, ntm, ntn, tm, tn);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w / WC, wc = w % WC;
+  const int64_t m0 = tm * BM, n0 = tn * BN;
+  // A: thread (row tid / 8 + rows-per-pass * pass, k pair (tid % 8) * 2), lanes along k
+  // B: thread (k tid / (BN/2) + ..., col pair (tid % (BN/2)) * 2), lanes along w
+  double ra[PA][2], rb[PB][2];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int s = 0; s < PA; s++) {
+      const int row = tid / 8 + (NT / 8) * s, kk = (tid % 8) * 2;
+      const int64_t h = m0 + row;
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        const int64_t t = k0 + kk + e;
+        ra[s][e] = (h < p.H && t < p.T) ? a[p.off[0] + h * p.a_h + t * p.a_t] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < PB; s++) {
+      const int kr = tid / (BN / 2) + (NT / (BN / 2)) * s, col = (tid % (BN / 2)) * 2;
+      const int64_t t = k0 + kr;
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        const int64_t wv = n0 + col + e;
+        rb[s][e] = (wv < p.W && t < p.T) ? b[p.off[1] + wv * p.b_w + t * p.b_t] : 0.0;
+      }
+    }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int s = 0; s < PA; s++) {
+      const int row = tid / 8 + (NT / 8) * s, kk = (tid % 8) * 2;
+      sA[buf][kk][row] = ra[s][0];
+      sA[buf][kk + 1][row] = ra[s][1];
+    }
+#pragma unroll
+    for (int s = 0; s < PB; s++) {
+      const int kr = tid / (BN / 2) + (NT / (BN / 2)) * s, col = (tid % (BN / 2)) * 2;
+      sB[buf][kr][col] = rb[s][0];
+      sB[buf][kr][col + 1] = rb[s][1];
+    }
+  };
+  d4 acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; i++)
+#pragma unroll
+    for (int j = 0; j < TJ; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  load(0);
+  stage(0);
+  __syncthreads();
+  const int fr = lane & 15, fk = lane >> 4;
+  int cur = 0;
+  for (int64_t k0 = 0; k0 < p.T; k0 += DKB) {
+    const bool more = k0 + DKB < p.T;
+    if (more) load(k0 + DKB);
+#pragma unroll
+    for (int kk = 0; kk < DKB; kk += 4) {
+      double af[TI], bf[TJ];
+#pragma unroll
+      for (int i = 0; i < TI; i++) af[i] = sA[cur][kk + fk][wr * 16 * TI + i * 16 + fr];
+#pragma unroll
+      for (int j = 0; j < TJ; j++) bf[j] = sB[cur][kk + fk][wc * 16 * TJ + j * 16 + fr];
+#pragma unroll
+      for (int i = 0; i < TI; i++)
+#pragma unroll
+        for (int j = 0; j < TJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) stage(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int i = 0; i < TI; i++)
+#pragma unroll
+    for (int j = 0; j < TJ; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int64_t h = m0 + wr * 16 * TI + i * 16 + 4 * r + fk;
+        const int64_t wv = n0 + wc * 16 * TJ + j * 16 + fr;
+        if (h < p.H && wv < p.W) c[p.off[2] + wv * p.c_w + h * p.c_h] = acc[i][j][r];
+      }
+}
+
+hipError_t launch_mfma_f64(const MmPlan &p, const double *a, const double *b, double *c, hipStream_t s) {
+  // one product per call, where a 128x128 block is mostly useful
+  if (p.nbatch != 1 || p.H < 64 || p.W < 64 || p.T < 16) return hipErrorNotSupported;
+  const int64_t ntm = (p.H + 127) / 128, ntn = (p.W + 127) / 128;
+  if (ntm * ntn >= ((int64_t)1 << 31)) return hipErrorNotSupported;
+  hipLaunchKernelGGL((dgemm_mfma<128, 128, 2, 2>), dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, a, b, c, p, ntm, ntn);
+  return hipGetLastError();
+}
+
 hipError_t launch_mfma_f32(const MmPlan &p, const float *a, const float *b, float *c, int mode, hipStream_t s) {
   const bool fp8 = (mode == PDLHIP_MM_FP8);
   // small products (broadcast batches of them in one launch)

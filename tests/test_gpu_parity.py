@@ -397,6 +397,26 @@ def test_matmult_small_batches_split16(shape, layout):
     assert err.max() < MM_TOL["split16"], (shape, layout, err.max())
 
 
+@pytest.mark.parametrize("shape", [(300, 257, 190), (64, 16, 64), (1100, 100, 1030), (129, 33, 70)],
+                         ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("layout", ["dense", "a transposed"])
+def test_matmult_double_default_accuracy(shape, layout):
+    """double products in the default mode (FP64 matrix cores from 64 per
+    side): per-row relative error against a long-double product"""
+    h, k, w = shape
+    rng = np.random.default_rng(500 + h + k)
+    a = rng.standard_normal((h, k)) * np.exp(3.0 * rng.standard_normal((h, 1)))
+    b = rng.standard_normal((k, w))
+    if layout == "dense":
+        da = C.ndarray.from_numpy(a, O.D)
+    else:
+        da = C.ndarray.from_numpy(np.ascontiguousarray(a.T), O.D).xchg(0, 1)
+    d = C.matmult(da, C.ndarray.from_numpy(b, O.D)).numpy()
+    truth = a.astype(np.longdouble) @ b.astype(np.longdouble)
+    err = np.linalg.norm((d - truth).astype(np.float64), axis=1) / np.linalg.norm(truth.astype(np.float64), axis=1)
+    assert err.max() < 2e-14, (shape, layout, err.max())
+
+
 def test_matmult_bad_values_exact():
     rng = np.random.default_rng(8)
     for t in (O.F, O.D, O.L):
