@@ -784,10 +784,12 @@ __global__ __launch_bounds__(WR * WC * 64, 2) void dgemm_mfma(const double *__re
 }
 
 hipError_t launch_mfma_f64(const MmPlan &p, const double *a, const double *b, double *c, hipStream_t s) {
-  // one product per call, where a 128x128 block is mostly useful
+  // one product per call, where a 128x128 block is mostly useful and there
+  // are at least 64 of them (fewer leave most CUs idle: the SIMT kernel's
+  // smaller tiles win below about 1024x1024)
   if (p.nbatch != 1 || p.H < 64 || p.W < 64 || p.T < 16) return hipErrorNotSupported;
   const int64_t ntm = (p.H + 127) / 128, ntn = (p.W + 127) / 128;
-  if (ntm * ntn >= ((int64_t)1 << 31)) return hipErrorNotSupported;
+  if (ntm * ntn < 64 || ntm * ntn >= ((int64_t)1 << 31)) return hipErrorNotSupported;
   hipLaunchKernelGGL((dgemm_mfma<128, 128, 2, 2>), dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, a, b, c, p, ntm, ntn);
   return hipGetLastError();
 }

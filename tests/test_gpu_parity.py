@@ -397,12 +397,13 @@ def test_matmult_small_batches_split16(shape, layout):
     assert err.max() < MM_TOL["split16"], (shape, layout, err.max())
 
 
-@pytest.mark.parametrize("shape", [(300, 257, 190), (64, 16, 64), (1100, 100, 1030), (129, 33, 70)],
+@pytest.mark.parametrize("shape", [(300, 257, 190), (1100, 100, 1030), (1030, 40, 1100), (129, 33, 70)],
                          ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("layout", ["dense", "a transposed"])
 def test_matmult_double_default_accuracy(shape, layout):
-    """double products in the default mode (FP64 matrix cores from 64 per
-    side): per-row relative error against a long-double product"""
+    """double products in the default mode (FP64 matrix cores once there are
+    64 or more 128x128 blocks, the SIMT kernel below): per-row relative error
+    against a long-double product"""
     h, k, w = shape
     rng = np.random.default_rng(500 + h + k)
     a = rng.standard_normal((h, k)) * np.exp(3.0 * rng.standard_normal((h, 1)))
