@@ -26,14 +26,28 @@ __device__ inline void list_tile(const MmItem &it, int64_t tm, int64_t tn, T (*A
   for (int i = 0; i < PT; i++)
 #pragma unroll
     for (int j = 0; j < PT; j++) acc[i][j] = T(0);
+  constexpr int NE = TS * KS / 256;   // staged elements per thread per operand
   for (int64_t k0 = 0; k0 < it.T; k0 += KS) {
-    for (int e = tid; e < TS * KS; e += 256) {
-      int kk = e % KS, mm = e / KS;
-      int64_t gh = h0 + mm, gt = k0 + kk;
-      As[kk][mm] = (gh < it.H && gt < it.T) ? a[it.oa + gt * it.a_t + gh * it.a_h] : T(0);
-      int nn = e % TS, kb = e / TS;
-      int64_t gw = w0 + nn, gt2 = k0 + kb;
-      Bs[kb][nn] = (gw < it.W && gt2 < it.T) ? b[it.ob + gw * it.b_w + gt2 * it.b_t] : T(0);
+    // every load of the chunk is issued before the first LDS write; indices
+    // are clamped into the operand and the element zeroed afterwards, so no
+    // load sits behind a branch
+    T ra[NE], rb[NE];
+#pragma unroll
+    for (int q = 0; q < NE; q++) {
+      const int e = tid + 256 * q;
+      const int kk = e % KS, mm = e / KS, nn = e % TS, kb = e / TS;
+      const int64_t gh = h0 + mm, gt = k0 + kk, gw = w0 + nn, gt2 = k0 + kb;
+      const int64_t ch = gh < it.H ? gh : it.H - 1, ct = gt < it.T ? gt : it.T - 1;
+      const int64_t cw = gw < it.W ? gw : it.W - 1, ct2 = gt2 < it.T ? gt2 : it.T - 1;
+      ra[q] = a[it.oa + ct * it.a_t + ch * it.a_h];
+      rb[q] = b[it.ob + cw * it.b_w + ct2 * it.b_t];
+    }
+#pragma unroll
+    for (int q = 0; q < NE; q++) {
+      const int e = tid + 256 * q;
+      const int kk = e % KS, mm = e / KS, nn = e % TS, kb = e / TS;
+      As[kk][mm] = (h0 + mm < it.H && k0 + kk < it.T) ? ra[q] : T(0);
+      Bs[kb][nn] = (w0 + nn < it.W && k0 + kb < it.T) ? rb[q] : T(0);
     }
     __syncthreads();
     const int kmax = (it.T - k0 < KS) ? (int)(it.T - k0) : KS;
