@@ -151,22 +151,42 @@ __global__ __launch_bounds__(EW_THREADS) void ew_tile(const T *__restrict__ a, c
   {
     const int jj = (threadIdx.x % LJ) * V;
     const int ii0 = threadIdx.x / LJ;
+    if (V > 1 && full) {
+      // every row pass's vector load is issued before the first LDS write
+      constexpr int NP = TI / LR;
+      Vec<T, V> xv[2][NP];
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
-      if (!(TMASK & (1 << k))) continue;
-      constexpr int dummy = 0;
-      (void)dummy;
-      const int slot = (k == 1 && (TMASK & 1)) ? 1 : 0;
-      const int64_t inc_i = p.inc[k][0];
+      for (int k = 0; k < 2; k++) {
+        if (!(TMASK & (1 << k))) continue;
+        const int64_t inc_i = p.inc[k][0];
+        const T *sp = src[k] + o[k] + (i0 + ii0) * inc_i + j0 + jj;
+        if (stream) {
 #pragma unroll
-      for (int ii = ii0; ii < TI; ii += LR) {
-        const int64_t gi = i0 + ii, gj = j0 + jj;
-        const T *sp = src[k] + o[k] + gi * inc_i + gj;
-        if (V > 1 && full) {
-          Vec<T, V> v = stream ? load_nt<T, V>(sp) : *reinterpret_cast<const Vec<T, V> *>(sp);
-#pragma unroll
-          for (int e = 0; e < V; e++) lds[slot][ii][jj + e] = v.v[e];
+          for (int q = 0; q < NP; q++) xv[k][q] = load_nt<T, V>(sp + (int64_t)q * LR * inc_i);
         } else {
+#pragma unroll
+          for (int q = 0; q < NP; q++) xv[k][q] = *reinterpret_cast<const Vec<T, V> *>(sp + (int64_t)q * LR * inc_i);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        if (!(TMASK & (1 << k))) continue;
+        const int slot = (k == 1 && (TMASK & 1)) ? 1 : 0;
+#pragma unroll
+        for (int q = 0; q < NP; q++)
+#pragma unroll
+          for (int e = 0; e < V; e++) lds[slot][ii0 + q * LR][jj + e] = xv[k][q].v[e];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        if (!(TMASK & (1 << k))) continue;
+        const int slot = (k == 1 && (TMASK & 1)) ? 1 : 0;
+        const int64_t inc_i = p.inc[k][0];
+#pragma unroll
+        for (int ii = ii0; ii < TI; ii += LR) {
+          const int64_t gi = i0 + ii, gj = j0 + jj;
+          const T *sp = src[k] + o[k] + gi * inc_i + gj;
 #pragma unroll
           for (int e = 0; e < V; e++)
             if (gi < ni && gj + e < nj) lds[slot][ii][jj + e] = sp[e];
