@@ -564,6 +564,7 @@ static hipError_t prep_operand(const float *src, int64_t o, int64_t inc_r, int64
 // scales' exact reciprocals.
 constexpr int SMM = 64, SMP = 68, SMPB = 65;
 
+template <bool VEC>
 __global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict__ a, const float *__restrict__ b,
                                                         float *__restrict__ c, MmPlan p) {
   __shared__ __align__(16) float fA[SMM][SMP];
@@ -588,12 +589,28 @@ __global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict_
     // (and holding) 32 per-load offsets across the loop
     int rr = r0;
     asm volatile("" : "+v"(rr));
+    if constexpr (VEC) {
+      // 64x64 unit-stride aligned operands: thread (row tid/16 + 16j, 4
+      // consecutive k or w at (tid%16)*4) with 16-byte loads
+      const int rv = (tid >> 4) + (rr - r0), cq = (tid & 15) * 4;
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-      const int r = rr + 4 * j;
-      const int ra = r < H ? r : H - 1, rb = r < T ? r : T - 1;
-      va[j] = pa[ra * a_h + qoa];
-      vb[j] = pb[rb * b_t + qob];
+      for (int j = 0; j < 4; j++) {
+        const Vec<float, 4> x = *reinterpret_cast<const Vec<float, 4> *>(pa + (rv + 16 * j) * a_h + cq);
+        const Vec<float, 4> y = *reinterpret_cast<const Vec<float, 4> *>(pb + (rv + 16 * j) * b_t + cq);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          va[4 * j + e] = x.v[e];
+          vb[4 * j + e] = y.v[e];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        const int r = rr + 4 * j;
+        const int ra = r < H ? r : H - 1, rb = r < T ? r : T - 1;
+        va[j] = pa[ra * a_h + qoa];
+        vb[j] = pb[rb * b_t + qob];
+      }
     }
   };
   const int wr = w >> 1, wc = w & 1;
@@ -608,11 +625,24 @@ __global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict_
     int tl = tid;
     asm volatile("" : "+v"(tl));
     const int ql = tl & 63, rl = tl >> 6, lanel = tl & 63;
+    if constexpr (VEC) {
+      const int rv = tl >> 4, cq = (tl & 15) * 4;
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-      const int r = rl + 4 * j;
-      fA[r][ql] = (r < H && ql < T) ? va[j] : 0.0f;
-      fB[ql][r] = (ql < W && r < T) ? vb[j] : 0.0f;
+      for (int j = 0; j < 4; j++) {
+        Vec<float, 4> x;
+#pragma unroll
+        for (int e = 0; e < 4; e++) x.v[e] = va[4 * j + e];
+        *reinterpret_cast<Vec<float, 4> *>(&fA[rv + 16 * j][cq]) = x;
+#pragma unroll
+        for (int e = 0; e < 4; e++) fB[cq + e][rv + 16 * j] = vb[4 * j + e];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        const int r = rl + 4 * j;
+        fA[r][ql] = (r < H && ql < T) ? va[j] : 0.0f;
+        fB[ql][r] = (ql < W && r < T) ? vb[j] : 0.0f;
+      }
     }
     __syncthreads();
     if (bi + gridDim.x < p.nbatch) fetch(bi + gridDim.x);
@@ -806,7 +836,13 @@ hipError_t launch_mfma_f32(const MmPlan &p, const float *a, const float *b, floa
     // a block per product (the block loop then runs once); more products
     // than 2^31 - 1 loop
     const int64_t blocks = p.nbatch < ((int64_t)1 << 31) - 1 ? p.nbatch : ((int64_t)1 << 31) - 1;
-    hipLaunchKernelGGL(split16_small, dim3((unsigned)blocks), dim3(256), 0, s, a, b, c, p);
+    // 64x64x64 products with unit-stride k (A) and w (B), every product's
+    // rows 16-byte aligned: 16-byte staging loads
+    bool vec = p.H == SMM && p.W == SMM && p.T == SMM && p.a_t == 1 && p.b_w == 1 && p.a_h % 4 == 0 &&
+               p.b_t % 4 == 0 && ((uintptr_t)(a + p.off[0])) % 16 == 0 && ((uintptr_t)(b + p.off[1])) % 16 == 0;
+    for (int d = 0; d < p.nd && vec; d++) vec = p.inc[0][d] % 4 == 0 && p.inc[1][d] % 4 == 0;
+    if (vec) hipLaunchKernelGGL(split16_small<true>, dim3((unsigned)blocks), dim3(256), 0, s, a, b, c, p);
+    else hipLaunchKernelGGL(split16_small<false>, dim3((unsigned)blocks), dim3(256), 0, s, a, b, c, p);
     return hipGetLastError();
   }
   // otherwise one matrix product per call (broadcast batches use the SIMT
