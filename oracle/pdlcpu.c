@@ -26,8 +26,11 @@
  *                               BAD/NaN rows
  *   RNG                         oracle/_ref (the reference's own
  *                               Primitive-xoshiro256plus.c compiled here)
- *   power / atan2 (libm)        t/ops.t:33,126-127 only: beyond that PARITY
- *                               UNPINNED (glibc's pow/atan2 are the reference)
+ *   power / atan2 (libm)        t/ops.t:33,126-127 vectors, plus
+ *                               tests/test_oracle_libm.py: within 1 ulp of a
+ *                               120-bit mpmath value on random operands in
+ *                               every sign/quadrant, and C99 Annex F special
+ *                               cases (zeros, infinities, NaN) bit-exact
  */
 #include "pdlcpu.h"
 

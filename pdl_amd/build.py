@@ -42,8 +42,19 @@ def units():
     return u
 
 
-def compile_one(src, name, extra):
+def _fresh(out, src):
+    """True when out is newer than its source and every shared header."""
+    if not os.path.exists(out):
+        return False
+    t = os.path.getmtime(out)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
+    return all(os.path.getmtime(d) <= t for d in deps)
+
+
+def compile_one(src, name, extra, force=True):
     out = os.path.join(OBJ, name + ".o")
+    if not force and _fresh(out, src):
+        return out
     cmd = [HIPCC] + FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -51,12 +62,13 @@ def compile_one(src, name, extra):
     return out
 
 
-def build(jobs=None, verbose=False):
+def build(jobs=None, verbose=False, force=True):
+    """Compiles every unit (force=True) or only the stale ones, then links."""
     os.makedirs(OBJ, exist_ok=True)
     jobs = jobs or min(16, os.cpu_count() or 4)
     objs = []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = [ex.submit(compile_one, *u) for u in units()]
+        futs = [ex.submit(compile_one, *u, force=force) for u in units()]
         for f in futs:
             objs.append(f.result())
             if verbose:
@@ -71,4 +83,4 @@ def build(jobs=None, verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, force="--changed" not in sys.argv))

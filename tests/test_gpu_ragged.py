@@ -96,8 +96,10 @@ def make_items(rng, t, lens, packed):
     return items_o, items_d
 
 
-# libm power on the device and in glibc may differ by an ulp or two per
-# element (the elementwise binop tests hold power to these relative bounds)
+# a power node (x ** 2 is the correctly rounded x*x; other exponents the
+# device's powf/pow) is within 2 ulp of glibc's value per element (the
+# elementwise binop tests check that bound); these relative bounds cover
+# 2-3 ulp of the chain's type
 POW_TOL = {O.F: 4e-7, O.D: 4e-15}
 
 
