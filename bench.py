@@ -136,6 +136,25 @@ def run_gpu(args):
     want = np.where(good, i16h.astype(np.int64), 0).sum(axis=1)
     want = np.where(good.any(axis=1), want, -2**31).astype(np.int32)
     check["i16bad_sumover_exact"] = bool(np.array_equal(ib_out.numpy(), want))
+    # sampled k-planes of z (bit-exact float32 products) and of the four float
+    # reductions (sums within the compensated bound, min/max exact)
+    sums = {k: outs[k].numpy() for k in outs}
+    z_ok, red_ok = True, True
+    for k in (0, NK // 2, NK - 1):
+        xk = x.slice(None, None, (k, k, 1)).numpy()[0]          # xk[j, i] = x(i, j, k)
+        yk = y.slice(None, (k, k, 1)).numpy()[0]                # yk[i] = y(i, k)
+        zk = z.slice(None, None, (k, k, 1)).numpy()[0]          # zk[j, i] = z(i, j, k) = x(j, i, k) * y(i, k)
+        want = xk.T * yk[None, :]
+        z_ok &= bool(np.array_equal(zk.view(np.uint32), want.view(np.uint32)))
+        tr = want.astype(np.float64).sum(axis=1)
+        ab = np.abs(want).astype(np.float64).sum(axis=1)
+        for name, t, a in (("sumover", tr, ab), ("average", tr / NX, ab / NX)):
+            bound = 2 * 2.0 ** -23 * np.abs(t) + 64 * 2.0 ** -46 * a + 1e-30
+            red_ok &= bool(np.all(np.abs(sums[name][k].astype(np.float64) - t) <= bound))
+        red_ok &= bool(np.array_equal(sums["minimum"][k], want.min(axis=1)))
+        red_ok &= bool(np.array_equal(sums["maximum"][k], want.max(axis=1)))
+    check["z_sampled_planes_exact"] = z_ok
+    check["f32_reductions_sampled_planes"] = red_ok
     return {
         "value": value, "ms_per_step": ms_per_step, "ms_per_step_events": step_ms_events,
         "per_op": per_op, "dominant": dom, "dominant_bytes": nbytes[dom], "total_bytes": total_bytes,

@@ -168,7 +168,7 @@ typedef struct vtable {
   const int *gentypes;           /* -1 terminated */
   readdata_fn readdata;
   int opcode;
-  int badmode_biop;              /* 1: biop gating, 2: bifunc gating */
+  int badmode_biop;              /* 1: biop gating, 2: bifunc gating, 3: double OtherPar */
 } vtable;
 
 /* GenericTypes lists, in the order PP emits them (Ops.pd:1-14) */
@@ -178,6 +178,8 @@ static const int GT_INT[] = {0, 1, 2, 3, 4, 5, 6, 7, 8, -1};
 /* [@$C, @$F] with $F = F,LD,D (D moved last so non-floats default to D) */
 static const int GT_POWER[] = {12, 13, 14, 9, 11, 10, -1};
 static const int GT_ATAN2[] = {9, 11, 10, -1};
+/* $AF of Ops.pd/Bad.pd: the non-integer types, D moved last */
+static const int GT_AF[] = {9, 11, 12, 13, 14, 10, -1};
 
 /* ------------------------------------------------------------------ */
 /* Broadcast struct (pdlbroadcast.h:18-61), implicit broadcasting only */
@@ -198,6 +200,7 @@ typedef struct trans {
   int datatype;
   int bvalflag;
   int64_t swap;
+  double dpar;                 /* a double OtherPar (Bad.pd: value / newval) */
   brc_t brc;
   int64_t ind_sizes[3];
   int64_t inc_sizes[MAXP][2];
@@ -433,6 +436,129 @@ static void rd_assgn(trans *tr, int thr) {
 #undef X
   default: break;
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* Ops.pd ufunc (Ops.pd:222-265, 327-333, 356-381): b() =              */
+/* ($GENERIC(b))func($a()) through <tgmath.h> (integer arguments take  */
+/* the double function), bad mode: ISBAD(a) -> SETBAD(b).  _rabs       */
+/* (Ops.pd:343-358): unsigned copy, PDL_ABS otherwise.                 */
+/* ------------------------------------------------------------------ */
+enum { U_BITNOT = 100, U_SQRT, U_SIN, U_COS, U_NOT, U_EXP, U_LOG, U_LOG10, U_ABS };
+
+#define TG1(fn, x) _Generic((x), float: fn##f, default: fn)(x)
+
+#define OPSUFUNC_LOOP(ctype, ppsym, EXPR) do { \
+  DECL(0, ctype, ppsym, a) DECL(1, ctype, ppsym, b) \
+  if (!tr->bvalflag) { \
+    BLOOP(tr, thr, { ctype av = a[O0]; (void)av; b[O1] = (ctype)(EXPR); }); \
+  } else { \
+    BLOOP(tr, thr, { ctype av = a[O0]; (void)av; \
+      if (ISBAD2(av, a_bv, a_bvnan, ppsym)) b[O1] = b_bv; else b[O1] = (ctype)(EXPR); }); \
+  } \
+} while (0)
+
+static void rd_opsufunc(trans *tr, int thr) {
+  const int op = tr->vt->opcode;
+#define X(code, ctype, ppsym) case code: \
+    if (op == U_SQRT) OPSUFUNC_LOOP(ctype, ppsym, TG1(sqrt, av)); \
+    else if (op == U_SIN) OPSUFUNC_LOOP(ctype, ppsym, TG1(sin, av)); \
+    else if (op == U_COS) OPSUFUNC_LOOP(ctype, ppsym, TG1(cos, av)); \
+    else if (op == U_NOT) OPSUFUNC_LOOP(ctype, ppsym, !av); \
+    else if (op == U_EXP) OPSUFUNC_LOOP(ctype, ppsym, TG1(exp, av)); \
+    else if (op == U_LOG) OPSUFUNC_LOOP(ctype, ppsym, TG1(log, av)); \
+    else if (op == U_LOG10) OPSUFUNC_LOOP(ctype, ppsym, TG1(log10, av)); \
+    break;
+  switch (tr->datatype) {
+    ARITH_CASES(X)
+  default: break;
+  }
+#undef X
+  if (op == U_BITNOT) {
+    switch (tr->datatype) {
+#define X(code, ctype, ppsym) case code: OPSUFUNC_LOOP(ctype, ppsym, ~av); break;
+      INT_CASES(X)
+#undef X
+    default: break;
+    }
+  }
+  if (op == U_ABS) {
+    switch (tr->datatype) {
+#define X(code, ctype, ppsym) case code: OPSUFUNC_LOOP(ctype, ppsym, av); break;
+      UNSIGNED_CASES(X)
+#undef X
+#define X(code, ctype, ppsym) case code: OPSUFUNC_LOOP(ctype, ppsym, PDL_ABS(av)); break;
+      SIGNED_AND_FLOAT_CASES(X)
+#undef X
+    default: break;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Bad.pd elementwise ops.  $ISBAD compiles to PDL_ISBAD2 in good and  */
+/* bad code alike (PP/PDLCode.pm:664-668); only PDL_IF_BAD switches.   */
+/*   isbad / isgood      Bad.pd:343-395: b = IF_BAD(ISBAD(a), 0) / IF_BAD(ISGOOD(a), 1) */
+/*   setbadif            Bad.pd:584-634                                 */
+/*   setvaltobad         Bad.pd:639-676                                 */
+/*   setnantobad etc.    Bad.pd:679-775                                 */
+/*   setbadtonan         Bad.pd:777-806                                 */
+/*   setbadtoval         Bad.pd:808-840                                 */
+/* ------------------------------------------------------------------ */
+enum { B_ISBAD = 200, B_ISGOOD, B_SETBADIF, B_SETVALTOBAD, B_SETNANTOBAD, B_SETINFTOBAD, B_SETNONFINITETOBAD,
+       B_SETBADTONAN, B_SETBADTOVAL };
+
+static void rd_badop(trans *tr, int thr) {
+  const int op = tr->vt->opcode;
+  const int bad = tr->bvalflag;
+  if (op == B_ISBAD || op == B_ISGOOD) {
+    int32_t *b = (int32_t *)tr->pdls[1]->data;
+    switch (tr->datatype) {
+#define X(code, ctype, ppsym) case code: { DECL(0, ctype, ppsym, a) \
+      BLOOP(tr, thr, { int isb = ISBAD2(a[O0], a_bv, a_bvnan, ppsym); \
+        b[O1] = op == B_ISBAD ? (bad ? isb : 0) : (bad ? !isb : 1); }); break; }
+      ARITH_CASES(X)
+#undef X
+    default: break;
+    }
+    return;
+  }
+  if (op == B_SETBADIF) {
+    int32_t *m = (int32_t *)tr->pdls[1]->data;
+    int32_t m_bv;
+    memcpy(&m_bv, &tr->badval[1].value, sizeof m_bv);
+    switch (tr->datatype) {
+#define X(code, ctype, ppsym) case code: { DECL(0, ctype, ppsym, a) DECL(2, ctype, ppsym, b) (void)a_bv; (void)a_bvnan; \
+      BLOOP(tr, thr, { if ((bad && m[O1] == m_bv) || m[O1]) b[O2] = b_bv; else b[O2] = a[O0]; }); break; }
+      ARITH_CASES(X)
+#undef X
+    default: break;
+    }
+    tr->setbad[2] = 1;
+    return;
+  }
+  char flag = 0;
+  switch (tr->datatype) {
+#define X(code, ctype, ppsym) case code: { DECL(0, ctype, ppsym, a) DECL(1, ctype, ppsym, b) \
+    (void)a_bvnan; (void)b_bv; \
+    const ctype val = (ctype)tr->dpar; (void)val; \
+    BLOOP(tr, thr, { ctype av = a[O0]; \
+      switch (op) { \
+      case B_SETVALTOBAD: if (av == val) b[O1] = b_bv; else b[O1] = av; break; \
+      case B_SETNANTOBAD: if (ISNAN_##ppsym(av)) { b[O1] = b_bv; flag = 1; } else b[O1] = av; break; \
+      case B_SETINFTOBAD: if (!isfinite((double)av) && !ISNAN_##ppsym(av)) { b[O1] = b_bv; flag = 1; } \
+                          else b[O1] = av; break; \
+      case B_SETNONFINITETOBAD: if (!isfinite((double)av)) { b[O1] = b_bv; flag = 1; } else b[O1] = av; break; \
+      case B_SETBADTONAN: b[O1] = ISBAD2(av, a_bv, a_bvnan, ppsym) ? (ctype)NAN : av; break; \
+      case B_SETBADTOVAL: b[O1] = (bad && ISBAD2(av, a_bv, a_bvnan, ppsym)) ? val : av; break; \
+      default: break; } }); \
+    break; }
+    ARITH_CASES(X)
+#undef X
+  default: break;
+  }
+  if (op == B_SETVALTOBAD || flag) tr->setbad[1] = 1;
+  if (op == B_SETBADTONAN || op == B_SETBADTOVAL) tr->setgood[1] = 1;
 }
 
 /* ------------------------------------------------------------------ */
@@ -722,6 +848,11 @@ static void rd_append(trans *tr, int thr) {
 /* ------------------------------------------------------------------ */
 #define BIOP_VT(nm, opc, gt, rd, mode) \
   {nm, 3, 2, {"a", "b", "c"}, {0, 0, 0}, {{0}}, {0}, {0, 0, P_ISOUT | P_ISCREAT}, {-1, -1, -1}, gt, rd, opc, mode}
+#define UFUNC_VT(nm, opc, gt) \
+  {nm, 2, 1, {"a", "b"}, {0, 0}, {{0}}, {0}, {0, P_ISOUT | P_ISCREAT}, {-1, -1}, gt, rd_opsufunc, opc, 0}
+/* mode 3: takes a double OtherPar */
+#define BADOP_VT(nm, opc, gt, mode) \
+  {nm, 2, 1, {"a", "b"}, {0, 0}, {{0}}, {0}, {0, P_ISOUT | P_ISCREAT}, {-1, -1}, gt, rd_badop, opc, mode}
 #define RED_VT(nm, opc, oflags, otype) \
   {nm, 2, 1, {"a", "b"}, {1, 0}, {{0}, {0}}, {"n"}, {0, P_ISOUT | P_ISCREAT | (oflags)}, {-1, otype}, NULL, rd_ufunc, opc, 0}
 #define RED_VT_C(nm, opc, oflags, otype) \
@@ -750,6 +881,27 @@ static vtable vtables[] = {
   BIOP_VT("modulo", OP_MODULO, GT_REAL, rd_bifunc, 2),
   BIOP_VT("spaceship", OP_SPACESHIP, GT_REAL, rd_bifunc, 2),
   {"assgn", 2, 1, {"a", "b"}, {0, 0}, {{0}}, {0}, {0, P_ISOUT | P_ISCREAT}, {-1, -1}, GT_ALL, rd_assgn, OP_ASSGN, 0},
+  UFUNC_VT("bitnot", U_BITNOT, GT_INT),
+  UFUNC_VT("sqrt", U_SQRT, GT_ALL),
+  UFUNC_VT("sin", U_SIN, GT_ALL),
+  UFUNC_VT("cos", U_COS, GT_ALL),
+  UFUNC_VT("not", U_NOT, GT_REAL),
+  UFUNC_VT("exp", U_EXP, GT_POWER),
+  UFUNC_VT("log", U_LOG, GT_POWER),
+  UFUNC_VT("log10", U_LOG10, GT_ALL),
+  UFUNC_VT("abs", U_ABS, GT_REAL),
+  {"isbad", 2, 1, {"a", "b"}, {0, 0}, {{0}}, {0}, {0, P_ISOUT | P_ISCREAT | P_ISTYPED}, {-1, PDLC_L}, GT_ALL,
+   rd_badop, B_ISBAD, 0},
+  {"isgood", 2, 1, {"a", "b"}, {0, 0}, {{0}}, {0}, {0, P_ISOUT | P_ISCREAT | P_ISTYPED}, {-1, PDLC_L}, GT_ALL,
+   rd_badop, B_ISGOOD, 0},
+  {"setbadif", 3, 2, {"a", "mask", "b"}, {0, 0, 0}, {{0}}, {0}, {0, P_ISTYPED, P_ISOUT | P_ISCREAT}, {-1, PDLC_L, -1},
+   GT_ALL, rd_badop, B_SETBADIF, 0},
+  BADOP_VT("setvaltobad", B_SETVALTOBAD, GT_ALL, 3),
+  BADOP_VT("setnantobad", B_SETNANTOBAD, GT_AF, 0),
+  BADOP_VT("setinftobad", B_SETINFTOBAD, GT_AF, 0),
+  BADOP_VT("setnonfinitetobad", B_SETNONFINITETOBAD, GT_AF, 0),
+  BADOP_VT("setbadtonan", B_SETBADTONAN, GT_AF, 0),
+  BADOP_VT("setbadtoval", B_SETBADTOVAL, GT_ALL, 3),
   RED_VT("sumover", R_SUMOVER, P_ISTYPED | P_ISTPLUS, PDLC_L),
   RED_VT("prodover", R_PRODOVER, P_ISTYPED | P_ISTPLUS, PDLC_L),
   RED_VT("dsumover", R_DSUMOVER, P_ISTYPED, PDLC_D),
@@ -1186,6 +1338,29 @@ static void run_readdata(trans *tr) {
 /* ------------------------------------------------------------------ */
 void pdlc_free(void *p) { free(p); }
 
+static pdlc_plan *g_capture;
+void pdlc_capture_plan(pdlc_plan *dst) { g_capture = dst; }
+
+static void capture_plan(const trans *tr) {
+  if (!g_capture) return;
+  pdlc_plan *c = g_capture;
+  memset(c, 0, sizeof *c);
+  c->npdls = tr->brc.npdls;
+  c->ndims = tr->brc.ndims;
+  for (int d = 0; d < tr->brc.ndims && d < 20; d++) {
+    c->dims[d] = tr->brc.dims[d];
+    for (int j = 0; j < tr->brc.npdls; j++) c->incs[d * tr->brc.npdls + j] = tr->brc.incs[d * tr->brc.npdls + j];
+  }
+  for (int j = 0; j < tr->brc.npdls && j < 6; j++) {
+    c->offs[j] = tr->brc.offs[j];
+    c->inc_sizes[j][0] = tr->inc_sizes[j][0];
+    c->inc_sizes[j][1] = tr->inc_sizes[j][1];
+  }
+  for (int k = 0; k < 3; k++) c->ind_sizes[k] = tr->ind_sizes[k];
+  c->datatype = tr->datatype;
+  c->bvalflag = tr->bvalflag;
+}
+
 int pdlc_run(const char *op, pdlc_pdl **user_pdls, int npdls, const int64_t *otherpars, int nthreads,
              int pthread_size_mi, pdlc_error *err) {
   memset(err, 0, sizeof *err);
@@ -1198,7 +1373,7 @@ int pdlc_run(const char *op, pdlc_pdl **user_pdls, int npdls, const int64_t *oth
   int i, j;
   for (i = 0; i < npdls; i++) tr.pdls[i] = user_pdls[i];
   /* biop/bifunc HdrCode: swap the inputs (Ops.pd:108-114) */
-  if (vt->badmode_biop && otherpars && otherpars[0]) {
+  if ((vt->badmode_biop == 1 || vt->badmode_biop == 2) && otherpars && otherpars[0]) {
     pdlc_pdl *t = tr.pdls[0]; tr.pdls[0] = tr.pdls[1]; tr.pdls[1] = t;
   }
   /* bvalflag: any input with the BADVAL state (pdlapi.c:760-766) */
@@ -1296,7 +1471,9 @@ int pdlc_run(const char *op, pdlc_pdl **user_pdls, int npdls, const int64_t *oth
     effective_badval(tr.pdls[i], &tr.badval[i], &isn);
     tr.badval_isnan[i] = isn;
   }
-  if (vt->badmode_biop && otherpars) tr.swap = otherpars[0];
+  if ((vt->badmode_biop == 1 || vt->badmode_biop == 2) && otherpars) tr.swap = otherpars[0];
+  if (vt->badmode_biop == 3 && otherpars) memcpy(&tr.dpar, &otherpars[0], sizeof tr.dpar);
+  capture_plan(&tr);
   run_readdata(&tr);
   for (i = 0; i < npdls; i++) {
     if (tr.setgood[i]) tr.pdls[i]->state &= ~PDLC_BADVAL;

@@ -85,6 +85,22 @@ typedef struct {
 int pdlc_run(const char *op, pdlc_pdl **pdls, int npdls, const int64_t *otherpars,
              int nthreads, int pthread_size_mi, pdlc_error *err);
 
+/* The broadcast plan of the next pdlc_run calls (after redodims, before the
+ * loop), for tests that hand PDL's own plan to a readdata-level entry:
+ * dims/incs (incs[d*npdls + p]) and offsets of the broadcast struct
+ * (pdlbroadcast.h:18-61), the trans's core ind_sizes and inc_sizes
+ * (pdlapi.c:846-864), the trans type and bvalflag.  pdlc_capture_plan(dst)
+ * fills *dst on each later run until called with NULL. */
+typedef struct {
+  int npdls, ndims;
+  int64_t dims[20];
+  int64_t incs[20 * 6];
+  int64_t offs[6];
+  int64_t ind_sizes[3];
+  int64_t inc_sizes[6][2];
+  int datatype, bvalflag;
+} pdlc_plan;
+void pdlc_capture_plan(pdlc_plan *dst);
 void pdlc_free(void *p);
 
 /* pdl_find_max_pthread (pdlbroadcast.c:95-159) on plain dim lists, for the

@@ -43,6 +43,14 @@ class CPdl(ctypes.Structure):
                 ("data", ctypes.c_void_p), ("badvalue", Anyval)]
 
 
+class CPlan(ctypes.Structure):
+    """pdlc_plan: the broadcast plan of a run (pdlbroadcast.h:18-61 fields,
+    core ind_sizes / inc_sizes), captured for readdata-level tests"""
+    _fields_ = [("npdls", ctypes.c_int), ("ndims", ctypes.c_int), ("dims", ctypes.c_int64 * 20),
+                ("incs", ctypes.c_int64 * 120), ("offs", ctypes.c_int64 * 6), ("ind_sizes", ctypes.c_int64 * 3),
+                ("inc_sizes", ctypes.c_int64 * 12), ("datatype", ctypes.c_int), ("bvalflag", ctypes.c_int)]
+
+
 class CError(ctypes.Structure):
     _fields_ = [("error", ctypes.c_int32), ("message", ctypes.c_char * 4096)]
 
@@ -71,6 +79,7 @@ def lib():
         l.pdlc_splitmix64_next.restype = ctypes.c_uint64
         l.pdlc_truth_matmult_ld.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int64] * 3
         l.pdlc_convert.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]
+        l.pdlc_capture_plan.argtypes = [ctypes.POINTER(CPlan)]
         _lib = l
     return _lib
 
@@ -257,8 +266,10 @@ class OPdl:
             lib().pdlc_free(c.data)
 
 
-def run(op, *pdls, swap=0, nthreads=0, size_mi=1, m_size=None):
-    """pdl_run_<op>(pdls..., swap). Raises OracleError with PDL's message."""
+def run(op, *pdls, swap=0, nthreads=0, size_mi=1, m_size=None, dpar=None):
+    """pdl_run_<op>(pdls..., swap).  m_size: the *_n_ind OtherPar; dpar: a
+    double OtherPar (setvaltobad's value, setbadtoval's newval), passed as
+    its IEEE bit pattern.  Raises OracleError with PDL's message."""
     cs = [p.to_c() for p in pdls]
     # the same OPdl passed twice (inplace) must be one C struct
     uniq = {}
@@ -268,7 +279,10 @@ def run(op, *pdls, swap=0, nthreads=0, size_mi=1, m_size=None):
             uniq[id(p)] = cs[i]
         ptrs[i] = ctypes.pointer(uniq[id(p)])
     created = [p.isnull() for p in pdls]
-    other = (ctypes.c_int64 * 1)(swap if m_size is None else m_size)
+    if dpar is not None:
+        other = (ctypes.c_int64 * 1)(int(np.array([dpar], np.float64).view(np.int64)[0]))
+    else:
+        other = (ctypes.c_int64 * 1)(swap if m_size is None else m_size)
     err = CError()
     rc = lib().pdlc_run(op.encode(), ptrs, len(pdls), other, nthreads, size_mi, ctypes.byref(err))
     if rc:
@@ -283,6 +297,18 @@ def run(op, *pdls, swap=0, nthreads=0, size_mi=1, m_size=None):
         done.add(id(p))
         p.from_c(uniq[id(p)], created[i])
     return pdls
+
+
+def run_capturing_plan(op, *pdls, swap=0, m_size=None, dpar=None):
+    """run() on one thread, returning the broadcast plan PDL's redodims built
+    for it (a CPlan)"""
+    plan = CPlan()
+    lib().pdlc_capture_plan(ctypes.byref(plan))
+    try:
+        run(op, *pdls, swap=swap, m_size=m_size, dpar=dpar)
+    finally:
+        lib().pdlc_capture_plan(None)
+    return plan
 
 
 def find_max_pthread(dims, target, realdims=0):

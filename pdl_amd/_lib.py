@@ -102,6 +102,13 @@ EXPORTS["pdlhip_run_minmaximum"] = ([PP] * 5, Error)
 EXPORTS["pdlhip_run_minimum_n_ind"] = ([PP, PP, ctypes.c_int64], Error)
 EXPORTS["pdlhip_run_maximum_n_ind"] = ([PP, PP, ctypes.c_int64], Error)
 EXPORTS["pdlhip_run_matmult"] = ([PP, PP, PP], Error)
+UFUNCS = ["bitnot", "sqrt", "sin", "cos", "not", "exp", "log", "log10", "abs"]
+BADOPS = ["isbad", "isgood", "setnantobad", "setinftobad", "setnonfinitetobad", "setbadtonan"]
+for _n in UFUNCS + BADOPS:
+    EXPORTS["pdlhip_run_" + _n] = ([PP, PP], Error)
+EXPORTS["pdlhip_run_setbadif"] = ([PP, PP, PP], Error)
+EXPORTS["pdlhip_run_setvaltobad"] = ([PP, PP, ctypes.c_double], Error)
+EXPORTS["pdlhip_run_setbadtoval"] = ([PP, PP, ctypes.c_double], Error)
 EXPORTS["pdlhip_ragged_prepare"] = ([ctypes.c_int, ctypes.c_int, ctypes.POINTER(PP), ctypes.c_int,
                                      ctypes.POINTER(Node), ctypes.c_char_p, ctypes.POINTER(PP), ctypes.POINTER(PP),
                                      ctypes.POINTER(ctypes.c_void_p)], Error)

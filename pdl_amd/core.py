@@ -443,6 +443,42 @@ class ndarray:
         _run("minmaximum", [self] + outs)
         return tuple(outs)
 
+    # ------------------------------------------------------------ Ops.pd ufuncs
+    def bitnot(self, out=None): return self._red("bitnot", out)
+    def __invert__(self): return self._red("bitnot")
+    def sqrt(self, out=None): return self._red("sqrt", out)
+    def sin(self, out=None): return self._red("sin", out)
+    def cos(self, out=None): return self._red("cos", out)
+    def not_(self, out=None): return self._red("not", out)
+    def exp(self, out=None): return self._red("exp", out)
+    def log(self, out=None): return self._red("log", out)
+    def log10(self, out=None): return self._red("log10", out)
+    def abs(self, out=None): return self._red("abs", out)
+    def __abs__(self): return self._red("abs")
+
+    # ------------------------------------------------------------ Bad.pd
+    def isbad(self, out=None): return self._red("isbad", out)
+    def isgood(self, out=None): return self._red("isgood", out)
+    def setnantobad(self, out=None): return self._red("setnantobad", out)
+    def setinftobad(self, out=None): return self._red("setinftobad", out)
+    def setnonfinitetobad(self, out=None): return self._red("setnonfinitetobad", out)
+    def setbadtonan(self, out=None): return self._red("setbadtonan", out)
+
+    def setbadif(self, mask, out=None):
+        out = ndarray() if out is None else out
+        _run("setbadif", [self, topdl(mask) if not isinstance(mask, ndarray) else mask, out])
+        return out
+
+    def setvaltobad(self, value, out=None):
+        out = ndarray() if out is None else out
+        _run("setvaltobad", [self, out], dpar=float(value))
+        return out
+
+    def setbadtoval(self, newval, out=None):
+        out = ndarray() if out is None else out
+        _run("setbadtoval", [self, out], dpar=float(newval))
+        return out
+
     # whole-array reducers: $x->flat->op (Ufunc.pd:618-663)
     def sum(self): return self.flat().sumover()
     def prod(self): return self.flat().prodover()
@@ -545,8 +581,9 @@ def random_normal(*dims, dtype=L.F, seed=0, mean=0.0, sd=1.0):
     return p
 
 
-def _run(op, pdls, swap=0, m_size=None):
-    """One pdl_run_<op> call through the C ABI."""
+def _run(op, pdls, swap=0, m_size=None, dpar=None):
+    """One pdl_run_<op> call through the C ABI (dpar: a double OtherPar,
+    passed as its IEEE bit pattern)."""
     _need()
     if op == "assgn_convert":
         op = "assgn"
@@ -557,7 +594,10 @@ def _run(op, pdls, swap=0, m_size=None):
         cs.append(uniq[id(p)])
     created = {id(p): p.isnull() for p in pdls}
     arr = (L.PP * len(pdls))(*[ctypes.pointer(c) for c in cs])
-    other = (ctypes.c_int64 * 1)(swap if m_size is None else m_size)
+    if dpar is not None:
+        other = (ctypes.c_int64 * 1)(int(np.array([dpar], np.float64).view(np.int64)[0]))
+    else:
+        other = (ctypes.c_int64 * 1)(swap if m_size is None else m_size)
     L.check(L.lib().pdlhip_run(op.encode(), arr, len(pdls), other))
     done = set()
     for p in pdls:
@@ -568,9 +608,10 @@ def _run(op, pdls, swap=0, m_size=None):
     return pdls
 
 
-def run(op, *pdls, swap=0, m_size=None):
+def run(op, *pdls, swap=0, m_size=None, dpar=None):
     """PDL::<op>(pdls...) with explicit outputs (null() to create)"""
-    return _run(op, [topdl(p) if not isinstance(p, ndarray) else p for p in pdls], swap=swap, m_size=m_size)
+    return _run(op, [topdl(p) if not isinstance(p, ndarray) else p for p in pdls], swap=swap, m_size=m_size,
+                dpar=dpar)
 
 
 def matmult(x, y, c=None):
@@ -748,18 +789,25 @@ class Chain(_Prepared):
 
 
 class Graph:
-    """A HIP graph of the work `fn` enqueues on the library's stream (only
-    launches that allocate nothing, such as Ragged.run)."""
+    """A HIP graph of the work `fn` enqueues on the library's stream.  Ops
+    may take scratch memory only from the library's cache while capturing
+    (run an op once before capturing it); blocks they release stay with the
+    graph until destroy().  Ops that wait for the stream (downloads, bad-value
+    *_n_ind, ragged bad flags) are refused."""
 
     def __init__(self, fn):
         _need()
+        self._g = None
         L.check(L.lib().pdlhip_graph_begin())
+        g = ctypes.c_void_p()
         try:
             fn()
-        finally:
-            g = ctypes.c_void_p()
-            e = L.lib().pdlhip_graph_end(ctypes.byref(g))
-        L.check(e)
+        except BaseException:
+            L.lib().pdlhip_graph_end(ctypes.byref(g))
+            if g.value:
+                L.lib().pdlhip_graph_destroy(g)
+            raise
+        L.check(L.lib().pdlhip_graph_end(ctypes.byref(g)))
         self._g = g
 
     def launch(self):

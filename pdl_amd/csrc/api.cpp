@@ -49,7 +49,22 @@ static pdlhip_error make_err(int kind, const char *fmt, ...) {
   return pdlhip_error{kind, g_msg.c_str()};
 }
 
+static bool capturing();
+
+// device memory that could not be had: during graph capture only cached
+// blocks are handed out (no hipMalloc, no synchronisation), so an op whose
+// scratch is not cached yet must run once before the capture
+static pdlhip_error oom_err(const char *what) {
+  if (capturing())
+    return make_err(PDLHIP_EUSERERROR,
+                    "pdlhip: %s: needs device memory that is not cached while a graph is being captured "
+                    "(run it once before pdlhip_graph_begin)",
+                    what);
+  return make_err(PDLHIP_EFATAL, "pdlhip: %s: out of device memory", what);
+}
+
 static pdlhip_error hip_err(hipError_t e, const char *what) {
+  if (e == hipErrorOutOfMemory) return oom_err(what);
   return make_err(PDLHIP_EFATAL, "pdlhip: %s: %s", what, hipGetErrorString(e));
 }
 
@@ -77,8 +92,22 @@ struct Global {
   int mm_mode = PDLHIP_MM_SPLIT16;
   bool plan_only = false;
   std::mutex mu;
+  // graph capture (pdlhip_graph_begin .. _end): blocks released while
+  // capturing stay out of the cache until the graph that may read or write
+  // them is destroyed
+  bool capturing = false;
+  std::vector<void *> pinned;
 };
 static Global g;
+
+// true while work enqueued on the library stream is being captured (by
+// pdlhip_graph_begin or by a caller's own capture of pdlhip_stream())
+static bool capturing() {
+  if (g.capturing) return true;
+  if (!g.stream) return false;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(g.stream, &st) == hipSuccess && st == hipStreamCaptureStatusActive;
+}
 
 static pdlhip_error ensure_init() {
   if (g.plan_only) return ok();
@@ -91,6 +120,7 @@ static pdlhip_error ensure_init() {
 // return so the host buffer may go away
 static hipError_t upload_now(void *dst, const void *src, size_t bytes) {
   if (!bytes) return hipSuccess;
+  if (capturing()) return hipErrorStreamCaptureUnsupported;   // preparing steps run before a capture
   hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g.stream);
   return e == hipSuccess ? hipStreamSynchronize(g.stream) : e;
 }
@@ -132,6 +162,7 @@ void *dev_alloc(size_t n) {
     g_cache.freelist.erase(it);
     return p;
   }
+  if (capturing()) return nullptr;   // no hipMalloc or synchronisation inside a capture
   void *p = nullptr;
   if (hipMalloc(&p, sz) != hipSuccess) {
     (void)hipGetLastError();
@@ -149,6 +180,12 @@ bool dev_free(void *p) {
   std::lock_guard<std::mutex> lk(g_cache.mu);
   auto it = g_cache.owned.find(p);
   if (it == g_cache.owned.end()) return false;
+  if (capturing()) {
+    // a captured kernel may use this block at every replay: it returns to
+    // the cache when the graph is destroyed
+    g.pinned.push_back(p);
+    return true;
+  }
   g_cache.freelist.insert({it->second, p});
   g_cache.cached += it->second;
   if (g_cache.cached > ((size_t)64 << 30)) {  // keep the cache bounded
@@ -162,7 +199,7 @@ void *scratch_alloc(size_t bytes) { return dev_alloc(bytes); }
 void scratch_free(void *p) { dev_free(p); }
 
 // ------------------------------------------------------------------ signatures
-enum Family { FAM_BIOP, FAM_BIFUNC, FAM_ASSGN, FAM_RED, FAM_NIND, FAM_MM };
+enum Family { FAM_BIOP, FAM_BIFUNC, FAM_ASSGN, FAM_RED, FAM_NIND, FAM_MM, FAM_UFUNC, FAM_BADOP };
 enum ParFlag { P_OUT = 1, P_CREAT = 2, P_TYPED = 4, P_TPLUS = 8 };
 
 struct Sig {
@@ -184,6 +221,7 @@ static const std::vector<int> GT_REAL = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
 static const std::vector<int> GT_INT = {0, 1, 2, 3, 4, 5, 6, 7, 8};
 static const std::vector<int> GT_POWER = {12, 13, 14, 9, 11, 10};  // [@$C, @$F], $F = F,LD,D
 static const std::vector<int> GT_ATAN2 = {9, 11, 10};
+static const std::vector<int> GT_AF = {9, 11, 12, 13, 14, 10};     // $AF: non-integer, D last
 
 static std::vector<Sig> build_sigs() {
   std::vector<Sig> v;
@@ -244,6 +282,35 @@ static std::vector<Sig> build_sigs() {
   nind("maximum_n_ind", R_MAX_N_IND);
   v.push_back(Sig{"matmult", 3, 2, {"a", "b", "c"}, {2, 2, 2}, {{0, 1}, {2, 0}, {2, 1}}, {"t", "h", "w"},
                   {0, 0, P_OUT | P_CREAT}, {-1, -1, -1}, GT_ALL, FAM_MM, 0});
+  // Ops.pd ufuncs: Pars a(); [o]b() (Ops.pd:222-265); GenericTypes of
+  // Ops.pd:327-333, 365 ($T, $A, reals, [@$C, @$F]) and _rabs (343-358)
+  auto uf = [&](const char *n, int op, const std::vector<int> &gt) {
+    v.push_back(Sig{n, 2, 1, {"a", "b"}, {0, 0}, {{}, {}}, {}, {0, P_OUT | P_CREAT}, {-1, -1}, gt, FAM_UFUNC, op});
+  };
+  uf("bitnot", OP_BITNOT, GT_INT);
+  uf("sqrt", OP_SQRT, GT_ALL);
+  uf("sin", OP_SIN, GT_ALL);
+  uf("cos", OP_COS, GT_ALL);
+  uf("not", OP_NOT, GT_REAL);
+  uf("exp", OP_EXP, GT_POWER);
+  uf("log", OP_LOG, GT_POWER);
+  uf("log10", OP_LOG10, GT_ALL);
+  uf("abs", OP_ABS, GT_REAL);
+  // Bad.pd elementwise ops (Bad.pd:343-395, 584-840)
+  auto bo = [&](const char *n, int op, const std::vector<int> &gt, int oflags, int otype) {
+    v.push_back(Sig{n, 2, 1, {"a", "b"}, {0, 0}, {{}, {}}, {}, {0, P_OUT | P_CREAT | oflags}, {-1, otype}, gt,
+                    FAM_BADOP, op});
+  };
+  bo("isbad", B_ISBAD, GT_ALL, P_TYPED, PDLHIP_L);
+  bo("isgood", B_ISGOOD, GT_ALL, P_TYPED, PDLHIP_L);
+  bo("setvaltobad", B_SETVALTOBAD, GT_ALL, 0, -1);
+  bo("setnantobad", B_SETNANTOBAD, GT_AF, 0, -1);
+  bo("setinftobad", B_SETINFTOBAD, GT_AF, 0, -1);
+  bo("setnonfinitetobad", B_SETNONFINITETOBAD, GT_AF, 0, -1);
+  bo("setbadtonan", B_SETBADTONAN, GT_AF, 0, -1);
+  bo("setbadtoval", B_SETBADTOVAL, GT_ALL, 0, -1);
+  v.push_back(Sig{"setbadif", 3, 2, {"a", "mask", "b"}, {0, 0, 0}, {{}, {}, {}}, {}, {0, P_TYPED, P_OUT | P_CREAT},
+                  {-1, PDLHIP_L, -1}, GT_ALL, FAM_BADOP, B_SETBADIF});
   return v;
 }
 
@@ -584,6 +651,13 @@ static bool fill_ew(EwPlan &e, Dimset &d) {
   return true;
 }
 
+// a fill kernel rather than hipMemsetAsync, so the clear is an ordinary
+// kernel node wherever the work is captured
+__global__ void zero_i32(int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 // the readdata of one op over a planned broadcast (the pdlhip_readdata body)
 static pdlhip_error readdata(const Sig &s, int T, bool bad, const Brc &b, pdlhip_pdl **pdls, const int64_t *other) {
   if (g.plan_only) return ok();
@@ -602,6 +676,65 @@ static pdlhip_error readdata(const Sig &s, int T, bool bad, const Brc &b, pdlhip
     for (int k = 0; k < 3; k++)
       bi[k] = badinfo(pdls[k], s.family == FAM_BIFUNC ? 1 : (pdls[k]->state & PDLHIP_BADVAL) != 0);
     e = binop_dispatch(T, s.op, p, pdls[0]->data, pdls[1]->data, pdls[2]->data, bad, bi, st);
+  } else if (s.family == FAM_UFUNC) {
+    // the binop kernels with the second operand a zero-stride view of a's
+    // first element, which the ufunc ignores (and the bad test skips)
+    int which[3] = {0, 0, 1};
+    Dimset d = brc_dimset(b, which, 3);
+    for (int k = 0; k < d.n; k++) d.inc[1][k] = 0;
+    if (!canonicalise(d, 2)) return ok();
+    EwPlan p;
+    if (!fill_ew(p, d)) return make_err(PDLHIP_EFATAL, "pdlhip: %s: too many distinct dims", s.name.c_str());
+    p.off[0] = b.offs[0];
+    p.off[1] = b.offs[0];
+    p.off[2] = b.offs[1];
+    BadInfo bi[3] = {badinfo(pdls[0], 1), badinfo(pdls[0], 0), badinfo(pdls[1], 1)};
+    e = binop_dispatch(T, s.op, p, pdls[0]->data, pdls[0]->data, pdls[1]->data, bad, bi, st);
+  } else if (s.family == FAM_BADOP) {
+    const bool mask = s.op == B_SETBADIF;
+    const int out = mask ? 2 : 1;
+    int which[3] = {0, mask ? 1 : 0, out};
+    Dimset d = brc_dimset(b, which, 3);
+    if (!mask)
+      for (int k = 0; k < d.n; k++) d.inc[1][k] = 0;
+    if (!canonicalise(d, 2)) return ok();
+    EwPlan p;
+    if (!fill_ew(p, d)) return make_err(PDLHIP_EFATAL, "pdlhip: %s: too many distinct dims", s.name.c_str());
+    p.off[0] = b.offs[0];
+    p.off[1] = mask ? b.offs[1] : b.offs[0];
+    p.off[2] = b.offs[out];
+    // a double OtherPar arrives as its IEEE bit pattern
+    double dval = 0.0;
+    if (other && (s.op == B_SETVALTOBAD || s.op == B_SETBADTOVAL)) std::memcpy(&dval, &other[0], sizeof dval);
+    const bool found = s.op == B_SETNANTOBAD || s.op == B_SETINFTOBAD || s.op == B_SETNONFINITETOBAD;
+    int *flag = nullptr;
+    if (found) {
+      // the output's bad flag depends on the data (read back below)
+      if (capturing())
+        return make_err(PDLHIP_EUSERERROR,
+                        "pdlhip: %s sets its output's bad flag from the data and cannot be captured in a graph",
+                        s.name.c_str());
+      flag = (int *)dev_alloc(sizeof(int));
+      if (!flag) return oom_err(s.name.c_str());
+      hipLaunchKernelGGL(zero_i32, dim3(1), dim3(64), 0, st, flag, 1);
+    }
+    int32_t mbv = 0;
+    if (mask) {
+      const pdlhip_value mv = badinfo(pdls[1], 1).bv;
+      std::memcpy(&mbv, &mv, sizeof mbv);
+    }
+    e = launch_badop(T, s.op, p, pdls[0]->data, mask ? pdls[1]->data : nullptr, pdls[out]->data,
+                     badinfo(pdls[0], 1), mbv, badinfo(pdls[out], 1).bv, dval, bad, flag, st);
+    // $PDLSTATESETBAD / $PDLSTATESETGOOD of the reference's code
+    if (s.op == B_SETBADIF || s.op == B_SETVALTOBAD) pdls[out]->state |= PDLHIP_BADVAL;
+    if (s.op == B_SETBADTONAN || s.op == B_SETBADTOVAL) pdls[out]->state &= ~PDLHIP_BADVAL;
+    if (flag) {
+      int h = 0;
+      if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      dev_free(flag);
+      if (h) pdls[out]->state |= PDLHIP_BADVAL;
+    }
   } else if (s.family == FAM_ASSGN) {
     int which[3] = {0, 1, 1};
     Dimset d = brc_dimset(b, which, 3);
@@ -645,9 +778,15 @@ static pdlhip_error readdata(const Sig &s, int T, bool bad, const Brc &b, pdlhip
     BadInfo ba = badinfo(pdls[0], 1);
     int *flag = nullptr;
     if (s.family == FAM_NIND && bad && p.nout > 0) {
+      // the output's bad flag depends on the data (read back below)
+      if (capturing())
+        return make_err(PDLHIP_EUSERERROR,
+                        "pdlhip: %s with bad values sets its output's bad flag from the data and cannot be "
+                        "captured in a graph",
+                        s.name.c_str());
       flag = (int *)dev_alloc(sizeof(int));
-      if (!flag) return make_err(PDLHIP_EFATAL, "pdlhip: out of device memory");
-      (void)hipMemsetAsync(flag, 0, sizeof(int), st);
+      if (!flag) return oom_err(s.name.c_str());
+      hipLaunchKernelGGL(zero_i32, dim3(1), dim3(64), 0, st, flag, 1);
     }
     e = reduce_dispatch(T, s.op, pdls[1]->datatype, p, pdls[0]->data, outs, bad, ba, obv, flag, st);
     // state flags the reference sets from inside readdata
@@ -727,7 +866,7 @@ static pdlhip_error make_converted(const pdlhip_pdl *p, int t, pdlhip_pdl *out, 
   out->data = nullptr;
   if (g.plan_only) return ok();
   out->data = dev_alloc((size_t)std::max<int64_t>(n, 1) * type_size(t));
-  if (!out->data) return make_err(PDLHIP_EFATAL, "pdlhip: out of device memory");
+  if (!out->data) return oom_err("converted copy");
   if (copy_in && n > 0) {
     Dimset d;
     d.np = 3;
@@ -856,7 +995,7 @@ static pdlhip_error run(const Sig &s, pdlhip_pdl **user, int npdls, const int64_
       pdls[i]->data = nullptr;
       if (!g.plan_only) {
         pdls[i]->data = dev_alloc((size_t)std::max<int64_t>(n, 1) * type_size(pdls[i]->datatype));
-        if (!pdls[i]->data) err = make_err(PDLHIP_EFATAL, "pdlhip: out of device memory");
+        if (!pdls[i]->data) err = oom_err(s.name.c_str());
       }
       pdls[i]->state &= ~PDLHIP_NOMYDIMS;
     }
@@ -909,7 +1048,8 @@ struct ListH {
   MmItem *d_items = nullptr;
   unsigned int *d_counter = nullptr;
   int3 *d_w32 = nullptr, *d_w64 = nullptr;
-  int tile = 64;
+  int tile = 64;        // 32 / 64: SIMT tiles; 0: the matrix-core kernel (64x64 work items)
+  bool mfma = false;    // float/double list in a default mode whose items fit 32-bit offsets
 };
 
 static void list_release(ListH &h) {
@@ -921,7 +1061,7 @@ static void list_release(ListH &h) {
 }
 
 static hipError_t list_launch(const ListH &h, int tile, bool each) {
-  const std::vector<int3> &w = tile == 32 ? h.work32 : h.work64;
+  const std::vector<int3> &w = tile == 32 ? h.work32 : h.work64;   // the MFMA kernel takes 64x64 items
   const int nblocks = (int)std::max<size_t>(std::min<size_t>(w.size(), 2048), 1);
   return launch_matmult_list(h.d.T, h.d_items, h.d.items.data(), (int)h.d.items.size(), tile == 32 ? h.d_w32 : h.d_w64,
                              (unsigned)w.size(), h.d_counter, tile, h.d.fused, each, nblocks, g.stream);
@@ -967,25 +1107,36 @@ static pdlhip_error list_build(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlhip_pdl
   };
   build_work(32, h.work32);
   build_work(64, h.work64);
+  // the matrix-core kernel indexes within a product with 32-bit offsets
+  auto span = [](int64_t n1, int64_t i1, int64_t n2, int64_t i2) {
+    return (n1 > 0 ? n1 - 1 : 0) * (i1 < 0 ? -i1 : i1) + (n2 > 0 ? n2 - 1 : 0) * (i2 < 0 ? -i2 : i2);
+  };
+  h.mfma = h.d.fused && (h.d.T == PDLHIP_F || h.d.T == PDLHIP_D);
+  for (const MmItem &it : h.d.items)
+    if (span(it.T, it.a_t, it.H, it.a_h) >= ((int64_t)1 << 31) || span(it.T, it.b_t, it.W, it.b_w) >= ((int64_t)1 << 31) ||
+        it.H >= ((int64_t)1 << 30) || it.W >= ((int64_t)1 << 30) || it.T >= ((int64_t)1 << 30))
+      h.mfma = false;
   h.d_items = (MmItem *)dev_alloc(sizeof(MmItem) * nit);
   h.d_counter = (unsigned int *)dev_alloc(sizeof(unsigned int));
   h.d_w32 = (int3 *)dev_alloc(sizeof(int3) * std::max<size_t>(h.work32.size(), 1));
   h.d_w64 = (int3 *)dev_alloc(sizeof(int3) * std::max<size_t>(h.work64.size(), 1));
   if (!h.d_items || !h.d_counter || !h.d_w32 || !h.d_w64) {
     list_release(h);
-    return make_err(PDLHIP_EFATAL, "pdlhip: out of device memory");
+    return oom_err("matmult list");
   }
   hipError_t e = upload_now(h.d_items, h.d.items.data(), sizeof(MmItem) * nit);
   if (e == hipSuccess) e = upload_now(h.d_w32, h.work32.data(), sizeof(int3) * h.work32.size());
   if (e == hipSuccess) e = upload_now(h.d_w64, h.work64.data(), sizeof(int3) * h.work64.size());
-  // tile by the autotuner: both candidates timed once per shape class
-  TuneKey key{h.d.T, h.d.fused ? 1 : 0, ilog2(maxdim), ilog2(nit)};
+  // tile by the autotuner: the candidates timed once per shape class
+  TuneKey key{h.d.T, (h.d.fused ? 1 : 0) + (h.mfma ? 2 : 0), ilog2(maxdim), ilog2(nit)};
   auto it = g_tune.find(key);
   if (it != g_tune.end()) {
     h.tile = it->second;
   } else if (e == hipSuccess) {
     float best = 1e30f;
-    for (int cand : {32, 64}) {
+    std::vector<int> cands = {32, 64};
+    if (h.mfma) cands.push_back(0);
+    for (int cand : cands) {
       (void)hipEventRecord(g.ev0, g.stream);
       e = list_launch(h, cand, false);
       (void)hipEventRecord(g.ev1, g.stream);
@@ -1137,7 +1288,7 @@ static pdlhip_error output_for(const char *who, const char *what, int64_t idx, p
     }
     o->offs = 0;
     o->data = g.plan_only ? nullptr : dev_alloc((size_t)std::max<int64_t>(n, 1) * type_size(type));
-    if (!g.plan_only && !o->data) return make_err(PDLHIP_EFATAL, "%s: out of device memory", who);
+    if (!g.plan_only && !o->data) return oom_err(who);
     o->state &= ~PDLHIP_NOMYDIMS;
     return ok();
   }
@@ -1219,7 +1370,7 @@ static pdlhip_error ragged_finish(const char *who, RaggedH &h, std::vector<RagIt
   h.d_empty = dev_alloc(std::max<size_t>(sizeof(int32_t) * n, 4));
   if (!okk || !h.d_scratch || !h.d_empty) {
     for (void *p : {h.d_items, h.d_segs, h.d_multi, h.d_order, h.d_bstart, h.d_scratch, h.d_empty}) dev_free(p);
-    return make_err(PDLHIP_EFATAL, "%s: out of device memory", who);
+    return oom_err(who);
   }
   h.L.empty = (int32_t *)h.d_empty;
   h.L.items = (const RagItem *)h.d_items;
@@ -1452,7 +1603,7 @@ pdlhip_error pdlhip_alloc(size_t nbytes, void **data) {
   RETURN_IF(ensure_init());
   if (g.plan_only) { *data = nullptr; return ok(); }
   *data = dev_alloc(nbytes);
-  if (!*data) return make_err(PDLHIP_EFATAL, "pdlhip: out of device memory (%zu bytes)", nbytes);
+  if (!*data) return oom_err("pdlhip_alloc");
   return ok();
 }
 
@@ -1464,6 +1615,7 @@ pdlhip_error pdlhip_free(void *data) {
 
 pdlhip_error pdlhip_upload(void *dst, const void *src, size_t nbytes) {
   RETURN_IF(ensure_init());
+  if (capturing()) return make_err(PDLHIP_EUSERERROR, "pdlhip_upload: waits for the stream; not allowed while a graph is being captured");
   if (!nbytes) return ok();
   hipError_t e = hipMemcpyAsync(dst, src, nbytes, hipMemcpyHostToDevice, g.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
@@ -1472,6 +1624,7 @@ pdlhip_error pdlhip_upload(void *dst, const void *src, size_t nbytes) {
 
 pdlhip_error pdlhip_download(void *dst, const void *src, size_t nbytes) {
   RETURN_IF(ensure_init());
+  if (capturing()) return make_err(PDLHIP_EUSERERROR, "pdlhip_download: waits for the stream; not allowed while a graph is being captured");
   if (!nbytes) return ok();
   hipError_t e = hipMemcpyAsync(dst, src, nbytes, hipMemcpyDeviceToHost, g.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
@@ -1480,6 +1633,7 @@ pdlhip_error pdlhip_download(void *dst, const void *src, size_t nbytes) {
 
 pdlhip_error pdlhip_synchronize(void) {
   RETURN_IF(ensure_init());
+  if (capturing()) return make_err(PDLHIP_EUSERERROR, "pdlhip_synchronize: waits for the stream; not allowed while a graph is being captured");
   hipError_t e = hipStreamSynchronize(g.stream);
   return e == hipSuccess ? ok() : hip_err(e, "synchronize");
 }
@@ -1560,6 +1714,41 @@ pdlhip_error pdlhip_run_maximum_n_ind(pdlhip_pdl *a, pdlhip_pdl *c, int64_t m_si
 pdlhip_error pdlhip_run_matmult(pdlhip_pdl *a, pdlhip_pdl *b, pdlhip_pdl *c) {
   pdlhip_pdl *v[3] = {a, b, c};
   return pdlhip_run("matmult", v, 3, nullptr);
+}
+
+UNARY_ENTRY(bitnot)
+UNARY_ENTRY(sqrt)
+UNARY_ENTRY(sin)
+UNARY_ENTRY(cos)
+UNARY_ENTRY(not)
+UNARY_ENTRY(exp)
+UNARY_ENTRY(log)
+UNARY_ENTRY(log10)
+UNARY_ENTRY(abs)
+UNARY_ENTRY(isbad)
+UNARY_ENTRY(isgood)
+UNARY_ENTRY(setnantobad)
+UNARY_ENTRY(setinftobad)
+UNARY_ENTRY(setnonfinitetobad)
+UNARY_ENTRY(setbadtonan)
+
+pdlhip_error pdlhip_run_setbadif(pdlhip_pdl *a, pdlhip_pdl *mask, pdlhip_pdl *b) {
+  pdlhip_pdl *v[3] = {a, mask, b};
+  return pdlhip_run("setbadif", v, 3, nullptr);
+}
+
+pdlhip_error pdlhip_run_setvaltobad(pdlhip_pdl *a, pdlhip_pdl *b, double value) {
+  pdlhip_pdl *v[2] = {a, b};
+  int64_t o[1];
+  std::memcpy(&o[0], &value, sizeof value);
+  return pdlhip_run("setvaltobad", v, 2, o);
+}
+
+pdlhip_error pdlhip_run_setbadtoval(pdlhip_pdl *a, pdlhip_pdl *b, double newval) {
+  pdlhip_pdl *v[2] = {a, b};
+  int64_t o[1];
+  std::memcpy(&o[0], &newval, sizeof newval);
+  return pdlhip_run("setbadtoval", v, 2, o);
 }
 
 pdlhip_error pdlhip_run_matmult_list(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlhip_pdl **c, int mode) {
@@ -1649,6 +1838,9 @@ pdlhip_error pdlhip_ragged_badflags(pdlhip_ragged *handle, int32_t *flags) {
     return ok();
   }
   if (g.plan_only || h.n == 0) return ok();
+  if (capturing())
+    return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_badflags: waits for the stream; not allowed while a graph is "
+                                       "being captured");
   hipError_t e = hipStreamSynchronize(g.stream);
   if (e == hipSuccess) e = hipMemcpyAsync(flags, h.d_empty, sizeof(int32_t) * h.n, hipMemcpyDeviceToHost, g.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
@@ -1669,33 +1861,72 @@ pdlhip_error pdlhip_ragged_free(pdlhip_ragged *handle) {
   return ok();
 }
 
+// a captured graph and the blocks its kernels use (released to the cache
+// while it was captured): they go back to the cache when it is destroyed
+struct GraphH {
+  hipGraphExec_t exec = nullptr;
+  std::vector<void *> pinned;
+};
+
+static void unpin(std::vector<void *> &blocks) {
+  for (void *p : blocks) dev_free(p);
+  blocks.clear();
+}
+
 pdlhip_error pdlhip_graph_begin(void) {
   RETURN_IF(ensure_init());
+  if (capturing()) return make_err(PDLHIP_EUSERERROR, "pdlhip_graph_begin: a capture is already in progress");
   hipError_t e = hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal);
-  return e == hipSuccess ? ok() : hip_err(e, "hipStreamBeginCapture");
+  if (e != hipSuccess) return hip_err(e, "hipStreamBeginCapture");
+  g.capturing = true;
+  return ok();
 }
 
 pdlhip_error pdlhip_graph_end(void **graph) {
   RETURN_IF(ensure_init());
+  if (!graph) return make_err(PDLHIP_EUSERERROR, "pdlhip_graph_end: graph is NULL");
+  *graph = nullptr;
   hipGraph_t gr = nullptr;
   hipError_t e = hipStreamEndCapture(g.stream, &gr);
-  if (e != hipSuccess) return hip_err(e, "hipStreamEndCapture");
+  g.capturing = false;
+  std::vector<void *> pinned;
+  {
+    std::lock_guard<std::mutex> lk(g_cache.mu);
+    pinned.swap(g.pinned);
+  }
+  if (e != hipSuccess) {
+    unpin(pinned);
+    return hip_err(e, "hipStreamEndCapture");
+  }
   hipGraphExec_t ex = nullptr;
   e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
   (void)hipGraphDestroy(gr);
-  if (e != hipSuccess) return hip_err(e, "hipGraphInstantiate");
-  *graph = (void *)ex;
+  if (e != hipSuccess) {
+    unpin(pinned);
+    return hip_err(e, "hipGraphInstantiate");
+  }
+  GraphH *h = new GraphH;
+  h->exec = ex;
+  h->pinned.swap(pinned);
+  *graph = (void *)h;
   return ok();
 }
 
 pdlhip_error pdlhip_graph_launch(void *graph) {
   RETURN_IF(ensure_init());
-  hipError_t e = hipGraphLaunch((hipGraphExec_t)graph, g.stream);
+  if (!graph) return make_err(PDLHIP_EUSERERROR, "pdlhip_graph_launch: graph is NULL");
+  hipError_t e = hipGraphLaunch(((GraphH *)graph)->exec, g.stream);
   return e == hipSuccess ? ok() : hip_err(e, "hipGraphLaunch");
 }
 
 pdlhip_error pdlhip_graph_destroy(void *graph) {
-  hipError_t e = hipGraphExecDestroy((hipGraphExec_t)graph);
+  if (!graph) return ok();
+  GraphH *h = (GraphH *)graph;
+  hipError_t e = hipGraphExecDestroy(h->exec);
+  // replays already queued on the library stream come before any later use
+  // of these blocks
+  unpin(h->pinned);
+  delete h;
   return e == hipSuccess ? ok() : hip_err(e, "hipGraphExecDestroy");
 }
 

@@ -16,7 +16,16 @@ constexpr int MAXP = 6;      // pdls per trans on this path
 enum BinOp : int {
   OP_PLUS, OP_MULT, OP_MINUS, OP_DIVIDE, OP_GT, OP_LT, OP_LE, OP_GE, OP_EQ, OP_NE,
   OP_SHL, OP_SHR, OP_OR, OP_AND, OP_XOR, OP_POWER, OP_ATAN2, OP_MODULO, OP_SPACESHIP,
-  OP_NBINOPS
+  OP_NBINOPS,
+  // Ops.pd ufuncs (Ops.pd:222-265, 327-381): one input; the elementwise
+  // kernels' second operand is ignored.  Not valid in fused chains.
+  OP_BITNOT = 32, OP_SQRT, OP_SIN, OP_COS, OP_NOT, OP_EXP, OP_LOG, OP_LOG10, OP_ABS
+};
+
+// Bad.pd elementwise ops (Bad.pd:343-395, 584-840)
+enum BadOp : int {
+  B_ISBAD, B_ISGOOD, B_SETBADIF, B_SETVALTOBAD, B_SETNANTOBAD, B_SETINFTOBAD, B_SETNONFINITETOBAD,
+  B_SETBADTONAN, B_SETBADTOVAL
 };
 
 // Ufunc.pd reductions / scans (lib/PDL/Ufunc.pd:88-140, 413-613)

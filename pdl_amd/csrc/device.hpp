@@ -133,6 +133,51 @@ template <int OP, class T> __device__ inline T binop(T a, T b) {
   }
   else if constexpr (OP == OP_MODULO) return op_mod<T>(a, b);
   else if constexpr (OP == OP_SPACESHIP) return (T)(((a) < (b)) ? -1 : ((a) != (b)));
+  // ufuncs: b = ($GENERIC(b))func(a) through <tgmath.h> (an integer
+  // argument takes the double function; the cast back truncates)
+  else if constexpr (OP == OP_BITNOT) { if constexpr (IsFloat<T>::v) return a; else return (T)(~(P)a); }
+  else if constexpr (OP == OP_NOT) return (T)(!a);
+  else if constexpr (OP == OP_ABS) {
+    if constexpr (!IsFloat<T>::v && (T)-1 > (T)0) return a;     // unsigned: _rabs copies
+    else return (T)PDLHIP_ABS(a);                                // PDL_ABS: -0.0 stays, NaN flips sign
+  }
+  else if constexpr (OP == OP_SQRT) {
+    if constexpr (sizeof(T) == 4 && IsFloat<T>::v) return sqrtf(a);
+    else return (T)sqrt((double)a);
+  }
+  else if constexpr (OP == OP_SIN) {
+    if constexpr (sizeof(T) == 4 && IsFloat<T>::v) return sinf(a);
+    else return (T)sin((double)a);
+  }
+  else if constexpr (OP == OP_COS) {
+    if constexpr (sizeof(T) == 4 && IsFloat<T>::v) return cosf(a);
+    else return (T)cos((double)a);
+  }
+  else if constexpr (OP == OP_EXP) {
+    if constexpr (sizeof(T) == 4 && IsFloat<T>::v) return expf(a);
+    else return (T)exp((double)a);
+  }
+  else if constexpr (OP == OP_LOG) {
+    if constexpr (sizeof(T) == 4 && IsFloat<T>::v) return logf(a);
+    else return (T)log((double)a);
+  }
+  else if constexpr (OP == OP_LOG10) {
+    if constexpr (sizeof(T) == 4 && IsFloat<T>::v) return log10f(a);
+    else if constexpr (IsFloat<T>::v) return log10(a);
+    else {
+      // an exact power of ten gives its exponent exactly (as glibc's log10
+      // does), so the truncation back to the integer type cannot land one
+      // below it
+      const double x = (double)a, r = log10(x);
+      const double k = rint(r);
+      if (k >= 0.0 && k <= 22.0 && fabs(r - k) < 1e-9) {
+        double t = 1.0;
+        for (int i = 0; i < (int)k; i++) t *= 10.0;   // exact up to 1e22
+        if (t == x) return (T)k;
+      }
+      return (T)r;
+    }
+  }
   else return a;
 }
 

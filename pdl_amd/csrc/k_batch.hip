@@ -3,9 +3,11 @@
 // per item).  The host builds one work item per (product, C tile), ordered
 // by decreasing inner dimension (longest-processing-time first); persistent
 // blocks take items from a device counter until the queue is drained, so
-// every block reaches the exit condition.  Each item runs the same SIMT
-// tile as k_matmult.hip (accumulation order per output: ascending t, so
-// the PDL-order variant stays bit-identical to PDL's loop).
+// every block reaches the exit condition.  Float and double lists in the
+// default modes run on the matrix cores (list_mfma_*); PDL order and the
+// integer types run the SIMT tile of k_matmult.hip (accumulation order per
+// output: ascending t, so the PDL-order variant stays bit-identical to PDL's
+// loop).
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
@@ -111,6 +113,186 @@ __global__ __launch_bounds__(256) void list_one(const MmItem *__restrict__ items
   list_tile<T, FUSED, TS>(items[idx], blockIdx.x / ntn, blockIdx.x % ntn, As, Bs);
 }
 
+// ---------------------------------------------------------------- matrix cores
+// Float and double lists in the default modes run on the matrix cores with
+// FP32 / FP64 products and sums (no operand scaling: the same arithmetic as
+// the FP32/FP64 SIMT kernel, summed in the matrix core's order): one work
+// item = a 64x64 tile of C, four waves of 32x32 each; a wave whose quadrant
+// lies outside the product skips its MFMAs.  k chunks (32 floats / 16
+// doubles) go global -> registers -> LDS ([k][row], [k][col]) with the next
+// chunk's loads in flight while the matrix cores work on the current one
+// (two LDS buffers, one barrier per chunk).
+//   float:  v_mfma_f32_32x32x2f32, lane l holds A[l%32][k + l/32] and
+//           B[k + l/32][l%32]; D reg r -> row 8(r/4) + 4(l/32) + r%4, col l%32
+//   double: v_mfma_f64_16x16x4f64 as 2x2 tiles, lane l holds A[l%16][k + l/16]
+//           and B[k + l/16][l%16]; D reg r -> row 4r + l/16, col l%16
+template <class T> struct ListMfma;
+template <> struct ListMfma<float> { static constexpr int KS = 32; };
+template <> struct ListMfma<double> { static constexpr int KS = 16; };
+
+typedef float lf32x16 __attribute__((ext_vector_type(16)));
+typedef double ld4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+__device__ __forceinline__ void list_mfma_tile(const MmItem &it, int64_t tm, int64_t tn,
+                                      T (*As)[ListMfma<T>::KS][65], T (*Bs)[ListMfma<T>::KS][65]) {
+  constexpr int KS = ListMfma<T>::KS;
+  constexpr int NE = KS / 4;            // staged elements per thread per operand (64 x KS / 256)
+  constexpr int SK = 256 / KS;          // rows apart of a thread's elements when it runs along k
+  // a product's own extent fits 32-bit offsets (checked by the host)
+  const T *a = (const T *)it.a + it.oa, *b = (const T *)it.b + it.ob;
+  T *c = (T *)it.c;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+  const int H = (int)it.H, W = (int)it.W, K = (int)it.T;
+  const int h0 = (int)tm * 64, w0 = (int)tn * 64;
+  const int a_t = (int)it.a_t, a_h = (int)it.a_h, b_w = (int)it.b_w, b_t = (int)it.b_t;
+  // threads run along whichever index of each operand is unit-stride: a
+  // thread keeps one "fast" index (k, or the row/column) and steps the other
+  const bool a_kfast = a_t == 1 || a_h != 1;
+  const bool b_wfast = b_w == 1 || b_t != 1;
+  const int fa = a_kfast ? tid % KS : tid % 64, sa = a_kfast ? tid / KS : tid / 64, da = a_kfast ? SK : 4;
+  const int fb = b_wfast ? tid % 64 : tid % KS, sb = b_wfast ? tid / 64 : tid / KS, db = b_wfast ? 4 : SK;
+  const bool active = h0 + 32 * wr < H && w0 + 32 * wc < W;
+  T ra[NE], rb[NE];
+  // loads clamped into the operand (the element is zeroed when staged), so
+  // no load sits behind a branch
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < NE; q++) {
+      const int s1 = sa + da * q, s2 = sb + db * q;
+      const int ah = a_kfast ? h0 + s1 : h0 + fa, ak = a_kfast ? k0 + fa : k0 + s1;
+      const int bw = b_wfast ? w0 + fb : w0 + s2, bk = b_wfast ? k0 + s2 : k0 + fb;
+      ra[q] = a[(ak < K ? ak : K - 1) * a_t + (ah < H ? ah : H - 1) * a_h];
+      rb[q] = b[(bw < W ? bw : W - 1) * b_w + (bk < K ? bk : K - 1) * b_t];
+    }
+  };
+  auto stage = [&](int buf, int k0) {
+#pragma unroll
+    for (int q = 0; q < NE; q++) {
+      const int s1 = sa + da * q, s2 = sb + db * q;
+      const int mm = a_kfast ? s1 : fa, kk = a_kfast ? fa : s1;
+      const int nn = b_wfast ? fb : s2, kb = b_wfast ? s2 : fb;
+      As[buf][kk][mm] = (h0 + mm < H && k0 + kk < K) ? ra[q] : T(0);
+      Bs[buf][kb][nn] = (w0 + nn < W && k0 + kb < K) ? rb[q] : T(0);
+    }
+  };
+  if constexpr (sizeof(T) == 4) {
+    lf32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+    load(0);
+    int buf = 0;
+    for (int k0 = 0; k0 < K; k0 += KS) {
+      stage(buf, k0);
+      __syncthreads();
+      if (k0 + KS < K) load(k0 + KS);
+      if (active) {
+#pragma unroll
+        for (int kk = 0; kk < KS; kk += 2) {
+          const float av = As[buf][kk + (lane >> 5)][32 * wr + (lane & 31)];
+          const float bv = Bs[buf][kk + (lane >> 5)][32 * wc + (lane & 31)];
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+        }
+      }
+      buf ^= 1;
+    }
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int64_t gh = h0 + 32 * wr + 8 * (r / 4) + 4 * (lane >> 5) + (r & 3);
+        const int64_t gw = w0 + 32 * wc + (lane & 31);
+        if (gh < H && gw < W) c[it.oc + gw * it.c_w + gh * it.c_h] = acc[r];
+      }
+    }
+  } else {
+    ld4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) acc[i][j] = ld4{0.0, 0.0, 0.0, 0.0};
+    load(0);
+    int buf = 0;
+    for (int k0 = 0; k0 < K; k0 += KS) {
+      stage(buf, k0);
+      __syncthreads();
+      if (k0 + KS < K) load(k0 + KS);
+      if (active) {
+#pragma unroll
+        for (int kk = 0; kk < KS; kk += 4) {
+          double af[2], bf[2];
+#pragma unroll
+          for (int i = 0; i < 2; i++) af[i] = As[buf][kk + (lane >> 4)][32 * wr + 16 * i + (lane & 15)];
+#pragma unroll
+          for (int j = 0; j < 2; j++) bf[j] = Bs[buf][kk + (lane >> 4)][32 * wc + 16 * j + (lane & 15)];
+#pragma unroll
+          for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+      }
+      buf ^= 1;
+    }
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int64_t gh = h0 + 32 * wr + 16 * i + 4 * r + (lane >> 4);
+            const int64_t gw = w0 + 32 * wc + 16 * j + (lane & 15);
+            if (gh < H && gw < W) c[it.oc + gw * it.c_w + gh * it.c_h] = acc[i][j][r];
+          }
+    }
+  }
+  // the LDS buffers are refilled by the next work item
+  __syncthreads();
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void list_mfma_queue(const MmItem *__restrict__ items,
+                                                       const int3 *__restrict__ work, unsigned int nwork,
+                                                       unsigned int *counter) {
+  constexpr int KS = ListMfma<T>::KS;
+  __shared__ T As[2][KS][65];
+  __shared__ T Bs[2][KS][65];
+  __shared__ unsigned int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+    __syncthreads();
+    const unsigned int w = next;
+    __syncthreads();
+    if (w >= nwork) break;
+    const int3 job = work[w];
+    list_mfma_tile<T>(items[job.x], job.y, job.z, As, Bs);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void list_mfma_one(const MmItem *__restrict__ items, int idx, int64_t ntn) {
+  constexpr int KS = ListMfma<T>::KS;
+  __shared__ T As[2][KS][65];
+  __shared__ T Bs[2][KS][65];
+  list_mfma_tile<T>(items[idx], blockIdx.x / ntn, blockIdx.x % ntn, As, Bs);
+}
+
+template <class T>
+static hipError_t list_mfma_T(const MmItem *d_items, const MmItem *h_items, int nitems, const int3 *d_work,
+                              unsigned int nwork, unsigned int *d_counter, bool each, int nblocks, hipStream_t s) {
+  if (!each) {
+    hipLaunchKernelGGL(list_reset, dim3(1), dim3(64), 0, s, d_counter);
+    hipLaunchKernelGGL(list_mfma_queue<T>, dim3(nblocks), dim3(256), 0, s, d_items, d_work, nwork, d_counter);
+    return hipGetLastError();
+  }
+  for (int i = 0; i < nitems; i++) {
+    const int64_t ntm = (h_items[i].H + 63) / 64, ntn = (h_items[i].W + 63) / 64;
+    if (ntm * ntn == 0 || h_items[i].T == 0) continue;
+    hipLaunchKernelGGL(list_mfma_one<T>, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, d_items, i, ntn);
+  }
+  return hipGetLastError();
+}
+
 template <class T>
 static hipError_t list_T(const MmItem *d_items, const MmItem *h_items, int nitems, const int3 *d_work,
                          unsigned int nwork, unsigned int *d_counter, int tile, bool fused, bool each, int nblocks,
@@ -138,6 +320,13 @@ static hipError_t list_T(const MmItem *d_items, const MmItem *h_items, int nitem
 hipError_t launch_matmult_list(int T, const MmItem *d_items, const MmItem *h_items, int nitems, const int3 *d_work,
                                unsigned int nwork, unsigned int *d_counter, int tile, bool fused, bool each,
                                int nblocks, hipStream_t s) {
+  // tile 0: the matrix-core kernels (64x64 work items), float/double lists
+  // in the default modes
+  if (tile == 0 && fused && T == PDLHIP_F)
+    return list_mfma_T<float>(d_items, h_items, nitems, d_work, nwork, d_counter, each, nblocks, s);
+  if (tile == 0 && fused && T == PDLHIP_D)
+    return list_mfma_T<double>(d_items, h_items, nitems, d_work, nwork, d_counter, each, nblocks, s);
+  if (tile == 0) tile = 64;
   switch (T) {
 #define X(TC) case TC: return list_T<CType<TC>::T>(d_items, h_items, nitems, d_work, nwork, d_counter, tile, \
                                                    fused, each, nblocks, s);

@@ -394,6 +394,12 @@ static hipError_t binop_T(int op, const EwPlan &p, const void *a, const void *b,
   case OP_NE: return binop_T_OP<T, OP_NE>(p, a, b, c, bad, bi, s);
   case OP_MODULO: return binop_T_OP<T, OP_MODULO>(p, a, b, c, bad, bi, s);
   case OP_SPACESHIP: return binop_T_OP<T, OP_SPACESHIP>(p, a, b, c, bad, bi, s);
+  case OP_SQRT: return binop_T_OP<T, OP_SQRT>(p, a, b, c, bad, bi, s);
+  case OP_SIN: return binop_T_OP<T, OP_SIN>(p, a, b, c, bad, bi, s);
+  case OP_COS: return binop_T_OP<T, OP_COS>(p, a, b, c, bad, bi, s);
+  case OP_NOT: return binop_T_OP<T, OP_NOT>(p, a, b, c, bad, bi, s);
+  case OP_LOG10: return binop_T_OP<T, OP_LOG10>(p, a, b, c, bad, bi, s);
+  case OP_ABS: return binop_T_OP<T, OP_ABS>(p, a, b, c, bad, bi, s);
   default: break;
   }
   if constexpr (!F) {
@@ -403,12 +409,15 @@ static hipError_t binop_T(int op, const EwPlan &p, const void *a, const void *b,
     case OP_OR: return binop_T_OP<T, OP_OR>(p, a, b, c, bad, bi, s);
     case OP_AND: return binop_T_OP<T, OP_AND>(p, a, b, c, bad, bi, s);
     case OP_XOR: return binop_T_OP<T, OP_XOR>(p, a, b, c, bad, bi, s);
+    case OP_BITNOT: return binop_T_OP<T, OP_BITNOT>(p, a, b, c, bad, bi, s);
     default: break;
     }
   } else {
     switch (op) {
     case OP_POWER: return binop_T_OP<T, OP_POWER>(p, a, b, c, bad, bi, s);
     case OP_ATAN2: return binop_T_OP<T, OP_ATAN2>(p, a, b, c, bad, bi, s);
+    case OP_EXP: return binop_T_OP<T, OP_EXP>(p, a, b, c, bad, bi, s);
+    case OP_LOG: return binop_T_OP<T, OP_LOG>(p, a, b, c, bad, bi, s);
     default: break;
     }
   }

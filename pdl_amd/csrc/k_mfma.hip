@@ -3,20 +3,35 @@
 //
 // Split-FP16 ("split16"): each float operand element x, pre-scaled by a
 // power of two per row of A / per column of B so that the row's largest
-// magnitude lies in [2^14, 2^15), is written as hi = fp16(x), lo = fp16(x - hi)
-// (22 significant bits together).  The GEMM accumulates hi*hi + hi*lo + lo*hi
-// in FP32 on the matrix cores (v_mfma_f32_16x16x32_f16) and the epilogue
-// undoes the scaling exactly (powers of two).  The dropped lo*lo term is
-// below 2^-22 relative per product.
+// finite magnitude lies in [2^14, 2^15), is written as hi = fp16(x),
+// lo = fp16(x - hi) (22 significant bits together).  The GEMM accumulates
+// hi*hi + hi*lo + lo*hi in FP32 on the matrix cores
+// (v_mfma_f32_32x32x16_f16) and the epilogue undoes the scaling exactly
+// (ldexp by the two scales' exponents, one rounding).
+//
+// Range and non-finite values.  An element far below its row's maximum
+// loses bits (lo, then hi, become FP16 subnormals; below 2^-40 of the max it
+// flushes to zero), and an Inf/NaN element cannot be scaled at all.  The
+// packing pass writes such elements' hi/lo as they fall (non-finite ones as
+// zero) and records per row, in scaled units, D = sum over the row of the
+// representation error beyond 2^-22 relative (+Inf for a row holding an
+// Inf or NaN).  That excess perturbs output c[h][w] by at most
+// D_A[h] * max|B col w| + D_B[w] * max|A row h| <= (D_A[h] + D_B[w]) * 2^15
+// in scaled units; the epilogue recomputes, in PDL's own order (multiply
+// then add, ascending t, from the FP32 operands), every element where that
+// bound exceeds 2^-22 |c| or c is not finite.  Rows of normal(0,1) data
+// flag a few dozen elements of an 8192^2 product; a row with an Inf or NaN
+// gets its whole row of C from PDL's loop, so non-finite results match the
+// reference bit for bit.
 //
 // FP8 ("fp8", opt-in): the same pre-scaling puts each row/column's max in
-// [128, 256) (below e4m3's largest normal, 448) and a single e4m3 product per
-// tile runs on v_mfma_f32_16x16x32_fp8_fp8.
+// [128, 256) (below e4m3's largest normal, 448); single e4m3 products on
+// v_mfma_scale_f32_32x32x64_f8f6f4 (256x256 blocks) or
+// v_mfma_f32_16x16x32_fp8_fp8.  Only non-finite rows are recomputed: the
+// FP8 path is approximate by design.
 //
 // Layout: packed operands are [row][Kp] with K contiguous, rows padded to the
-// 128-row block tile and K to 32 (zeros).  Block = 256 threads = 4 waves in a
-// 2x2 arrangement, each wave owns a 64x64 piece of the 128x128 C tile as
-// 4x4 MFMA 16x16 tiles.
+// block tile and K to the k step (zeros).
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
@@ -32,6 +47,43 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int MB = 128, NB = 128, KB = 32, KPAD = KB + 8;
+
+// |x| for finite x, 0 for Inf and NaN: row maxima ignore non-finite values
+__device__ __forceinline__ float fin_abs(float x) {
+  const float a = fabsf(x);
+  return a < __builtin_inff() ? a : 0.0f;
+}
+
+// exponent of a power of two held as a normal float
+__device__ __forceinline__ int pow2_exp(float s) { return ((__float_as_int(s) >> 23) & 0xff) - 127; }
+
+// hi/lo split of a scaled element y (finite), and the part of its
+// representation error beyond 2^-22 |y| (both differences are exact)
+__device__ __forceinline__ float split_hilo(float y, half_t &h, half_t &l) {
+  h = (half_t)y;
+  const float r1 = y - (float)h;
+  l = (half_t)r1;
+  const float e = fabsf(r1 - (float)l) - 0x1p-22f * fabsf(y);
+  return e > 0.0f ? e : 0.0f;
+}
+
+// Whether output c (unscaled; the accumulator was c * 2^e) must come from
+// PDL's loop: the range-loss bound (D_A[h] + D_B[w]) * 2^15 (scaled units)
+// exceeds 2^-22 |c|, or c is not finite.  The GEMM epilogue and the fixup
+// kernel evaluate the same test on the same stored c.
+__device__ __forceinline__ bool mm_needs_fix(float c, float dd, int e) {
+  if (dd == 0.0f && fabsf(c) < __builtin_inff()) return false;
+  return !(ldexpf(dd * 32768.0f, -e) <= 0x1p-22f * fabsf(c));
+}
+
+// the GEMM epilogue for one output: unscale the accumulator exactly and
+// mark the row of an output the fixup kernel must recompute
+__device__ __forceinline__ float mm_finish(float v, float da, float db, float sa, float sb, int *rowflag, int64_t h) {
+  const int e = pow2_exp(sa) + pow2_exp(sb);
+  const float c = ldexpf(v, -e);
+  if (mm_needs_fix(c, da + db, e)) rowflag[h] = 1;   // the same value from every writer
+  return c;
+}
 
 // power of two that brings max|row| into [2^(e-1), 2^e) with e = target
 __device__ inline float pow2_scale(float m, int target) {
@@ -51,7 +103,7 @@ __global__ __launch_bounds__(256) void absmax_rows_wave(const float *__restrict_
   const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (r >= R) return;
   float m = 0.0f;
-  for (int64_t k = lane; k < K; k += 64) m = fmaxf(m, fabsf(src[o + r * inc_r + k * inc_k]));
+  for (int64_t k = lane; k < K; k += 64) m = fmaxf(m, fin_abs(src[o + r * inc_r + k * inc_k]));
   for (int s = 32; s >= 1; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
   if (lane == 0) smax[r] = m;
 }
@@ -67,16 +119,16 @@ __global__ __launch_bounds__(256) void absmax_rows_vec(const float *__restrict__
   const int64_t K4 = K / 4;
   for (int64_t k = lane; k < K4; k += 64) {
     const Vec<float, 4> v = *reinterpret_cast<const Vec<float, 4> *>(row + 4 * k);
-    m = fmaxf(m, fmaxf(fmaxf(fabsf(v.v[0]), fabsf(v.v[1])), fmaxf(fabsf(v.v[2]), fabsf(v.v[3]))));
+    m = fmaxf(m, fmaxf(fmaxf(fin_abs(v.v[0]), fin_abs(v.v[1])), fmaxf(fin_abs(v.v[2]), fin_abs(v.v[3]))));
   }
-  for (int64_t k = 4 * K4 + lane; k < K; k += 64) m = fmaxf(m, fabsf(row[k]));
+  for (int64_t k = 4 * K4 + lane; k < K; k += 64) m = fmaxf(m, fin_abs(row[k]));
   for (int s = 32; s >= 1; s >>= 1) m = fmaxf(m, __shfl_xor(m, s, 64));
   if (lane == 0) smax[r] = m;
 }
 
 // load a 64x64 (rows x k) tile of src into t, coalesced along whichever
 // index is unit-stride; rows unit-stride and 16-byte aligned with the whole
-// tile in range take 16-byte loads.  |x| when ABS.  Zeros outside R x K.
+// tile in range take 16-byte loads.  fin_abs(x) when ABS.  Zeros outside R x K.
 template <bool ABS>
 __device__ inline void load_tile64(float (&t)[64][65], const float *__restrict__ src, int64_t o, int64_t inc_r,
                                    int64_t inc_k, int64_t R, int64_t K, int64_t r0, int64_t k0) {
@@ -87,7 +139,7 @@ __device__ inline void load_tile64(float (&t)[64][65], const float *__restrict__
       const int rr = (e % 16) * 4, kk = e / 16;
       const Vec<float, 4> v = *reinterpret_cast<const Vec<float, 4> *>(src + o + (r0 + rr) + (k0 + kk) * inc_k);
 #pragma unroll
-      for (int i = 0; i < 4; i++) t[rr + i][kk] = ABS ? fabsf(v.v[i]) : v.v[i];
+      for (int i = 0; i < 4; i++) t[rr + i][kk] = ABS ? fin_abs(v.v[i]) : v.v[i];
     }
     return;
   }
@@ -96,11 +148,11 @@ __device__ inline void load_tile64(float (&t)[64][65], const float *__restrict__
     if (kfast) { kk = e % 64; rr = e / 64; } else { rr = e % 64; kk = e / 64; }
     const int64_t gr = r0 + rr, gk = k0 + kk;
     const float x = (gr < R && gk < K) ? src[o + gr * inc_r + gk * inc_k] : 0.0f;
-    t[rr][kk] = ABS ? fabsf(x) : x;
+    t[rr][kk] = ABS ? fin_abs(x) : x;
   }
 }
 
-// max |x| per row of one 64x64 tile (read as pack_tile reads it, coalesced
+// max finite |x| per row of one 64x64 tile (read as pack_tile reads it, coalesced
 // along whichever index is unit-stride), folded into smax with an atomic max
 // on the bit pattern (non-negative floats order like their bits); smax starts
 // at zero (zero_u32).  For operands whose k index is strided, where a thread
@@ -126,11 +178,51 @@ __global__ __launch_bounds__(256) void zero_u32(unsigned *p, int64_t n) {
   if (i < n) p[i] = 0u;
 }
 
-// scale[r] from smax[r]; zero for padded rows
-__global__ __launch_bounds__(256) void make_scales(float *s, int64_t R, int64_t Rp, int target) {
+// scale[r] from smax[r] (1 for padded rows); the rows' range-loss sums (and
+// A's recompute marks) start at zero
+__global__ __launch_bounds__(256) void make_scales(float *s, float *d, int *rowflag, int64_t R, int64_t Rp,
+                                                   int target) {
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (r >= Rp) return;
   s[r] = r < R ? pow2_scale(s[r], target) : 1.0f;
+  d[r] = 0.0f;
+  if (rowflag) rowflag[r] = 0;
+}
+
+// packs 8 consecutive k of one row: hi/lo FP16 (or e4m3) of x * sc with
+// non-finite elements as zero; returns their range-loss excess (scaled
+// units; +Inf if any element is non-finite, which FP8 tracks alone)
+template <bool FP8>
+__device__ __forceinline__ float pack8(const float (&x)[8], float sc, half_t *hi, half_t *lo, uint8_t *q8) {
+  float ex = 0.0f;
+  float y[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const bool fin = fabsf(x[i]) < __builtin_inff();
+    y[i] = fin ? x[i] * sc : 0.0f;
+    ex = fin ? ex : __builtin_inff();
+  }
+  if constexpr (FP8) {
+    uint32_t w[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      uint32_t lo01 = __builtin_amdgcn_cvt_pk_fp8_f32(y[4 * h], y[4 * h + 1], 0, false);
+      w[h] = __builtin_amdgcn_cvt_pk_fp8_f32(y[4 * h + 2], y[4 * h + 3], lo01, true);
+    }
+    *reinterpret_cast<uint2 *>(q8) = make_uint2(w[0], w[1]);
+  } else {
+    half8 vh, vl;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      half_t h, l;
+      ex += split_hilo(y[i], h, l);
+      vh[i] = h;
+      vl[i] = l;
+    }
+    *reinterpret_cast<half8 *>(hi) = vh;
+    *reinterpret_cast<half8 *>(lo) = vl;
+  }
+  return ex;
 }
 
 // pack a (R x K) operand into hi/lo fp16 [Rp][Kp] through a 64x64 LDS tile so
@@ -138,9 +230,9 @@ __global__ __launch_bounds__(256) void make_scales(float *s, int64_t R, int64_t 
 template <bool FP8>
 __global__ __launch_bounds__(256) void pack_tile(const float *__restrict__ src, int64_t o, int64_t inc_r,
                                                  int64_t inc_k, int64_t R, int64_t K, int64_t Kp,
-                                                 const float *__restrict__ scale, half_t *__restrict__ hi,
-                                                 half_t *__restrict__ lo, uint8_t *__restrict__ q8,
-                                                 int64_t ntk) {
+                                                 const float *__restrict__ scale, float *__restrict__ dsum,
+                                                 half_t *__restrict__ hi, half_t *__restrict__ lo,
+                                                 uint8_t *__restrict__ q8, int64_t ntk) {
   __shared__ float t[64][65];
   const int64_t r0 = (blockIdx.x / ntk) * 64, k0 = (blockIdx.x % ntk) * 64;
   load_tile64<false>(t, src, o, inc_r, inc_k, R, K, r0, k0);
@@ -152,26 +244,11 @@ __global__ __launch_bounds__(256) void pack_tile(const float *__restrict__ src, 
     const int64_t gr = r0 + rr, gk = k0 + kg;
     if (gk >= Kp) continue;
     const float sc = scale[gr];
-    if constexpr (FP8) {
-      uint32_t w[2];
+    float x[8];
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        uint32_t lo01 = __builtin_amdgcn_cvt_pk_fp8_f32(t[rr][kg + 4 * h] * sc, t[rr][kg + 4 * h + 1] * sc, 0, false);
-        w[h] = __builtin_amdgcn_cvt_pk_fp8_f32(t[rr][kg + 4 * h + 2] * sc, t[rr][kg + 4 * h + 3] * sc, lo01, true);
-      }
-      *reinterpret_cast<uint2 *>(q8 + gr * Kp + gk) = make_uint2(w[0], w[1]);
-    } else {
-      half8 vh, vl;
-#pragma unroll
-      for (int i = 0; i < 8; i++) {
-        const float y = t[rr][kg + i] * sc;
-        const half_t h = (half_t)y;
-        vh[i] = h;
-        vl[i] = (half_t)(y - (float)h);
-      }
-      *reinterpret_cast<half8 *>(hi + gr * Kp + gk) = vh;
-      *reinterpret_cast<half8 *>(lo + gr * Kp + gk) = vl;
-    }
+    for (int i = 0; i < 8; i++) x[i] = t[rr][kg + i];
+    const float ex = pack8<FP8>(x, sc, hi + gr * Kp + gk, lo + gr * Kp + gk, q8 + gr * Kp + gk);
+    if (ex != 0.0f && gr < R) atomicAdd(&dsum[gr], ex);
   }
 }
 
@@ -181,8 +258,8 @@ __global__ __launch_bounds__(256) void pack_tile(const float *__restrict__ src, 
 template <bool FP8>
 __global__ __launch_bounds__(256) void pack_rows(const float *__restrict__ src, int64_t o, int64_t inc_r, int64_t R,
                                                  int64_t K, int64_t Rp, int64_t Kp, const float *__restrict__ scale,
-                                                 half_t *__restrict__ hi, half_t *__restrict__ lo,
-                                                 uint8_t *__restrict__ q8) {
+                                                 float *__restrict__ dsum, half_t *__restrict__ hi,
+                                                 half_t *__restrict__ lo, uint8_t *__restrict__ q8) {
   const int64_t ng = Kp / 8, total = Rp * ng;
   for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < total; g += (int64_t)gridDim.x * 256) {
     const int64_t row = g / ng, k0 = (g % ng) * 8;
@@ -197,27 +274,8 @@ __global__ __launch_bounds__(256) void pack_rows(const float *__restrict__ src, 
 #pragma unroll
       for (int i = 0; i < 8; i++) x[i] = (row < R && k0 + i < K) ? src[o + row * inc_r + k0 + i] : 0.0f;
     }
-    const float sc = scale[row];
-    if constexpr (FP8) {
-      uint32_t w[2];
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        uint32_t lo01 = __builtin_amdgcn_cvt_pk_fp8_f32(x[4 * h] * sc, x[4 * h + 1] * sc, 0, false);
-        w[h] = __builtin_amdgcn_cvt_pk_fp8_f32(x[4 * h + 2] * sc, x[4 * h + 3] * sc, lo01, true);
-      }
-      *reinterpret_cast<uint2 *>(q8 + row * Kp + k0) = make_uint2(w[0], w[1]);
-    } else {
-      half8 vh, vl;
-#pragma unroll
-      for (int i = 0; i < 8; i++) {
-        const float y = x[i] * sc;
-        const half_t h = (half_t)y;
-        vh[i] = h;
-        vl[i] = (half_t)(y - (float)h);
-      }
-      *reinterpret_cast<half8 *>(hi + row * Kp + k0) = vh;
-      *reinterpret_cast<half8 *>(lo + row * Kp + k0) = vl;
-    }
+    const float ex = pack8<FP8>(x, scale[row], hi + row * Kp + k0, lo + row * Kp + k0, q8 + row * Kp + k0);
+    if (ex != 0.0f && row < R) atomicAdd(&dsum[row], ex);
   }
 }
 
@@ -268,8 +326,11 @@ __global__ __launch_bounds__(WR * WC * 64) void split16_gemm(const half_t *__res
                                                              const half_t *__restrict__ Bhi,
                                                              const half_t *__restrict__ Blo,
                                                              const float *__restrict__ sA,
-                                                             const float *__restrict__ sB, float *__restrict__ c,
-                                                             MmPlan p, int64_t Kp, int64_t ntm, int64_t ntn) {
+                                                             const float *__restrict__ sB,
+                                                             const float *__restrict__ dA,
+                                                             const float *__restrict__ dB, int *rowflag,
+                                                             float *__restrict__ c, MmPlan p, int64_t Kp,
+                                                             int64_t ntm, int64_t ntn) {
   constexpr int NW = WR * WC, TI = BM / WR / 32, TJ = BN / WC / 32;
   constexpr int RA = BM / NW, RB = BN / NW;   // rows of A / B each wave stages
   static_assert(RA % 16 == 0 && RB % 16 == 0, "a wave stages whole 16-row loads");
@@ -359,15 +420,17 @@ __global__ __launch_bounds__(WR * WC * 64) void split16_gemm(const half_t *__res
         const int64_t h = tm * BM + wr * 32 * TI + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r & 3);
         const int64_t wcol = tn * BN + wc * 32 * TJ + j * 32 + (lane & 31);
         if (h < p.H && wcol < p.W)
-          c[p.off[2] + wcol * p.c_w + h * p.c_h] = (acc[i][j][r] * (1.0f / sA[h])) * (1.0f / sB[wcol]);
+          c[p.off[2] + wcol * p.c_w + h * p.c_h] = mm_finish(acc[i][j][r], dA[h], dB[wcol], sA[h], sB[wcol],
+                                                             rowflag, h);
       }
 }
 
 // FP8 e4m3: one product per tile, pipelined like split16_gemm
 __global__ __launch_bounds__(256) void fp8_gemm(const uint8_t *__restrict__ A8, const uint8_t *__restrict__ B8,
                                                const float *__restrict__ sA, const float *__restrict__ sB,
-                                               float *__restrict__ c, MmPlan p, int64_t Kp, int64_t ntm,
-                                               int64_t ntn) {
+                                               const float *__restrict__ dA, const float *__restrict__ dB,
+                                               int *rowflag, float *__restrict__ c, MmPlan p, int64_t Kp,
+                                               int64_t ntm, int64_t ntn) {
   __shared__ uint8_t lA[MB][KB + 16];
   __shared__ uint8_t lB[NB][KB + 16];
   int64_t tm, tn;
@@ -414,7 +477,8 @@ __global__ __launch_bounds__(256) void fp8_gemm(const uint8_t *__restrict__ A8, 
         const int64_t h = tm * MB + wr * 64 + i * 16 + 4 * (lane >> 4) + r;
         const int64_t wcol = tn * NB + wc * 64 + j * 16 + (lane & 15);
         if (h < p.H && wcol < p.W)
-          c[p.off[2] + wcol * p.c_w + h * p.c_h] = (acc[i][j][r] * (1.0f / sA[h])) * (1.0f / sB[wcol]);
+          c[p.off[2] + wcol * p.c_w + h * p.c_h] = mm_finish(acc[i][j][r], dA[h], dB[wcol], sA[h], sB[wcol],
+                                                             rowflag, h);
       }
 }
 
@@ -429,8 +493,11 @@ template <int BM, int BN, int WR, int WC>
 __global__ __launch_bounds__(WR * WC * 64) void fp8_gemm_big(const uint8_t *__restrict__ A8,
                                                              const uint8_t *__restrict__ B8,
                                                              const float *__restrict__ sA,
-                                                             const float *__restrict__ sB, float *__restrict__ c,
-                                                             MmPlan p, int64_t Kp, int64_t ntm, int64_t ntn) {
+                                                             const float *__restrict__ sB,
+                                                             const float *__restrict__ dA,
+                                                             const float *__restrict__ dB, int *rowflag,
+                                                             float *__restrict__ c, MmPlan p, int64_t Kp,
+                                                             int64_t ntm, int64_t ntn) {
   constexpr int NW = WR * WC, TI = BM / WR / 32, TJ = BN / WC / 32;
   constexpr int RA = BM / NW, RB = BN / NW;
   static_assert(RA % 16 == 0 && RB % 16 == 0, "a wave stages whole 16-row loads");
@@ -506,13 +573,86 @@ __global__ __launch_bounds__(WR * WC * 64) void fp8_gemm_big(const uint8_t *__re
         const int64_t h = tm * BM + wr * 32 * TI + i * 32 + 8 * (r / 4) + 4 * (lane >> 5) + (r & 3);
         const int64_t wcol = tn * BN + wc * 32 * TJ + j * 32 + (lane & 31);
         if (h < p.H && wcol < p.W)
-          c[p.off[2] + wcol * p.c_w + h * p.c_h] = (acc[i][j][r] * (1.0f / sA[h])) * (1.0f / sB[wcol]);
+          c[p.off[2] + wcol * p.c_w + h * p.c_h] = mm_finish(acc[i][j][r], dA[h], dB[wcol], sA[h], sB[wcol],
+                                                             rowflag, h);
       }
 }
 
+// PDL's loop for one output (Primitive.pd:226-242 without bad values):
+// multiply, then add, in ascending t, in float; eight loads in flight
+__device__ __forceinline__ float pdl_dot_f32(const float *a, int64_t at, const float *b, int64_t bt, int64_t T) {
+  float cc = 0.0f;
+  int64_t t = 0;
+  for (; t + 8 <= T; t += 8) {
+    float x[8], y[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      x[k] = a[(t + k) * at];
+      y[k] = b[(t + k) * bt];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) cc = cc + x[k] * y[k];
+  }
+  for (; t < T; t++) cc = cc + a[t * at] * b[t * bt];
+  return cc;
+}
+
+// Recomputes with PDL's loop the outputs the GEMM epilogue marked: for each
+// marked row, its columns are re-tested with mm_needs_fix on the stored
+// values, 256 columns at a time.  A chunk with 64 or more marked outputs (a
+// row with an Inf or NaN) has each thread compute its own; fewer, and each
+// wave computes one output at a time, its 64 lanes forming 64 products per
+// step (the next step's loads in flight) that the wave adds in ascending t.
+// Blocks stride over the rows, so every block reaches the end.
+__global__ __launch_bounds__(256) void mm_fixup(const float *a, const float *b, float *c, MmPlan p,
+                                                const float *__restrict__ sA, const float *__restrict__ sB,
+                                                const float *__restrict__ dA, const float *__restrict__ dB,
+                                                const int *__restrict__ rowflag) {
+  __shared__ int list[256];
+  __shared__ int nlist;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int64_t h = blockIdx.x; h < p.H; h += gridDim.x) {
+    if (!rowflag[h]) continue;
+    const float *arow = a + p.off[0] + h * p.a_h;
+    const int eA = pow2_exp(sA[h]);
+    const float da = dA[h];
+    for (int64_t w0 = 0; w0 < p.W; w0 += 256) {
+      if (tid == 0) nlist = 0;
+      __syncthreads();
+      const int64_t wc = w0 + tid;
+      bool mark = false;
+      if (wc < p.W) mark = mm_needs_fix(c[p.off[2] + wc * p.c_w + h * p.c_h], da + dB[wc], eA + pow2_exp(sB[wc]));
+      if (mark) list[atomicAdd(&nlist, 1)] = tid;
+      __syncthreads();
+      const int n = nlist;
+      if (n >= 64) {
+        if (mark) c[p.off[2] + wc * p.c_w + h * p.c_h] = pdl_dot_f32(arow, p.a_t, b + p.off[1] + wc * p.b_w, p.b_t, p.T);
+      } else {
+        for (int i = w; i < n; i += 4) {
+          const int64_t col = w0 + list[i];
+          const float *bcol = b + p.off[1] + col * p.b_w;
+          auto prod_at = [&](int64_t t0) {
+            const int64_t t = t0 + lane;
+            return t < p.T ? arow[t * p.a_t] * bcol[t * p.b_t] : 0.0f;
+          };
+          float cc = 0.0f, cur = prod_at(0);
+          for (int64_t t0 = 0; t0 < p.T; t0 += 64) {
+            const float nxt = t0 + 64 < p.T ? prod_at(t0 + 64) : 0.0f;
+            const int m = p.T - t0 < 64 ? (int)(p.T - t0) : 64;
+            for (int l = 0; l < m; l++) cc = cc + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur), l));
+            cur = nxt;
+          }
+          if (lane == 0) c[p.off[2] + col * p.c_w + h * p.c_h] = cc;
+        }
+      }
+      __syncthreads();   // the list is refilled for the next chunk
+    }
+  }
+}
+
 static hipError_t prep_operand(const float *src, int64_t o, int64_t inc_r, int64_t inc_k, int64_t R, int64_t K,
-                               int64_t Rp, int64_t Kp, float *scale, half_t *hi, half_t *lo, uint8_t *q8, bool fp8,
-                               hipStream_t s) {
+                               int64_t Rp, int64_t Kp, float *scale, float *dsum, int *rowflag, half_t *hi,
+                               half_t *lo, uint8_t *q8, bool fp8, hipStream_t s) {
   // rows along unit-stride k that start 16-byte aligned take the vector kernels
   const bool rows16 = inc_k == 1 && ((uintptr_t)(src + o) % 16 == 0) && inc_r % 4 == 0;
   if (rows16)
@@ -526,25 +666,26 @@ static hipError_t prep_operand(const float *src, int64_t o, int64_t inc_r, int64
     hipLaunchKernelGGL(absmax_tile, dim3((unsigned)(nr * nk)), dim3(256), 0, s, src, o, inc_r, inc_k, R, K,
                        (unsigned *)scale, nk);
   }
-  hipLaunchKernelGGL(make_scales, dim3((unsigned)((Rp + 255) / 256)), dim3(256), 0, s, scale, R, Rp, fp8 ? 8 : 15);
+  hipLaunchKernelGGL(make_scales, dim3((unsigned)((Rp + 255) / 256)), dim3(256), 0, s, scale, dsum, rowflag, R, Rp,
+                     fp8 ? 8 : 15);
   if (rows16) {
     int64_t blocks = (Rp * (Kp / 8) + 255) / 256;
     blocks = blocks > 65536 ? 65536 : blocks;
     if (fp8)
       hipLaunchKernelGGL((pack_rows<true>), dim3((unsigned)blocks), dim3(256), 0, s, src, o, inc_r, R, K, Rp, Kp, scale,
-                         hi, lo, q8);
+                         dsum, hi, lo, q8);
     else
       hipLaunchKernelGGL((pack_rows<false>), dim3((unsigned)blocks), dim3(256), 0, s, src, o, inc_r, R, K, Rp, Kp,
-                         scale, hi, lo, q8);
+                         scale, dsum, hi, lo, q8);
     return hipGetLastError();
   }
   const int64_t ntr = (Rp + 63) / 64, ntk = (Kp + 63) / 64;
   if (fp8)
     hipLaunchKernelGGL((pack_tile<true>), dim3((unsigned)(ntr * ntk)), dim3(256), 0, s, src, o, inc_r, inc_k, R, K, Kp,
-                       scale, hi, lo, q8, ntk);
+                       scale, dsum, hi, lo, q8, ntk);
   else
     hipLaunchKernelGGL((pack_tile<false>), dim3((unsigned)(ntr * ntk)), dim3(256), 0, s, src, o, inc_r, inc_k, R, K,
-                       Kp, scale, hi, lo, q8, ntk);
+                       Kp, scale, dsum, hi, lo, q8, ntk);
   return hipGetLastError();
 }
 
@@ -569,7 +710,7 @@ __global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict_
                                                         float *__restrict__ c, MmPlan p) {
   __shared__ __align__(16) float fA[SMM][SMP];
   __shared__ __align__(16) float fB[SMM][SMPB];
-  __shared__ float scA[SMM], scB[SMM], inA[SMM], inB[SMM];
+  __shared__ float scA[SMM], scB[SMM], dsA[SMM], dsB[SMM];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int H = (int)p.H, W = (int)p.W, T = (int)p.T, Tp = (T + 15) & ~15;
   // staging: thread (r0, q) holds rows r0 + 4j of A(h = r, t = q) and of
@@ -647,29 +788,60 @@ __global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict_
     __syncthreads();
     if (bi + gridDim.x < p.nbatch) fetch(bi + gridDim.x);
     // row scales: four threads per row of fA and of fB, 16 values each
-    // (16-byte LDS reads), combined within the quad
+    // (16-byte LDS reads), combined within the quad; then each row's
+    // range-loss excess (scaled units, +Inf for a non-finite element) over
+    // the same 16 values, summed within the quad
     {
       const int r = tl >> 2, k0 = (tl & 3) * 16;
-      float ma = 0.0f, mb = 0.0f;
+      // largest finite and smallest nonzero magnitudes (0 for a row with an
+      // Inf or NaN, which takes the excess pass below)
+      float ma = 0.0f, mb = 0.0f, na = __builtin_inff(), nb = __builtin_inff();
+      auto low = [](float ax) { return ax < __builtin_inff() ? (ax > 0.0f ? ax : __builtin_inff()) : 0.0f; };
 #pragma unroll
       for (int g = 0; g < 4; g++) {
-        const Vec<float, 4> xa = *reinterpret_cast<const Vec<float, 4> *>(&fA[r][k0 + 4 * g]);
+        const Vec<float, 4> v = *reinterpret_cast<const Vec<float, 4> *>(&fA[r][k0 + 4 * g]);
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-          ma = fmaxf(ma, fabsf(xa.v[e]));
-          mb = fmaxf(mb, fabsf(fB[r][k0 + 4 * g + e]));
+          const float xa = fabsf(v.v[e]), xb = fabsf(fB[r][k0 + 4 * g + e]);
+          ma = fmaxf(ma, fin_abs(xa));
+          mb = fmaxf(mb, fin_abs(xb));
+          na = fminf(na, low(xa));
+          nb = fminf(nb, low(xb));
         }
       }
-      ma = fmaxf(ma, __shfl_xor(ma, 1, 64));
-      mb = fmaxf(mb, __shfl_xor(mb, 1, 64));
-      ma = fmaxf(ma, __shfl_xor(ma, 2, 64));
-      mb = fmaxf(mb, __shfl_xor(mb, 2, 64));
+#pragma unroll
+      for (int x = 1; x <= 2; x <<= 1) {
+        ma = fmaxf(ma, __shfl_xor(ma, x, 64));
+        mb = fmaxf(mb, __shfl_xor(mb, x, 64));
+        na = fminf(na, __shfl_xor(na, x, 64));
+        nb = fminf(nb, __shfl_xor(nb, x, 64));
+      }
+      const float s1 = pow2_scale(ma, 15), s2 = pow2_scale(mb, 15);
+      // an element's hi/lo split loses bits beyond 2^-22 only when its
+      // scaled magnitude is below 2^-3 (lo then falls among FP16's
+      // subnormals): rows without one skip the excess pass
+      float ea = 0.0f, eb = 0.0f;
+      if (na * s1 < 0.125f)
+        for (int e = 0; e < 16; e++) {
+          const float xa = fA[r][k0 + e];
+          half_t h, l;
+          ea += fabsf(xa) < __builtin_inff() ? split_hilo(xa * s1, h, l) : __builtin_inff();
+        }
+      if (nb * s2 < 0.125f)
+        for (int e = 0; e < 16; e++) {
+          const float xb = fB[r][k0 + e];
+          half_t h, l;
+          eb += fabsf(xb) < __builtin_inff() ? split_hilo(xb * s2, h, l) : __builtin_inff();
+        }
+      ea += __shfl_xor(ea, 1, 64);
+      eb += __shfl_xor(eb, 1, 64);
+      ea += __shfl_xor(ea, 2, 64);
+      eb += __shfl_xor(eb, 2, 64);
       if ((tl & 3) == 0) {
-        const float s1 = pow2_scale(ma, 15), s2 = pow2_scale(mb, 15);
         scA[r] = s1;
         scB[r] = s2;
-        inA[r] = 1.0f / s1;   // powers of two: exact
-        inB[r] = 1.0f / s2;
+        dsA[r] = ea;
+        dsB[r] = eb;
       }
     }
     __syncthreads();
@@ -683,8 +855,10 @@ __global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict_
       const Vec<float, 4> a1 = *reinterpret_cast<const Vec<float, 4> *>(&fA[ar][k0 + kq + 4]);
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        const float ya = (i < 4 ? a0.v[i] : a1.v[i - 4]) * sa;
-        const float yb = fB[br][k0 + kq + i] * sb;
+        // non-finite elements enter as zero (their rows are recomputed)
+        const float xa = i < 4 ? a0.v[i] : a1.v[i - 4], xb = fB[br][k0 + kq + i];
+        const float ya = fabsf(xa) < __builtin_inff() ? xa * sa : 0.0f;
+        const float yb = fabsf(xb) < __builtin_inff() ? xb * sb : 0.0f;
         const half_t ha = (half_t)ya, hb = (half_t)yb;
         ah[i] = ha;
         al[i] = (half_t)(ya - (float)ha);
@@ -697,13 +871,30 @@ __global__ __launch_bounds__(256, 4) void split16_small(const float *__restrict_
     }
     int64_t o[3];
     mm_batch_offsets_m(p, bi, o);
-    // D layout of a 32x32 tile: lane l, reg r -> row 8*(r/4) + 4*(l/32) + r%4, col l%32
+    // D layout of a 32x32 tile: lane l, reg r -> row 8*(r/4) + 4*(l/32) + r%4, col l%32.
+    // Elements whose range-loss bound exceeds 2^-22 |c| (see mm_finish) are
+    // noted in a mask and then recomputed with PDL's loop from the staged
+    // FP32 operands, once the accumulators are no longer live
+    unsigned redo = 0;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
       const int h = 32 * wr + 8 * (r / 4) + 4 * (lanel >> 5) + (r & 3);
       const int wcol = 32 * wc + (lanel & 31);
-      if (h < H && wcol < W)
-        c[o[2] + wcol * p.c_w + h * p.c_h] = (acc[r] * inA[h]) * inB[wcol];
+      if (h < H && wcol < W) {
+        const float v = acc[r], dd = dsA[h] + dsB[wcol];
+        if ((dd != 0.0f || !(fabsf(v) < __builtin_inff())) && !(dd * 32768.0f <= 0x1p-22f * fabsf(v)))
+          redo |= 1u << r;
+        c[o[2] + wcol * p.c_w + h * p.c_h] = ldexpf(v, -(pow2_exp(scA[h]) + pow2_exp(scB[wcol])));
+      }
+    }
+    while (redo) {
+      const int r = __builtin_ctz(redo);
+      redo &= redo - 1;
+      const int h = 32 * wr + 8 * (r / 4) + 4 * (lanel >> 5) + (r & 3);
+      const int wcol = 32 * wc + (lanel & 31);
+      float out = 0.0f;
+      for (int t = 0; t < T; t++) out = out + fA[h][t] * fB[wcol][t];
+      c[o[2] + wcol * p.c_w + h * p.c_h] = out;
     }
     __syncthreads();   // fA, fB and the scales are refilled next
   }
@@ -853,10 +1044,14 @@ hipError_t launch_mfma_f32(const MmPlan &p, const float *a, const float *b, floa
   const int64_t BMr = big ? 256 : MB, BNr = big ? 256 : NB, KS = (big && fp8) ? KB8 : KB;
   const int64_t Hp = (p.H + BMr - 1) / BMr * BMr, Wp = (p.W + BNr - 1) / BNr * BNr, Kp = (p.T + KS - 1) / KS * KS;
   const size_t esz = fp8 ? 1 : 2, ncopy = fp8 ? 1 : 2;
-  char *buf = (char *)scratch_alloc((size_t)(Hp + Wp) * Kp * esz * ncopy + (size_t)(Hp + Wp) * sizeof(float) + 256);
+  // [scales A | scales B | range-loss sums A | B | recompute marks per row
+  // of A] then the packed panels
+  const size_t head = 2 * (size_t)(Hp + Wp) * sizeof(float) + (size_t)Hp * sizeof(int);
+  char *buf = (char *)scratch_alloc((size_t)(Hp + Wp) * Kp * esz * ncopy + head + 256);
   if (!buf) return hipErrorOutOfMemory;
-  float *sA = (float *)buf, *sB = sA + Hp;
-  char *q = buf + ((size_t)(Hp + Wp) * sizeof(float) + 255) / 256 * 256;
+  float *sA = (float *)buf, *sB = sA + Hp, *dA = sB + Wp, *dB = dA + Hp;
+  int *rowflag = (int *)(dB + Wp);
+  char *q = buf + (head + 255) / 256 * 256;
   half_t *Ahi = nullptr, *Alo = nullptr, *Bhi = nullptr, *Blo = nullptr;
   uint8_t *A8 = nullptr, *B8 = nullptr;
   if (fp8) {
@@ -870,22 +1065,27 @@ hipError_t launch_mfma_f32(const MmPlan &p, const float *a, const float *b, floa
   }
   int64_t o[3] = {p.off[0], p.off[1], p.off[2]};
   // A rows = h (stride a_h), k = t (stride a_t); B "rows" = w (b_w), k = t (b_t)
-  hipError_t e = prep_operand(a, o[0], p.a_h, p.a_t, p.H, p.T, Hp, Kp, sA, Ahi, Alo, A8, fp8, s);
-  if (e == hipSuccess) e = prep_operand(b, o[1], p.b_w, p.b_t, p.W, p.T, Wp, Kp, sB, Bhi, Blo, B8, fp8, s);
+  hipError_t e = prep_operand(a, o[0], p.a_h, p.a_t, p.H, p.T, Hp, Kp, sA, dA, rowflag, Ahi, Alo, A8, fp8, s);
+  if (e == hipSuccess) e = prep_operand(b, o[1], p.b_w, p.b_t, p.W, p.T, Wp, Kp, sB, dB, nullptr, Bhi, Blo, B8, fp8, s);
   const int64_t ntm = Hp / BMr, ntn = Wp / BNr;
   if (e == hipSuccess) {
     if (fp8 && big)
       hipLaunchKernelGGL((fp8_gemm_big<256, 256, 4, 2>), dim3((unsigned)(ntm * ntn)), dim3(512), 0, s, A8, B8, sA, sB,
-                         c, p, Kp, ntm, ntn);
+                         dA, dB, rowflag, c, p, Kp, ntm, ntn);
     else if (fp8)
-      hipLaunchKernelGGL(fp8_gemm, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, A8, B8, sA, sB, c, p, Kp, ntm,
-                         ntn);
+      hipLaunchKernelGGL(fp8_gemm, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, A8, B8, sA, sB, dA, dB, rowflag, c,
+                         p, Kp, ntm, ntn);
     else if (big)
       hipLaunchKernelGGL((split16_gemm<256, 256, 4, 2>), dim3((unsigned)(ntm * ntn)), dim3(512), 0, s, Ahi, Alo, Bhi,
-                         Blo, sA, sB, c, p, Kp, ntm, ntn);
+                         Blo, sA, sB, dA, dB, rowflag, c, p, Kp, ntm, ntn);
     else
       hipLaunchKernelGGL((split16_gemm<128, 128, 2, 2>), dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, Ahi, Alo, Bhi,
-                         Blo, sA, sB, c, p, Kp, ntm, ntn);
+                         Blo, sA, sB, dA, dB, rowflag, c, p, Kp, ntm, ntn);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    const int64_t fb = p.H < 2048 ? p.H : 2048;
+    hipLaunchKernelGGL(mm_fixup, dim3((unsigned)fb), dim3(256), 0, s, a, b, c, p, sA, sB, dA, dB, rowflag);
     e = hipGetLastError();
   }
   scratch_free(buf);

@@ -35,6 +35,12 @@ hipError_t launch_matmult_list(int T, const MmItem *d_items, const MmItem *h_ite
                                unsigned int nwork, unsigned int *d_counter, int tile, bool fused, bool each,
                                int nblocks, hipStream_t s);
 
+// Bad.pd elementwise ops (k_badops.hip): a of type T, mask int32 (setbadif),
+// b of type T (int32 for isbad/isgood); dval the double OtherPar; flag set
+// where an op found values to set bad (setnantobad and its like)
+hipError_t launch_badop(int T, int op, const EwPlan &p, const void *a, const void *m, void *b, const BadInfo &ba,
+                        int32_t mbv, const pdlhip_value &bbv, double dval, bool bad, int *flag, hipStream_t s);
+
 // ragged lists (k_ragged.hip)
 hipError_t launch_ragged_reduce(int T, int op, const RagLaunch &L, hipStream_t s);
 hipError_t launch_ragged_scan(int T, int op, const RagLaunch &L, hipStream_t s);

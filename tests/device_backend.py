@@ -13,8 +13,8 @@ class DeviceBackend:
     def null(self):
         return C.ndarray()
 
-    def run(self, op, pdls, swap=0, m_size=None):
-        C._run(op, pdls, swap=swap, m_size=m_size)
+    def run(self, op, pdls, swap=0, m_size=None, dpar=None):
+        C._run(op, pdls, swap=swap, m_size=m_size, dpar=dpar)
 
     def dims(self, p):
         return list(p.dims)

@@ -250,6 +250,55 @@ Y3 = arr("double", [3, 3], [0, BAD, BAD, 3, BAD, BAD, 6, BAD, BAD], bad=[1, 2, 4
 case("bad_inplace_bifunc_power", "t/bad.t:296-300", "power", [dict(seq("double", 3, 3)), Y3, "inplace:0"],
      out(0, "double", [3, 3], [1, BAD, BAD, 27, BAD, BAD, 6.0 ** 6, BAD, BAD], bad=[1, 2, 4, 5, 7, 8], badflag=True))
 
+# Bad.pd elementwise ops (t/bad.t)
+DMAX = 1.7976931348623157e308       # double's default bad value is -DBL_MAX (Types.pm defbval)
+FMAX = 3.4028234663852886e38
+case("bad_isbad", "t/bad.t:118-119", "isbad", [arr("byte", [9], [1, 2, 255, 255, 5, 6, 255, 8, 9], bad=[2, 3, 6]), NULL],
+     out(1, "long", [9], [0, 0, 1, 1, 0, 0, 1, 0, 0]))
+case("bad_isgood", "t/bad.t:118-120", "isgood", [arr("byte", [9], [1, 2, 255, 255, 5, 6, 255, 8, 9], bad=[2, 3, 6]), NULL],
+     out(1, "long", [9], [1, 1, 0, 0, 1, 1, 0, 1, 1]))
+X20 = [42, 47, 98, 13, 22, 96, 74, 41, 79, 76, 96, 3, 32, 76, 25, 59, 5, 96, 32, 6]
+Y20 = [(-DMAX if v < 20 else v) for v in X20]
+case("bad_setbadif_lt20", "t/bad.t:173-182", "setbadif",
+     [arr("double", [20], X20), arr("double", [20], [1.0 if v < 20 else 0.0 for v in X20]), NULL],
+     out(2, "double", [20], Y20, badflag=True))
+case("bad_setbadtoval_20", "t/bad.t:175-176", "setbadtoval",
+     [arr("double", [20], Y20, bad=[i for i, v in enumerate(X20) if v < 20]), NULL],
+     out(1, "double", [20], [20 if v < 20 else v for v in X20], badflag=False), dpar=20.0)
+case("bad_setvaltobad_seq_mod4", "t/bad.t:318-321", "setvaltobad", [arr("double", [10], [i % 4 for i in range(10)]), NULL],
+     out(1, "double", [10], [0, -DMAX, 2, 3, 0, -DMAX, 2, 3, 0, -DMAX], badflag=True), dpar=1.0)
+case("bad_setbadtonan", "t/bad.t:323-324", "setbadtonan",
+     [arr("double", [10], [0, -DMAX, 2, 3, 0, -DMAX, 2, 3, 0, -DMAX], bad=[1, 5, 9]), NULL],
+     out(1, "double", [10], [0, math.nan, 2, 3, 0, math.nan, 2, 3, 0, math.nan], badflag=False))
+case("bad_setvaltobad_float", "t/bad.t:327", "setvaltobad", [arr("float", [4], [1, 2, 3, 4]), NULL],
+     out(1, "float", [4], [1, -FMAX, 3, 4], badflag=True), dpar=2.0)
+case("bad_setvaltobad_double", "t/bad.t:328", "setvaltobad", [arr("double", [4], [1, 2, 3, 4]), NULL],
+     out(1, "double", [4], [1, -DMAX, 3, 4], badflag=True), dpar=2.0)
+case("bad_setinftobad", "t/bad.t:330-332", "setinftobad", [arr("double", [3], [0, math.inf, math.nan]), NULL],
+     out(1, "double", [3], [0, -DMAX, math.nan], badflag=True))
+case("bad_setnonfinitetobad", "t/bad.t:334-336", "setnonfinitetobad",
+     [arr("double", [10], [0, math.inf, 2, 3, 0, math.nan, 2, 3, 0, math.nan]), NULL],
+     out(1, "double", [10], [0, -DMAX, 2, 3, 0, -DMAX, 2, 3, 0, -DMAX], badflag=True))
+case("bad_setnantobad", "t/bad.t:323-343", "setnantobad",
+     [arr("double", [10], [0, math.nan, 2, 3, 0, math.nan, 2, 3, 0, math.nan]), NULL],
+     out(1, "double", [10], [0, -DMAX, 2, 3, 0, -DMAX, 2, 3, 0, -DMAX], badflag=True))
+
+# Ops.pd ufuncs (t/ops.t)
+case("ops_exp", "t/ops.t:111-112", "exp", [arr("double", [2], [2, 3]), NULL],
+     out(1, "double", [2], [7.3891, 20.0855], tol=1e-2))
+case("ops_sqrt_approx", "t/ops.t:111-113", "sqrt", [arr("double", [2], [2, 3]), NULL],
+     out(1, "double", [2], [1.4142, 1.7321], tol=1e-2))
+case("ops_sqrt_exact", "t/ops.t:76-78", "sqrt", [arr("double", [3], [16, 64, 9]), NULL],
+     out(1, "double", [3], [4, 8, 3]))
+case("ops_log10", "t/ops.t:178-180", "log10", [arr("double", [2], [110, 23]), NULL],
+     out(1, "double", [2], [math.log(110) / math.log(10), math.log(23) / math.log(10)], tol=1e-12))
+case("ops_log_float_type", "t/ops.t:173-174", "log", [arr("float", [5, 4], [i + 2 for i in range(20)]), NULL],
+     out(1, "float", [5, 4], [math.log(i + 2) for i in range(20)], tol=1e-6))
+case("ops_bitnot_double_to_longlong", "t/ops.t:272", "bitnot", [arr("double", [3], [1, 2, 3]), NULL],
+     out(1, "longlong", [3], [-2, -3, -4]))
+case("ops_bitnot_ulonglong", "t/ops.t:267-269", "bitnot", [arr("ulonglong", [2], [0, 1]), NULL],
+     out(1, "ulonglong", [2], [18446744073709551615, 18446744073709551614]))
+
 # ---------------------------------------------------------------- t/core.t dim compatibility (append)
 def z(*dims):
     return zeroes("double", *dims)

@@ -108,7 +108,7 @@ def run_case(B, case):
     elif pm == "append":
         objs[2] = pm_append(B, objs[0], objs[1], objs[2])
     else:
-        B.run(case["op"], objs, swap=case.get("swap", 0), m_size=case.get("m_size"))
+        B.run(case["op"], objs, swap=case.get("swap", 0), m_size=case.get("m_size"), dpar=case.get("dpar"))
     return objs
 
 

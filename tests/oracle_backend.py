@@ -14,8 +14,8 @@ class OracleBackend:
     def null(self):
         return O.OPdl.null()
 
-    def run(self, op, pdls, swap=0, m_size=None):
-        O.run(op, *pdls, swap=swap, m_size=m_size)
+    def run(self, op, pdls, swap=0, m_size=None, dpar=None):
+        O.run(op, *pdls, swap=swap, m_size=m_size, dpar=dpar)
 
     def dims(self, p):
         return list(p.dims)

@@ -178,3 +178,107 @@ def test_square_via_pow_differs_from_the_product_rarely():
     # wherever they differ, the two are adjacent floats
     gi, si = got[diff].view(np.int32), sq[diff].view(np.int32)
     assert np.all(np.abs(gi.astype(np.int64) - si) == 1)
+
+
+# ---------------------------------------------------------------- Ops.pd ufuncs
+def run_u(op, a, t):
+    pa, pb = O.OPdl.from_numpy(a, t), O.OPdl.null()
+    O.run(op, pa, pb)
+    return pb
+
+
+# glibc's documented maximum errors on x86-64 (libm-test-ulps): 1 ulp for
+# these, 2 for double log10; sqrt is correctly rounded
+ULP_BOUND = {("sqrt", O.F): 0.5, ("sqrt", O.D): 0.5, ("log10", O.D): 2.0}
+UFUNC_TRUTH = {"sin": mpmath.sin, "cos": mpmath.cos, "exp": mpmath.exp, "log": mpmath.log,
+               "log10": mpmath.log10, "sqrt": mpmath.sqrt}
+
+
+@pytest.mark.parametrize("t", [O.F, O.D], ids=["float", "double"])
+@pytest.mark.parametrize("op", ["sin", "cos", "exp", "log", "log10", "sqrt"])
+def test_ufunc_within_one_ulp_of_the_exact_value(op, t):
+    """Ops.pd ufuncs through glibc (tgmath picks sinf/sin ...): within
+    glibc's documented bound of the 120-bit value (1 ulp; 2 for double
+    log10; sqrt: the correctly rounded value exactly); sin
+    and cos over several periods and near 0, exp over its finite range,
+    the logarithms over many binades"""
+    dt = FMT[t][0]
+    n = 1200
+    if op in ("sin", "cos"):
+        x = np.concatenate([RNG.uniform(-40, 40, n), RNG.uniform(-1e-3, 1e-3, 100)]).astype(dt)
+    elif op == "exp":
+        lim = 85.0 if t == O.F else 700.0
+        x = RNG.uniform(-lim, lim, n).astype(dt)
+    else:
+        x = np.exp(RNG.uniform(-60, 60, n)).astype(dt)
+    got = run_u(op, x, t).numpy()
+    f = UFUNC_TRUTH[op]
+    worst = 0.0
+    for xi, g in zip(x, got):
+        tr = f(mpmath.mpf(float(xi)))
+        err = ulp_error(g, tr, t)
+        worst = max(worst, err)
+        assert err <= ULP_BOUND.get((op, t), 1.0), (op, float(xi), float(g), str(tr), err)
+    assert worst > 0.0
+
+
+@pytest.mark.parametrize("t", [O.SB, O.B, O.S, O.US, O.L, O.UL, O.LL, O.ULL],
+                         ids=["sbyte", "byte", "short", "ushort", "long", "ulong", "longlong", "ulonglong"])
+def test_ufunc_integer_types_convert_the_double_result(t):
+    """sqrt/sin/cos/log10 ($A) on integer types: tgmath calls the double
+    function and PP casts back, ($GENERIC(b))sqrt((double)a): the truncated
+    value of the exact function (positive inputs, away from the undefined
+    conversions of NaN/-inf)"""
+    dt = np.dtype(O.NP_TYPES[t])
+    hi = min(int(np.iinfo(dt).max), 10 ** 9)
+    x = RNG.integers(1, hi, 300, endpoint=True).astype(dt)
+    x[:4] = np.array([1, 10, 100, min(1000, hi)], dtype=dt)[:4]
+    for op, f in (("sqrt", math.sqrt), ("sin", math.sin), ("cos", math.cos), ("log10", math.log10)):
+        o = run_u(op, x, t)
+        assert o.dtype == t, op
+        want = np.array([int(f(float(int(v)))) for v in x]).astype(dt)
+        np.testing.assert_array_equal(o.numpy(), want, err_msg=op)
+
+
+def test_ufunc_type_selection():
+    """GenericTypes of each ufunc: exp/log take [@$C, @$F] with D last, so an
+    integer input computes in double; bitnot takes the integer types ($T),
+    so a double input computes in longlong (t/ops.t:272); sqrt keeps an
+    integer type ($A); not keeps every real type"""
+    ints = np.array([1, 4, 9], np.int32)
+    assert run_u("exp", ints, O.L).dtype == O.D
+    assert run_u("log", np.array([1, 2], np.int16), O.S).dtype == O.D
+    assert run_u("exp", np.array([1, 2], np.float32), O.F).dtype == O.F
+    assert run_u("bitnot", np.array([1.0, 2.0]), O.D).dtype == O.LL
+    assert run_u("sqrt", ints, O.L).dtype == O.L
+    assert run_u("not", np.array([0, 3], np.uint8), O.B).dtype == O.B
+
+
+@pytest.mark.parametrize("t", [O.SB, O.B, O.S, O.US, O.L, O.UL, O.IND, O.ULL, O.LL, O.F, O.D],
+                         ids=lambda t: O.TYPE_NAMES[t])
+def test_not_bitnot_abs_closed_forms(t):
+    """`!a` (0/1 in the type), `~a` (integers), and _rabs (Ops.pd:343-358):
+    the identity for unsigned types, PDL_ABS(a) = a >= 0 ? a : -a otherwise
+    (so -0.0 stays -0.0, NaN changes sign, the most negative integer wraps)"""
+    dt = np.dtype(O.NP_TYPES[t])
+    if dt.kind in "iu":
+        info = np.iinfo(dt)
+        x = RNG.integers(info.min, info.max, 400, endpoint=True, dtype=dt)
+        x[:3] = [0, info.min, info.max]
+    else:
+        x = RNG.standard_normal(400).astype(dt)
+        x[:4] = [0.0, -0.0, np.nan, -np.inf]
+    got_not = run_u("not", x, t).numpy()
+    np.testing.assert_array_equal(got_not, (x == 0).astype(dt))
+    if dt.kind in "iu":
+        np.testing.assert_array_equal(run_u("bitnot", x, t).numpy(), ~x)
+    got_abs = run_u("abs", x, t).numpy()
+    with np.errstate(over="ignore"):
+        if dt.kind == "u":
+            want = x
+        elif dt.kind == "i":
+            want = np.where(x >= 0, x, (-x.astype(object)).astype(object)).astype(object)
+            want = np.array([int(v) if int(v) <= info.max else info.min for v in want], dtype=dt)
+        else:
+            want = np.where(x >= 0, x, -x)
+    np.testing.assert_array_equal(got_abs.view(np.uint8), np.asarray(want, dtype=dt).view(np.uint8))
