@@ -48,14 +48,32 @@ def measured_traffic(op):
         return None
 
 
-def make_int_inputs(ni, nj, mi, mj, seed=0):
-    rng = np.random.default_rng(seed)
-    i32 = rng.integers(-2**20, 2**20, size=(nj, ni), endpoint=True, dtype=np.int32)
-    i16 = rng.integers(-32767, 32767, size=(mj, mi), endpoint=True, dtype=np.int16)
-    flat = i16.reshape(-1)
-    flat[rng.random(flat.size) < 0.01] = -32768
-    i16[17, :] = -32768                       # an all-bad row
-    return i32, i16
+# ---------------------------------------------------------------- inputs
+# Every workload's inputs are drawn on the device from PDL's own stream
+# (srandom / random / grandom: one xoshiro256+ stream, the one PDL draws
+# with autopthread off), written as the PDL expressions that make them:
+#   uniform[-1,1]  random(type, dims) * 2 - 1
+#   normal(0,1)    float(grandom(dims))   (double draws, converted once: a
+#                  float grandom would round draws within 2^-25 of 1 to 1.0f
+#                  and give +Inf, as PDL's does)
+
+def uniform_pm1(C, *dims, dtype):
+    return C.random(*dims, dtype=dtype) * 2 - 1
+
+
+def normal_f32(C, *dims):
+    return C.grandom(*dims).convert(C.L.F)
+
+
+def make_int_inputs(C, ni, nj, mi, mj):
+    """int32 (ni,nj) uniform in [-2^20, 2^20]: long(random(ni,nj) * 2097153) - 1048576;
+    int16 (mi,mj) in [-32767, 32767]: short(long(random(mi,mj) * 65535) - 32767),
+    then setbadif(random(mi,mj) < 0.01, and all of row 17) -- an all-bad row"""
+    i32 = (C.random(ni, nj) * 2097153).convert(C.L.L) - 1048576
+    v = ((C.random(mi, mj) * 65535).convert(C.L.L) - 32767).convert(C.L.S)
+    mask = C.random(mi, mj) < 0.01
+    C._run("assgn", [C.topdl(1), mask.slice(None, (17, 17, 1))])
+    return i32, v.setbadif(mask)
 
 
 def run_gpu(args):
@@ -65,15 +83,16 @@ def run_gpu(args):
     NX, NY, NK = 4096, 4096, 16
     NI, NJ = 4096, 4096
     MI, MJ = 8192, 8192
-    x = C.random_normal(NX, NY, NK, dtype=C.L.F, seed=1)          # PDL dims (4096,4096,16)
-    y = C.random_normal(NY, NK, dtype=C.L.F, seed=2)              # (4096,16)
+    C.srandom(1)
+    x = normal_f32(C, NX, NY, NK)                                 # PDL dims (4096,4096,16)
+    y = normal_f32(C, NY, NK)                                     # (4096,16)
     xv = x.xchg(0, 1)
     yv = y.dummy(1)
     z = C.ndarray.new_dense(C.L.F, [NY, NX, NK])
     outs = {k: C.ndarray.new_dense(C.L.F, [NX, NK]) for k in ("sumover", "average", "minimum", "maximum")}
-    i32h, i16h = make_int_inputs(NI, NJ, MI, MJ)
-    i32 = C.ndarray.from_numpy(i32h, C.L.L)
-    i16 = C.ndarray.from_numpy(i16h, C.L.S, badflag=True)
+    i32, i16 = make_int_inputs(C, NI, NJ, MI, MJ)
+    i32h, i16h = i32.numpy(), i16.numpy()
+    assert i16.state & C.L.BADVAL and np.all(i16h[17] == -32768)
     is_out = C.ndarray.new_dense(C.L.L, [NJ])
     ic_out = C.ndarray.new_dense(C.L.L, [NI, NJ])
     ib_out = C.ndarray.new_dense(C.L.L, [MJ])
@@ -175,14 +194,12 @@ def run_matmult(n=8192, nrows_check=16, reps=3):
     res = {"n": n, "check_rows": nrows_check}
     rows = np.sort(rng.choice(n, nrows_check, replace=False))
     for case in ("normal", "lognormal_rows"):
-        a = C.random_normal(n, n, dtype=C.L.F, seed=21)
-        b = C.random_normal(n, n, dtype=C.L.F, seed=22)
+        C.srandom(21)
+        a = normal_f32(C, n, n)
+        b = normal_f32(C, n, n)
         if case == "lognormal_rows":
-            s = np.exp(3.0 * rng.standard_normal(n)).astype(np.float32)   # per row h of A
-            sd = C.ndarray.from_numpy(s.reshape(n, 1), C.L.F)               # dims (1,n): broadcast over t
-            a2 = C.ndarray.new_dense(C.L.F, [n, n])
-            C._run("mult", [a, sd, a2])
-            a = a2
+            # rows h of A times float(exp(grandom(1,n) * 3)): dims (1,n) broadcast over t
+            a = a * (C.grandom(1, n) * 3).exp().convert(C.L.F)
         ah = a.numpy()
         bh = b.numpy()
         truth = ah[rows].astype(np.float64) @ bh.astype(np.float64)
@@ -216,8 +233,9 @@ def run_matmult(n=8192, nrows_check=16, reps=3):
         res[case] = out
     # double on the FP64 path
     m = n // 2
-    a = C.random_uniform(m, m, dtype=C.L.D, seed=31, lo=-1, hi=1)
-    b = C.random_uniform(m, m, dtype=C.L.D, seed=32, lo=-1, hi=1)
+    C.srandom(31)
+    a = uniform_pm1(C, m, m, dtype=C.L.D)
+    b = uniform_pm1(C, m, m, dtype=C.L.D)
     c = C.ndarray.new_dense(C.L.D, [m, m])
     C._run("matmult", [a, b, c])
     C.synchronize()
@@ -243,11 +261,10 @@ def run_small(reps=20):
 
     N, M = 2048, 512
     D = C.L.D
-    u = O.random_uniform(2 * N * N + 2 * M * M, 0) * 2.0 - 1.0
-    ah, bh = u[:N * N].reshape(N, N), u[N * N:2 * N * N].reshape(N, N)
-    mah, mbh = u[2 * N * N:2 * N * N + M * M].reshape(M, M), u[2 * N * N + M * M:].reshape(M, M)
-    a, b = C.ndarray.from_numpy(ah, D), C.ndarray.from_numpy(bh, D)
-    ma, mb = C.ndarray.from_numpy(mah, D), C.ndarray.from_numpy(mbh, D)
+    C.srandom(0)
+    a, b = uniform_pm1(C, N, N, dtype=D), uniform_pm1(C, N, N, dtype=D)
+    ma, mb = uniform_pm1(C, M, M, dtype=D), uniform_pm1(C, M, M, dtype=D)
+    ah, bh, mah, mbh = a.numpy(), b.numpy(), ma.numpy(), mb.numpy()
     t, c = C.ndarray.new_dense(D, [N, N]), C.ndarray.new_dense(D, [N, N])
     s, m = C.ndarray.new_dense(D, [N]), C.ndarray.new_dense(D, [M, M])
     ops = (("a*b+a", lambda: (C._run("mult", [a, b, t]), C._run("plus", [t, a, c]))),
@@ -288,7 +305,8 @@ def run_small(reps=20):
     s_truth = cg.astype(np.longdouble).sum(axis=-1).astype(np.float64)
     m_truth = O.truth_matmult_ld(mah, mbh)
     rel = lambda x, tr: float(np.linalg.norm(x - tr) / np.linalg.norm(tr))
-    return {"workload": "a,b double (2048,2048) U[-1,1] seed 0: c=a*b+a, s=sumover(c); matmult double 512^2",
+    return {"workload": "srandom(0); a,b = random(2048,2048)*2-1; c=a*b+a, s=sumover(c); matmult of the next two "
+                        "random(512,512)*2-1",
             "gpu_ms": gpu, "cpu_ms": cpu, "cpu_threads": nthr,
             "c_bit_exact": bool(np.array_equal(c.numpy().view(np.uint64), cg.view(np.uint64))),
             "fused_c_bit_exact": bool(np.array_equal(fc.numpy().view(np.uint64), cg.view(np.uint64))),
@@ -305,15 +323,15 @@ def run_ragged_matmult(count=4096, lo=8, hi=256, seed=0, reps=5):
     matmult over 65536."""
     from pdl_amd import core as C
 
-    rng = np.random.default_rng(seed)
-    ns = rng.integers(lo, hi, size=count, endpoint=True)
+    C.srandom(seed)
+    ns = lo + np.floor(C.random(count).numpy() * (hi - lo + 1)).astype(np.int64)   # U{lo..hi}
     flops = float(np.sum(2.0 * ns.astype(np.float64) ** 3))
     out = {"count": count, "n_range": [lo, hi], "flop": flops,
            "bytes_f32": int(np.sum(3 * ns.astype(np.int64) ** 2) * 4)}
     tot = int(np.sum(ns.astype(np.int64) ** 2))
     for t, name in ((C.L.F, "f32"), (C.L.D, "f64")):
-        A = C.random_uniform(tot, dtype=t, seed=41, lo=-1, hi=1)
-        B = C.random_uniform(tot, dtype=t, seed=42, lo=-1, hi=1)
+        A = uniform_pm1(C, tot, dtype=t)
+        B = uniform_pm1(C, tot, dtype=t)
         Cb = C.ndarray.new_dense(t, [tot])
         As, Bs, Cs, off = [], [], [], 0
         for n in ns:
@@ -350,8 +368,8 @@ def run_ragged_matmult(count=4096, lo=8, hi=256, seed=0, reps=5):
         ml.free()
         out[name] = res
     # broadcast matmult (64,64) x (64,64) over a broadcast dim of 65536
-    a = C.random_uniform(64, 64, 65536, dtype=C.L.F, seed=51, lo=-1, hi=1)
-    b = C.random_uniform(64, 64, 65536, dtype=C.L.F, seed=52, lo=-1, hi=1)
+    a = uniform_pm1(C, 64, 64, 65536, dtype=C.L.F)
+    b = uniform_pm1(C, 64, 64, 65536, dtype=C.L.F)
     c = C.ndarray.new_dense(C.L.F, [64, 64, 65536])
     C._run("matmult", [a, b, c])
     C.synchronize()
@@ -382,14 +400,16 @@ def run_ragged_chain(nvec=100000, reps=100, loop_sample=1000, seed=0):
 
     F, D, B, LL = C.L.F, C.L.D, C.L.B, C.L.LL
     rng = np.random.default_rng(seed)
-    lens = np.clip(np.round(rng.lognormal(8.0, 1.5, nvec)), 1, 2 ** 20).astype(np.int64)
+    C.srandom(seed)
+    # lengths lognormal(mean 8, sigma 1.5) in numpy's sense: exp(8 + 1.5 N(0,1))
+    lens = np.clip(np.round(np.exp(8.0 + 1.5 * C.grandom(nvec).numpy())), 1, 2 ** 20).astype(np.int64)
     tot = int(lens.sum())
     offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
-    A = C.random_normal(tot, dtype=F, seed=61)
-    Bv = C.random_normal(tot, dtype=F, seed=62)
-    W = C.random_uniform(tot, dtype=F, seed=63, lo=0.0, hi=1.0)
-    M = C.random_uniform(tot, dtype=B, seed=64, lo=0.0, hi=2.0)       # 0/1 mask
-    I = C.random_uniform(tot, dtype=LL, seed=65, lo=-2.0 ** 40, hi=2.0 ** 40)
+    A = normal_f32(C, tot)
+    Bv = normal_f32(C, tot)
+    W = C.random(tot, dtype=F)
+    M = (C.random(tot) * 2).convert(B)                                  # 0/1 mask
+    I = (C.random(tot) * 2 ** 41).convert(LL) - 2 ** 40
     two = C.topdl(2)
     views = lambda big: [C.view_into(big, [int(n)], int(o)) for n, o in zip(lens, offs)]
     va, vb, vw, vm, vi = views(A), views(Bv), views(W), views(M), views(I)
@@ -493,10 +513,14 @@ def run_single(args):
     if args.only == "matmult":
         print(json.dumps({"matmult": run_matmult(), "ragged_matmult": run_ragged_matmult()}))
         return
+    if args.only == "lists":
+        print(json.dumps({"ragged_matmult": run_ragged_matmult()}))
+        return
     if args.only in ("mm16", "mm8"):
         C.set_matmult_precision("split16" if args.only == "mm16" else "fp8")
-        a = C.random_normal(8192, 8192, dtype=C.L.F, seed=21)
-        bm = C.random_normal(8192, 8192, dtype=C.L.F, seed=22)
+        C.srandom(21)
+        a = normal_f32(C, 8192, 8192)
+        bm = normal_f32(C, 8192, 8192)
         cm = C.ndarray.new_dense(C.L.F, [8192, 8192])
         for _ in range(args.steps):
             C._run("matmult", [a, bm, cm])
@@ -504,12 +528,14 @@ def run_single(args):
         print(json.dumps({"only": args.only, "steps": args.steps}))
         return
     if args.only == "copy":
-        src = C.random_uniform(256, 1024, 1024, dtype=C.L.F, seed=1)      # 1 GiB
+        C.srandom(1)
+        src = C.random(256, 1024, 1024, dtype=C.L.F)                      # 1 GiB
         dst = C.ndarray.new_dense(C.L.F, src.dims)
         f = lambda: C._run("assgn", [src, dst])
     else:
-        x = C.random_normal(4096, 4096, 16, dtype=C.L.F, seed=1)
-        y = C.random_normal(4096, 16, dtype=C.L.F, seed=2)
+        C.srandom(1)
+        x = normal_f32(C, 4096, 4096, 16)
+        y = normal_f32(C, 4096, 16)
         z = C.ndarray.new_dense(C.L.F, [4096, 4096, 16])
         if args.only == "binop":
             xv, yv = x.xchg(0, 1), y.dummy(1)
@@ -551,11 +577,18 @@ def run_cpu_baseline(frac_k=2, min_seconds=10.0, min_runs=5):
     from oracle import pdlcpu as O
 
     nthr, model = host_cores()
-    rng = np.random.default_rng(3)
     NX, NY, NK = 4096, 4096, frac_k
-    x = rng.standard_normal((NK, NY, NX), dtype=np.float32)
-    y = rng.standard_normal((NK, NY), dtype=np.float32)
-    i32h, i16h = make_int_inputs(4096, 1024, 8192, 2048)
+    # the same expressions as the device inputs, from the oracle's stream
+    u = O.random_from(NX * NY * NK + NY * NK, 1)
+    g = O.ndtri(u).astype(np.float32)
+    x = g[:NX * NY * NK].reshape(NK, NY, NX)
+    y = g[NX * NY * NK:].reshape(NK, NY)
+    r = O.random_from(4096 * 1024 + 2 * 8192 * 2048, 2)
+    i32h = ((r[:4096 * 1024] * 2097153).astype(np.int32) - 1048576).reshape(1024, 4096)
+    v = ((r[4096 * 1024:4096 * 1024 + 8192 * 2048] * 65535).astype(np.int32) - 32767).astype(np.int16)
+    v[r[4096 * 1024 + 8192 * 2048:] < 0.01] = -32768
+    i16h = v.reshape(2048, 8192)
+    i16h[17] = -32768
     px = O.OPdl.from_numpy(x, O.F).xchg(0, 1)
     py = O.OPdl.from_numpy(y, O.F).dummy(1)
     p32 = O.OPdl.from_numpy(i32h, O.L)

@@ -1546,6 +1546,88 @@ void pdlc_random_fill(double *out, int64_t n, uint64_t seed) {
   for (int64_t i = 0; i < n; i++) out[i] = pdlc_drand(s);
 }
 
+/* draws [skip, skip + n) of the single stream seeded with `seed` */
+void pdlc_random_fill_from(double *out, int64_t n, uint64_t seed, uint64_t skip) {
+  uint64_t s[4];
+  pdlc_srand(s, seed, 1);
+  for (uint64_t i = 0; i < skip; i++) (void)pdlc_xoshiro256plus_next(s);
+  for (int64_t i = 0; i < n; i++) out[i] = pdlc_drand(s);
+}
+
+/* the state after `skip` draws of the stream seeded with `seed` */
+void pdlc_rand_state_after(uint64_t *out, uint64_t seed, uint64_t skip) {
+  pdlc_srand(out, seed, 1);
+  for (uint64_t i = 0; i < skip; i++) (void)pdlc_xoshiro256plus_next(out);
+}
+
+/* ------------------------------------------------------------------ */
+/* ndtri: PDL::Math's inverse of the normal distribution function      */
+/* (Math.pd:346-356 -> Math/ndtri.c, Cephes 2.1, S. L. Moshier), which  */
+/* grandom applies to randsym draws (Primitive.pd:2256-2261).  Cephes'  */
+/* published rational approximations: around the middle                */
+/* x = w + w^3 P0(w^2)/Q0(w^2), w = y - 1/2, times sqrt(2 pi); in the    */
+/* tails z = sqrt(-2 log y), x = z - log(z)/z - P(1/z)/(z Q(1/z)) with   */
+/* (P1, Q1) for z < 8 and (P2, Q2) beyond; Q tables without their       */
+/* leading 1.  polevl/p1evl are Horner's rule from the leading          */
+/* coefficient (p1evl with an implicit leading 1).                      */
+/* ------------------------------------------------------------------ */
+static const double ND_P0[5] = {-5.99633501014107895267E1, 9.80010754185999661536E1, -5.66762857469070293439E1,
+                                1.39312609387279679503E1, -1.23916583867381258016E0};
+static const double ND_Q0[8] = {1.95448858338141759834E0, 4.67627912898881538453E0, 8.63602421390890590575E1,
+                                -2.25462687854119370527E2, 2.00260212380060660359E2, -8.20372256168333339912E1,
+                                1.59056225126211695515E1, -1.18331621121330003142E0};
+static const double ND_P1[9] = {4.05544892305962419923E0, 3.15251094599893866154E1, 5.71628192246421288162E1,
+                                4.40805073893200834700E1, 1.46849561928858024014E1, 2.18663306850790267539E0,
+                                -1.40256079171354495875E-1, -3.50424626827848203418E-2, -8.57456785154685413611E-4};
+static const double ND_Q1[8] = {1.57799883256466749731E1, 4.53907635128879210584E1, 4.13172038254672030440E1,
+                                1.50425385692907503408E1, 2.50464946208309415979E0, -1.42182922854787788574E-1,
+                                -3.80806407691578277194E-2, -9.33259480895457427372E-4};
+static const double ND_P2[9] = {3.23774891776946035970E0, 6.91522889068984211695E0, 3.93881025292474443415E0,
+                                1.33303460815807542389E0, 2.01485389549179081538E-1, 1.23716634817820021358E-2,
+                                3.01581553508235416007E-4, 2.65806974686737550832E-6, 6.23974539184983293730E-9};
+static const double ND_Q2[8] = {6.02427039364742014255E0, 3.67983563856160859403E0, 1.37702099489081330271E0,
+                                2.16236993594496635890E-1, 1.34204006088543189037E-2, 3.28014464682127739104E-4,
+                                2.89247864745380683936E-6, 6.79019408009981274425E-9};
+
+static double nd_polevl(double x, const double *c, int n) {
+  double r = c[0];
+  for (int i = 1; i <= n; i++) r = r * x + c[i];
+  return r;
+}
+static double nd_p1evl(double x, const double *c, int n) {
+  double r = x + c[0];
+  for (int i = 1; i < n; i++) r = r * x + c[i];
+  return r;
+}
+
+double pdlc_ndtri(double y0) {
+  const double expm2 = 0.13533528323661269189, s2pi = 2.50662827463100050242E0, maxnum = 1.79769313486231570815E308;
+  if (y0 <= 0.0) return -maxnum;
+  if (y0 >= 1.0) return maxnum;
+  int neg = 1;
+  double y = y0;
+  if (y > 1.0 - expm2) {
+    y = 1.0 - y;
+    neg = 0;
+  }
+  if (y > expm2) {
+    y = y - 0.5;
+    double y2 = y * y;
+    double x = y + y * (y2 * nd_polevl(y2, ND_P0, 4) / nd_p1evl(y2, ND_Q0, 8));
+    return x * s2pi;
+  }
+  double x = sqrt(-2.0 * log(y));
+  double x0 = x - log(x) / x;
+  double z = 1.0 / x;
+  double x1 = x < 8.0 ? z * nd_polevl(z, ND_P1, 8) / nd_p1evl(z, ND_Q1, 8) : z * nd_polevl(z, ND_P2, 8) / nd_p1evl(z, ND_Q2, 8);
+  x = x0 - x1;
+  return neg ? -x : x;
+}
+
+void pdlc_ndtri_n(const double *y, double *x, int64_t n) {
+  for (int64_t i = 0; i < n; i++) x[i] = pdlc_ndtri(y[i]);
+}
+
 /* ------------------------------------------------------------------ */
 /* Truth helper: row-major C[h][w] = sum_t A[h][t] B[t][w] in long     */
 /* double (64-bit significand) for scoring double results.             */

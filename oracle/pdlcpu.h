@@ -125,6 +125,10 @@ double pdlc_drand(uint64_t *s);
 /* n draws of pdl_drand from one stream seeded with `seed` (PDL random with
  * autopthread off). */
 void pdlc_random_fill(double *out, int64_t n, uint64_t seed);
+void pdlc_random_fill_from(double *out, int64_t n, uint64_t seed, uint64_t skip);
+void pdlc_rand_state_after(uint64_t *out, uint64_t seed, uint64_t skip);
+double pdlc_ndtri(double y);
+void pdlc_ndtri_n(const double *y, double *x, int64_t n);
 
 /* FP64 / extended-precision truth helpers (not PDL functions: the "PDL
  * computing in double" references the float results are scored against). */

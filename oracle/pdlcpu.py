@@ -73,6 +73,11 @@ def lib():
                                             ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_int)]
         l.pdlc_random_fill.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64]
+        l.pdlc_random_fill_from.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint64]
+        l.pdlc_rand_state_after.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64]
+        l.pdlc_ndtri.argtypes = [ctypes.c_double]
+        l.pdlc_ndtri.restype = ctypes.c_double
+        l.pdlc_ndtri_n.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
         l.pdlc_xoshiro256plus_next.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
         l.pdlc_xoshiro256plus_next.restype = ctypes.c_uint64
         l.pdlc_splitmix64_next.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
@@ -199,6 +204,9 @@ class OPdl:
                 a, b, s = sp
                 a = a if a >= 0 else self.dims[d] + a
                 b = b if b >= 0 else self.dims[d] + b
+                # Slices.pd:2367-2386 bounds
+                if not (0 <= a < self.dims[d] and 0 <= b < self.dims[d]):
+                    raise OracleError("slice out of bounds in pos %d" % d)
                 n = (b - a) // s + 1
                 dims.append(max(n, 0))
                 incs.append(self.incs[d] * s)
@@ -323,6 +331,37 @@ def random_uniform(n, seed):
     out = np.empty(n, dtype=np.float64)
     lib().pdlc_random_fill(out.ctypes.data, n, seed)
     return out
+
+
+def random_from(n, seed, skip=0):
+    """draws [skip, skip + n) of the single stream seeded with `seed`"""
+    out = np.empty(n, dtype=np.float64)
+    lib().pdlc_random_fill_from(out.ctypes.data, n, seed, skip)
+    return out
+
+
+def rand_state_after(seed, skip):
+    """the xoshiro256+ state after `skip` draws of the stream seeded with `seed`"""
+    out = np.empty(4, dtype=np.uint64)
+    lib().pdlc_rand_state_after(out.ctypes.data, seed, skip)
+    return out
+
+
+def ndtri(y):
+    """PDL::Math::ndtri (Cephes) of a float64 array"""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    x = np.empty_like(y)
+    lib().pdlc_ndtri_n(y.ctypes.data, x.ctypes.data, y.size)
+    return x
+
+
+def grandom(dtype, n, seed, skip=0):
+    """PDL's grandom(type, n) after srandom(seed) and `skip` earlier draws:
+    ndtri of the randsym draws rounded to the type, rounded back
+    (Primitive.pd:2256-2261; Math.pd:346-356)"""
+    u = random_from(n, seed, skip)
+    assert not np.any(u == 0.0)          # randsym would draw again
+    return ndtri(u.astype(dtype).astype(np.float64)).astype(dtype)
 
 
 def truth_matmult_ld(a, b):

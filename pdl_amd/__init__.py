@@ -7,7 +7,7 @@ HIP kernels for gfx950.
 """
 from ._lib import PDLError  # noqa: F401
 from .core import (TYPE_NAMES, Chain, Event, Graph, MatmultList, Ragged, Timer, matmult_list, view_into, default_badvalue, empty, init, matmult, ndarray, null, ones,  # noqa: F401
-                   pdl, random_normal, random_uniform, run, sequence, set_matmult_precision, synchronize,
+                   grandom, pdl, random, randsym, run, sequence, srandom, set_matmult_precision, synchronize,
                    topdl, typecode, zeroes)
 from . import core as _core
 

@@ -107,6 +107,13 @@ BADOPS = ["isbad", "isgood", "setnantobad", "setinftobad", "setnonfinitetobad", 
 for _n in UFUNCS + BADOPS:
     EXPORTS["pdlhip_run_" + _n] = ([PP, PP], Error)
 EXPORTS["pdlhip_run_setbadif"] = ([PP, PP, PP], Error)
+EXPORTS["pdlhip_tune_stats"] = ([ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)], Error)
+# PDL's random-number stream
+EXPORTS["pdlhip_srandom"] = ([ctypes.c_uint64], Error)
+EXPORTS["pdlhip_rand_skip"] = ([ctypes.c_uint64], Error)
+EXPORTS["pdlhip_rand_state"] = ([ctypes.POINTER(ctypes.c_uint64)], Error)
+for _n in ("pdlhip_run_random", "pdlhip_run_randsym", "pdlhip_grandom"):
+    EXPORTS[_n] = ([PP], Error)
 EXPORTS["pdlhip_run_setvaltobad"] = ([PP, PP, ctypes.c_double], Error)
 EXPORTS["pdlhip_run_setbadtoval"] = ([PP, PP, ctypes.c_double], Error)
 EXPORTS["pdlhip_ragged_prepare"] = ([ctypes.c_int, ctypes.c_int, ctypes.POINTER(PP), ctypes.c_int,

@@ -34,8 +34,10 @@ def units():
         u.append(("k_elementwise.hip", "ew_t%d" % t, ["-DPDLHIP_EW_TYPES_IN_THIS_TU=PDLHIP_EW_TYPE(%d)" % t]))
         u.append(("k_reduce.hip", "red_t%d" % t, ["-DPDLHIP_RED_TYPES_IN_THIS_TU=PDLHIP_RED_TYPE(%d)" % t]))
         u.append(("k_matmult.hip", "mm_t%d" % t, ["-DPDLHIP_MM_TYPES_IN_THIS_TU=PDLHIP_MM_TYPE(%d)" % t]))
+        u.append(("k_mixed.hip", "mx_t%d" % t, ["-DPDLHIP_MIXED_TYPES_IN_THIS_TU=PDLHIP_MIXED_TYPE(%d)" % t]))
     u.append(("k_misc.hip", "misc", []))
     u.append(("k_badops.hip", "badops", []))
+    u.append(("k_rng.hip", "rng", []))
     u.append(("k_mfma.hip", "mfma", []))
     u.append(("k_batch.hip", "batch", []))
     u.append(("k_ragged.hip", "ragged", []))

@@ -245,13 +245,23 @@ class ndarray:
             elif isinstance(sp, (int, np.integer)):
                 k = int(sp) if sp >= 0 else self.dims[d] + int(sp)
                 if not 0 <= k < self.dims[d]:
-                    raise PDLError(L.EUSERERROR, "slice: index %d out of range for dim %d" % (sp, d))
+                    raise PDLError(L.EUSERERROR, "slice starts out of bounds in pos %d (start is %d; end is %d; "
+                                   "inc is 0; source dim %d runs 0 to %d)" % (d, k, k, d, self.dims[d] - 1))
                 offs += k * self.incs[d]
             else:
+                # bounds and regularisation of Slices.pd:2367-2393
                 a, b, s = sp
-                a = a if a >= 0 else self.dims[d] + a
-                b = b if b >= 0 else self.dims[d] + b
-                n = (b - a) // s + 1
+                size = self.dims[d]
+                a = a if a >= 0 else size + a
+                if not 0 <= a < size:
+                    raise PDLError(L.EUSERERROR, "slice starts out of bounds in pos %d (start is %d; end is %d; "
+                                   "inc is %d; source dim %d runs 0 to %d)" % (d, a, b, s, d, size - 1))
+                b = b if b >= 0 else size + b
+                if not 0 <= b < size:
+                    raise PDLError(L.EUSERERROR, "slice ends out of bounds in pos %d (end is %d; source dim %d "
+                                   "runs 0 to %d)" % (d, b, d, size - 1))
+                s = s if s else (1 if a <= b else -1)
+                n = (b - a + s) // s
                 dims.append(max(n, 0))
                 incs.append(self.incs[d] * s)
                 offs += a * self.incs[d]
@@ -410,6 +420,11 @@ class ndarray:
         _run(op, [self, out])
         return out
 
+    # random / randsym / grandom of an ndarray: a new one of its dims and type
+    def random(self): return random(*self.dims, dtype=self.dtype)
+    def randsym(self): return randsym(*self.dims, dtype=self.dtype)
+    def grandom(self): return grandom(*self.dims, dtype=self.dtype)
+
     def sumover(self, out=None): return self._red("sumover", out)
     def prodover(self, out=None): return self._red("prodover", out)
     def dsumover(self, out=None): return self._red("dsumover", out)
@@ -566,19 +581,72 @@ def sequence(*dims, dtype=L.D):
     return ndarray.from_numpy(np.arange(n).reshape(dims[::-1]), dtype)
 
 
-def random_uniform(*dims, dtype=L.D, seed=0, lo=0.0, hi=1.0):
-    """counter-based uniform[lo,hi) generated on the device"""
-    p = ndarray.new_dense(dtype, dims)
-    c = p._to_c()
-    L.check(L.lib().pdlhip_fill_uniform(ctypes.byref(c), seed, lo, hi))
-    return p
+# ---------------------------------------------------------------- PDL's random numbers
+def srandom(seed):
+    """PDL::srandom (Primitive.pd:2113-2140): seed the stream"""
+    _need()
+    L.check(L.lib().pdlhip_srandom(int(seed) & (2**64 - 1)))
 
 
-def random_normal(*dims, dtype=L.F, seed=0, mean=0.0, sd=1.0):
-    p = ndarray.new_dense(dtype, dims)
-    c = p._to_c()
-    L.check(L.lib().pdlhip_fill_normal(ctypes.byref(c), seed, mean, sd))
-    return p
+def rand_skip(n):
+    """move the stream on by n draws, as n elements of random would"""
+    _need()
+    L.check(L.lib().pdlhip_rand_skip(int(n)))
+
+
+def rand_state():
+    """the xoshiro256+ state the next draw starts from (4 words)"""
+    _need()
+    st = (ctypes.c_uint64 * 4)()
+    L.check(L.lib().pdlhip_rand_state(st))
+    return [int(v) for v in st]
+
+
+def _rand_into(x, entry):
+    c = x._to_c()
+    L.check(getattr(L.lib(), entry)(ctypes.byref(c)))
+    return x
+
+
+def _rand_new(entry, dims, dtype):
+    _need()
+    if len(dims) == 1 and isinstance(dims[0], (list, tuple)):
+        dims = tuple(dims[0])
+    return _rand_into(ndarray.new_dense(L.D if dtype is None else dtype, dims), entry)
+
+
+def random(*dims, dtype=None):
+    """PDL::random([type], dims...) (Primitive.pd:2142-2179): uniform [0,1)
+    draws of PDL's stream, one per element in PDL order; double by default"""
+    return _rand_new("pdlhip_run_random", dims, dtype)
+
+
+def randsym(*dims, dtype=None):
+    """PDL::randsym (Primitive.pd:2181-2219): the same draws, 0 excluded"""
+    return _rand_new("pdlhip_run_randsym", dims, dtype)
+
+
+def grandom(*dims, dtype=None):
+    """PDL::grandom (Primitive.pd:2221-2262): ndtri(randsym) in the type
+    (float or double)"""
+    return _rand_new("pdlhip_grandom", dims, dtype)
+
+
+def random_into(x):
+    """pdl_run_random on an existing ndarray (a view fills through its
+    dims/incs, the parent's other elements untouched)"""
+    _need()
+    return _rand_into(x, "pdlhip_run_random")
+
+
+def randsym_into(x):
+    _need()
+    return _rand_into(x, "pdlhip_run_randsym")
+
+
+def grandom_into(x):
+    _need()
+    return _rand_into(x, "pdlhip_grandom")
 
 
 def _run(op, pdls, swap=0, m_size=None, dpar=None):
@@ -786,6 +854,15 @@ class Chain(_Prepared):
         if bad and self.result is not None:
             self.result.state |= L.BADVAL
         return bad
+
+
+def tune_stats():
+    """{'measured': n, 'reused': m}: autotuner choices this process timed and
+    found in the cache file"""
+    _need()
+    m, r = ctypes.c_int64(), ctypes.c_int64()
+    L.check(L.lib().pdlhip_tune_stats(ctypes.byref(m), ctypes.byref(r)))
+    return {"measured": m.value, "reused": r.value}
 
 
 class Graph:

@@ -21,12 +21,16 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <ctime>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <queue>
 #include <string>
 #include <tuple>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include "../../include/pdlhip.h"
 #include "common.hpp"
@@ -97,6 +101,10 @@ struct Global {
   // them is destroyed
   bool capturing = false;
   std::vector<void *> pinned;
+  // PDL's random-number stream (one state: PDL with autopthread off)
+  uint64_t rng[4] = {0, 0, 0, 0};
+  bool rng_seeded = false;
+  uint64_t *rng_jumps = nullptr;     // the in-chunk jump matrices, on the device
 };
 static Global g;
 
@@ -651,6 +659,10 @@ static bool fill_ew(EwPlan &e, Dimset &d) {
   return true;
 }
 
+static int tune(const std::string &family, const std::vector<int> &cands, int dflt,
+                const std::function<hipError_t(int)> &run_one);
+static int ilog2(int64_t v);
+
 // a fill kernel rather than hipMemsetAsync, so the clear is an ordinary
 // kernel node wherever the work is captured
 __global__ void zero_i32(int *p, int n) {
@@ -665,7 +677,9 @@ static pdlhip_error readdata(const Sig &s, int T, bool bad, const Brc &b, pdlhip
                                           s.name.c_str(), type_name(T));
   hipStream_t st = g.stream;
   hipError_t e = hipSuccess;
-  if (s.family == FAM_BIOP || s.family == FAM_BIFUNC) {
+  const bool mixed = (s.family == FAM_BIOP || s.family == FAM_BIFUNC || s.family == FAM_UFUNC) &&
+                     (pdls[0]->datatype != T || (s.family != FAM_UFUNC && pdls[1]->datatype != T));
+  if (!mixed && (s.family == FAM_BIOP || s.family == FAM_BIFUNC)) {
     int which[3] = {0, 1, 2};
     Dimset d = brc_dimset(b, which, 3);
     if (!canonicalise(d, 2)) return ok();
@@ -676,6 +690,37 @@ static pdlhip_error readdata(const Sig &s, int T, bool bad, const Brc &b, pdlhip
     for (int k = 0; k < 3; k++)
       bi[k] = badinfo(pdls[k], s.family == FAM_BIFUNC ? 1 : (pdls[k]->state & PDLHIP_BADVAL) != 0);
     e = binop_dispatch(T, s.op, p, pdls[0]->data, pdls[1]->data, pdls[2]->data, bad, bi, st);
+  } else if (mixed) {
+    // inputs of other types, converted in registers as their converted
+    // copies would hold them (k_mixed.hip)
+    const bool uf = s.family == FAM_UFUNC;
+    int which[3] = {0, uf ? 0 : 1, uf ? 1 : 2};
+    Dimset d = brc_dimset(b, which, 3);
+    if (uf)
+      for (int k = 0; k < d.n; k++) d.inc[1][k] = 0;
+    if (!canonicalise(d, 2)) return ok();
+    EwPlan p;
+    if (!fill_ew(p, d)) return make_err(PDLHIP_EFATAL, "pdlhip: %s: too many distinct dims", s.name.c_str());
+    p.off[0] = b.offs[0];
+    p.off[1] = uf ? b.offs[0] : b.offs[1];
+    p.off[2] = b.offs[uf ? 1 : 2];
+    const pdlhip_pdl *pa = pdls[0], *pb = uf ? pdls[0] : pdls[1];
+    MixedArgs m;
+    std::memset(&m, 0, sizeof m);
+    m.a = pa->data; m.ta = pa->datatype; m.sbad_a = (pa->state & PDLHIP_BADVAL) != 0; m.sba = badinfo(pa, 1);
+    m.b = pb->data; m.tb = pb->datatype; m.sbad_b = (pb->state & PDLHIP_BADVAL) != 0; m.sbb = badinfo(pb, 1);
+    // the op's bad test on the converted values: the copies' bad value is
+    // the default of T; a biop tests an input only if it is bad-flagged
+    BadInfo bi[3];
+    for (int k = 0; k < 2; k++) {
+      const pdlhip_pdl *src = k == 0 ? pa : pb;
+      std::memset(&bi[k], 0, sizeof(BadInfo));
+      bi[k].bv = default_badval(T);
+      bi[k].isnan = 0;
+      bi[k].check = uf ? (k == 0) : (s.family == FAM_BIFUNC ? 1 : (src->state & PDLHIP_BADVAL) != 0);
+    }
+    bi[2] = badinfo(pdls[uf ? 1 : 2], 1);
+    e = launch_mixed(T, s.op, m, p, pdls[uf ? 1 : 2]->data, bad, bi, st);
   } else if (s.family == FAM_UFUNC) {
     // the binop kernels with the second operand a zero-stride view of a's
     // first element, which the ufunc ignores (and the bad test skips)
@@ -846,6 +891,33 @@ static pdlhip_error readdata(const Sig &s, int T, bool bad, const Brc &b, pdlhip
       return ok();
     }
     if (g.defer) g.defer->missed++;
+    // the autotuner's block shape for float products on the matrix cores
+    // (128x128 or 256x256) and engine for mid-size double products (SIMT 1
+    // or FP64 matrix cores 2), timed into a scratch output
+    const bool mm16 = T == PDLHIP_F && (g.mm_mode == PDLHIP_MM_SPLIT16 || g.mm_mode == PDLHIP_MM_FP8);
+    const int64_t nb128 = ((p.H + 127) / 128) * ((p.W + 127) / 128);
+    const bool f64mid = T == PDLHIP_D && g.mm_mode != PDLHIP_MM_PDLORDER && g.mm_mode != PDLHIP_MM_FP32 &&
+                        p.nbatch == 1 && p.H >= 64 && p.W >= 64 && p.T >= 16 && nb128 >= 16 && nb128 < 256;
+    if (!bad && p.nbatch == 1 && ((mm16 && p.H >= 512 && p.W >= 512 && p.T >= 32) || f64mid)) {
+      char fam[160];
+      snprintf(fam, sizeof fam, "matmult:%s:%s:h2^%d:w2^%d:t2^%d", type_name(T),
+               g.mm_mode == PDLHIP_MM_FP8 ? "fp8" : "default", ilog2(p.H), ilog2(p.W), ilog2(p.T));
+      std::vector<int> cands = mm16 ? std::vector<int>{128, 256} : std::vector<int>{1, 2};
+      const int dflt = mm16 ? ((p.H >= 1024 && p.W >= 1024) ? 256 : 128) : (nb128 >= 64 ? 2 : 1);
+      void *tc = nullptr;
+      auto run_one = [&](int cand) -> hipError_t {
+        if (!tc) tc = dev_alloc((size_t)p.H * p.W * type_size(T));
+        if (!tc) return hipErrorOutOfMemory;
+        MmPlan q = p;
+        q.variant = cand;
+        q.off[2] = 0;
+        q.c_w = 1;
+        q.c_h = p.W;
+        return matmult_dispatch(T, q, pdls[0]->data, pdls[1]->data, tc, false, bi, g.mm_mode, st);
+      };
+      p.variant = tune(fam, cands, dflt, run_one);
+      if (tc) dev_free(tc);
+    }
     e = matmult_dispatch(T, p, pdls[0]->data, pdls[1]->data, pdls[2]->data, bad, bi, g.mm_mode, st);
   }
   if (e != hipSuccess) return hip_err(e, s.name.c_str());
@@ -959,12 +1031,20 @@ static pdlhip_error run(const Sig &s, pdlhip_pdl **user, int npdls, const int64_
   pdlhip_pdl *orig[MAXP];
   for (int i = 0; i < npdls; i++) orig[i] = pdls[i];
   pdlhip_error err = ok();
+  // elementwise ops read an input of another type directly and convert it
+  // in registers (k_mixed.hip): no copy, unless it is also an output
+  const bool ew = s.family == FAM_BIOP || s.family == FAM_BIFUNC || s.family == FAM_UFUNC;
   for (int i = npdls - 1; i >= 0 && !err.error; i--) {
     if (creating[i] || pdls[i]->datatype == req[i]) continue;
     if (!type_on_device(pdls[i]->datatype)) {
       err = make_err(PDLHIP_EUSERERROR, "pdlhip: %s: type %s is not supported on the device", s.name.c_str(),
                      type_name(pdls[i]->datatype));
       break;
+    }
+    if (ew && i < s.nparents) {
+      bool inout = false;
+      for (int j = s.nparents; j < npdls; j++) inout = inout || orig[j] == orig[i];
+      if (!inout) continue;
     }
     int dup = -1;
     for (int j = i + 1; j < npdls; j++)
@@ -1022,16 +1102,99 @@ static pdlhip_error run(const Sig &s, pdlhip_pdl **user, int npdls, const int64_
 }
 
 // ------------------------------------------------------------------ autotuner
-// Tile shape of the list kernel per shape class: (type, fused, log2 of the
-// largest dim, log2 of the item count).  On a miss every candidate runs once
-// under events and the fastest is kept for the process.
-struct TuneKey {
-  int T, fused, lgdim, lgcount;
-  bool operator<(const TuneKey &o) const {
-    return std::tie(T, fused, lgdim, lgcount) < std::tie(o.T, o.fused, o.lgdim, o.lgcount);
-  }
+// Measured choices of kernel variants per shape class, kept in a text file
+// per GPU architecture so that later processes reuse them: one line per
+// entry, "<key> <choice>", the key naming the kernel family, element type,
+// shape class and architecture (e.g. "list:float:mfma1:dim2^8:items2^12:gfx950").
+// The file is $PDLHIP_TUNE_FILE, else $XDG_CACHE_HOME/pdlhip/tune-<arch>.txt,
+// else $HOME/.cache/pdlhip/tune-<arch>.txt; PDLHIP_TUNE=0 turns timing off
+// (each family's default rule applies), PDLHIP_TUNE_VERBOSE=1 prints each
+// candidate's time to stderr.  A miss times every candidate with
+// HIP events on scratch outputs (never the caller's), keeps the fastest and
+// appends it to the file.  Not during a graph capture.
+struct Tuner {
+  std::mutex mu;
+  bool loaded = false, enabled = true, verbose = false;
+  std::string arch, path;
+  std::map<std::string, int> choice;
+  int64_t measured = 0, reused = 0;
 };
-static std::map<TuneKey, int> g_tune;
+static Tuner g_tuner;
+
+static void mkdirs_for(const std::string &file) {
+  for (size_t i = 1; i < file.size(); i++)
+    if (file[i] == '/') (void)mkdir(file.substr(0, i).c_str(), 0755);
+}
+
+static void tuner_load() {
+  if (g_tuner.loaded) return;
+  g_tuner.loaded = true;
+  hipDeviceProp_t prop;
+  g_tuner.arch = "gfx";
+  if (g.device >= 0 && hipGetDeviceProperties(&prop, g.device) == hipSuccess) {
+    g_tuner.arch = prop.gcnArchName;
+    const size_t colon = g_tuner.arch.find(':');
+    if (colon != std::string::npos) g_tuner.arch.resize(colon);
+  }
+  const char *off = getenv("PDLHIP_TUNE");
+  g_tuner.enabled = !(off && off[0] == '0');
+  const char *verbose = getenv("PDLHIP_TUNE_VERBOSE");
+  g_tuner.verbose = verbose && verbose[0] && verbose[0] != '0';
+  const char *f = getenv("PDLHIP_TUNE_FILE");
+  if (f && *f) {
+    g_tuner.path = f;
+  } else {
+    const char *xdg = getenv("XDG_CACHE_HOME"), *home = getenv("HOME");
+    std::string dir = (xdg && *xdg) ? std::string(xdg) : (home && *home) ? std::string(home) + "/.cache" : "";
+    if (!dir.empty()) g_tuner.path = dir + "/pdlhip/tune-" + g_tuner.arch + ".txt";
+  }
+  if (g_tuner.path.empty()) return;
+  FILE *fp = fopen(g_tuner.path.c_str(), "r");
+  if (!fp) return;
+  char key[512];
+  int v;
+  while (fscanf(fp, "%511s %d", key, &v) == 2) g_tuner.choice[key] = v;
+  fclose(fp);
+}
+
+// the variant for `key` among `cands`: cached, else timed with `run_one`
+// (which enqueues one run of a candidate; it is run twice, the second timed)
+static int tune(const std::string &family, const std::vector<int> &cands, int dflt,
+                const std::function<hipError_t(int)> &run_one) {
+  std::lock_guard<std::mutex> lk(g_tuner.mu);
+  tuner_load();
+  const std::string key = family + ":" + g_tuner.arch;
+  auto it = g_tuner.choice.find(key);
+  if (it != g_tuner.choice.end() && std::find(cands.begin(), cands.end(), it->second) != cands.end()) {
+    g_tuner.reused++;
+    return it->second;
+  }
+  if (!g_tuner.enabled || g.plan_only || capturing() || cands.size() < 2) return dflt;
+  int best = dflt;
+  float best_ms = 1e30f;
+  for (int c : cands) {
+    if (run_one(c) != hipSuccess) continue;   // warm-up
+    (void)hipEventRecord(g.ev0, g.stream);
+    hipError_t e = run_one(c);
+    (void)hipEventRecord(g.ev1, g.stream);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventSynchronize(g.ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, g.ev0, g.ev1);
+    if (e == hipSuccess && ms < best_ms) { best_ms = ms; best = c; }
+    if (g_tuner.verbose) fprintf(stderr, "pdlhip tune %s: candidate %d %.4f ms\n", key.c_str(), c, e == hipSuccess ? ms : -1.0f);
+  }
+  (void)hipGetLastError();
+  g_tuner.measured++;
+  g_tuner.choice[key] = best;
+  if (!g_tuner.path.empty()) {
+    mkdirs_for(g_tuner.path);
+    if (FILE *fp = fopen(g_tuner.path.c_str(), "a")) {
+      fprintf(fp, "%s %d\n", key.c_str(), best);
+      fclose(fp);
+    }
+  }
+  return best;
+}
 
 static int ilog2(int64_t v) {
   int r = 0;
@@ -1127,26 +1290,43 @@ static pdlhip_error list_build(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlhip_pdl
   hipError_t e = upload_now(h.d_items, h.d.items.data(), sizeof(MmItem) * nit);
   if (e == hipSuccess) e = upload_now(h.d_w32, h.work32.data(), sizeof(int3) * h.work32.size());
   if (e == hipSuccess) e = upload_now(h.d_w64, h.work64.data(), sizeof(int3) * h.work64.size());
-  // tile by the autotuner: the candidates timed once per shape class
-  TuneKey key{h.d.T, (h.d.fused ? 1 : 0) + (h.mfma ? 2 : 0), ilog2(maxdim), ilog2(nit)};
-  auto it = g_tune.find(key);
-  if (it != g_tune.end()) {
-    h.tile = it->second;
-  } else if (e == hipSuccess) {
-    float best = 1e30f;
+  // tile by the autotuner: 32 / 64 SIMT tiles, the FP32/FP64 matrix-core
+  // kernel (0) and, for float, the BF16 three-way split (1); timed on a copy
+  // of the item table whose outputs all point into one scratch block
+  if (e == hipSuccess) {
     std::vector<int> cands = {32, 64};
     if (h.mfma) cands.push_back(0);
-    for (int cand : cands) {
-      (void)hipEventRecord(g.ev0, g.stream);
-      e = list_launch(h, cand, false);
-      (void)hipEventRecord(g.ev1, g.stream);
-      if (e != hipSuccess) break;
-      float ms = 0;
-      (void)hipEventSynchronize(g.ev1);
-      (void)hipEventElapsedTime(&ms, g.ev0, g.ev1);
-      if (ms < best) { best = ms; h.tile = cand; }
+    if (h.mfma && h.d.T == PDLHIP_F) cands.push_back(1);
+    const int dflt = h.mfma ? 0 : 64;
+    char fam[160];
+    snprintf(fam, sizeof fam, "list:%s:mfma%d:fused%d:dim2^%d:items2^%d", type_name(h.d.T), h.mfma ? 1 : 0,
+             h.d.fused ? 1 : 0, ilog2(maxdim), ilog2(nit));
+    MmItem *t_items = nullptr;
+    void *t_c = nullptr;
+    auto run_one = [&](int cand) -> hipError_t {
+      if (!t_items) {
+        int64_t most = 1;
+        for (auto &it : h.d.items) most = std::max(most, it.H * it.W);
+        std::vector<MmItem> tv = h.d.items;
+        t_c = dev_alloc((size_t)most * type_size(h.d.T));
+        t_items = (MmItem *)dev_alloc(sizeof(MmItem) * nit);
+        if (!t_c || !t_items) return hipErrorOutOfMemory;
+        for (auto &it : tv) { it.c = t_c; it.oc = 0; it.c_w = 1; it.c_h = it.W; }
+        hipError_t ue = upload_now(t_items, tv.data(), sizeof(MmItem) * nit);
+        if (ue != hipSuccess) return ue;
+      }
+      ListH t = h;
+      t.d_items = t_items;
+      hipError_t le = list_launch(t, cand, false);
+      t.d_items = nullptr;
+      return le;
+    };
+    h.tile = tune(fam, cands, dflt, run_one);
+    if (t_items || t_c) {
+      (void)hipStreamSynchronize(g.stream);
+      dev_free(t_items);
+      dev_free(t_c);
     }
-    if (e == hipSuccess) g_tune[key] = h.tile;
   }
   if (e != hipSuccess) {
     list_release(h);
@@ -1586,6 +1766,8 @@ pdlhip_error pdlhip_shutdown(void) {
     std::lock_guard<std::mutex> lc(g_cache.mu);
     release_cached();
   }
+  if (g.rng_jumps) (void)hipFree(g.rng_jumps);
+  g.rng_jumps = nullptr;
   (void)hipEventDestroy(g.ev0);
   (void)hipEventDestroy(g.ev1);
   (void)hipStreamDestroy(g.stream);
@@ -1714,6 +1896,14 @@ pdlhip_error pdlhip_run_maximum_n_ind(pdlhip_pdl *a, pdlhip_pdl *c, int64_t m_si
 pdlhip_error pdlhip_run_matmult(pdlhip_pdl *a, pdlhip_pdl *b, pdlhip_pdl *c) {
   pdlhip_pdl *v[3] = {a, b, c};
   return pdlhip_run("matmult", v, 3, nullptr);
+}
+
+pdlhip_error pdlhip_tune_stats(int64_t *measured, int64_t *reused) {
+  if (!measured || !reused) return make_err(PDLHIP_EUSERERROR, "pdlhip_tune_stats: NULL argument");
+  std::lock_guard<std::mutex> lk(g_tuner.mu);
+  *measured = g_tuner.measured;
+  *reused = g_tuner.reused;
+  return ok();
 }
 
 UNARY_ENTRY(bitnot)
@@ -2026,6 +2216,99 @@ static pdlhip_error fill(pdlhip_pdl *p, uint64_t seed, double a, double b, int n
   hipError_t he = launch_fill(p->datatype, e, p->data, seed, a, b, normal, g.stream);
   return he == hipSuccess ? ok() : hip_err(he, "fill");
 }
+
+// ------------------------------------------------------------------ random numbers
+// srandom / random / randsym / grandom (Primitive.pd:2113-2262) on PDL's
+// stream (k_rng.hip).  PDL_MAYBE_SRAND (Primitive.pd:2095-2097) seeds from
+// the clock when nothing has seeded the stream yet.
+static pdlhip_error rand_fill(pdlhip_pdl *p, int kind, const char *name) {
+  RETURN_IF(ensure_init());
+  if (!p || (p->state & PDLHIP_NOMYDIMS))
+    return make_err(PDLHIP_EUSERERROR, "pdlhip: %s: the output ndarray must exist", name);
+  if (!type_on_device(p->datatype))
+    return make_err(PDLHIP_EUSERERROR, "pdlhip: %s: type %s is not supported on the device", name,
+                    type_name(p->datatype));
+  if (kind == 2 && !IsFloatCode(p->datatype))
+    return make_err(PDLHIP_EUSERERROR, "pdlhip: grandom: type %s is not a floating type", type_name(p->datatype));
+  if (capturing())
+    return make_err(PDLHIP_EUSERERROR, "pdlhip: %s: the stream's state lives on the host; it cannot be captured", name);
+  if (g.plan_only) return ok();
+  if (!g.rng_seeded) {
+    rng_seed((uint64_t)time(nullptr), g.rng);
+    g.rng_seeded = true;
+  }
+  RngPlan rp;
+  std::memset(&rp, 0, sizeof rp);
+  rp.nd = p->ndims;
+  rp.off = p->offs;
+  rp.n = 1;
+  int64_t dense_inc = 1;
+  rp.dense = 1;
+  for (int d = 0; d < p->ndims; d++) {
+    rp.dims[d] = p->dims[d];
+    rp.incs[d] = p->incs[d];
+    rp.n *= p->dims[d];
+    if (p->dims[d] > 1 && p->incs[d] != dense_inc) rp.dense = 0;
+    dense_inc *= p->dims[d];
+  }
+  if (rp.n == 0) return ok();
+  if (!g.rng_jumps) {
+    size_t nw = 0;
+    const uint64_t *jt = rng_jump_table(&nw);
+    hipError_t e = hipMalloc((void **)&g.rng_jumps, nw * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMemcpy(g.rng_jumps, jt, nw * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      if (g.rng_jumps) (void)hipFree(g.rng_jumps);
+      g.rng_jumps = nullptr;
+      return hip_err(e, name);
+    }
+  }
+  const int64_t nchunks = (rp.n + RNG_CHUNK - 1) / RNG_CHUNK;
+  std::vector<uint64_t> states((size_t)nchunks * 4);
+  rng_chunk_states(g.rng, nchunks, states.data());
+  uint64_t *d_states = (uint64_t *)dev_alloc(states.size() * sizeof(uint64_t));
+  int *d_flag = (int *)dev_alloc(sizeof(int));
+  if (!d_states || !d_flag) {
+    if (d_states) dev_free(d_states);
+    if (d_flag) dev_free(d_flag);
+    return oom_err(name);
+  }
+  hipError_t e = upload_now(d_states, states.data(), states.size() * sizeof(uint64_t));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(zero_i32, dim3(1), dim3(64), 0, g.stream, d_flag, 1);
+    e = launch_rng(p->datatype, kind, rp, p->data, d_states, g.rng_jumps, nchunks, d_flag, g.stream);
+  }
+  int h = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, d_flag, sizeof(int), hipMemcpyDeviceToHost, g.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+  dev_free(d_states);
+  dev_free(d_flag);
+  if (e != hipSuccess) return hip_err(e, name);
+  rng_advance(g.rng, (uint64_t)rp.n);
+  if (h)
+    return make_err(PDLHIP_EFATAL,
+                    "pdlhip: %s: the stream drew an exact 0.0, which PDL draws again; the device does not", name);
+  return ok();
+}
+
+pdlhip_error pdlhip_srandom(uint64_t seed) {
+  rng_seed(seed, g.rng);
+  g.rng_seeded = true;
+  return ok();
+}
+pdlhip_error pdlhip_rand_skip(uint64_t n) {
+  if (!g.rng_seeded) return make_err(PDLHIP_EUSERERROR, "pdlhip: the random stream has not been seeded");
+  rng_advance(g.rng, n);
+  return ok();
+}
+pdlhip_error pdlhip_rand_state(uint64_t state[4]) {
+  if (!g.rng_seeded) return make_err(PDLHIP_EUSERERROR, "pdlhip: the random stream has not been seeded");
+  std::memcpy(state, g.rng, sizeof g.rng);
+  return ok();
+}
+pdlhip_error pdlhip_run_random(pdlhip_pdl *a) { return rand_fill(a, 0, "random"); }
+pdlhip_error pdlhip_run_randsym(pdlhip_pdl *a) { return rand_fill(a, 1, "randsym"); }
+pdlhip_error pdlhip_grandom(pdlhip_pdl *a) { return rand_fill(a, 2, "grandom"); }
 
 pdlhip_error pdlhip_fill_uniform(pdlhip_pdl *p, uint64_t seed, double lo, double hi) { return fill(p, seed, lo, hi, 0); }
 pdlhip_error pdlhip_fill_normal(pdlhip_pdl *p, uint64_t seed, double mean, double sd) {

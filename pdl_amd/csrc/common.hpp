@@ -76,7 +76,7 @@ struct MmPlan {
   int64_t a_t, a_h, b_w, b_t, c_w, c_h;  // core strides (0 where size <= 1)
   int64_t atdi, btdi;                     // PDL_REPRINCS(a)[0], (b)[1]
   int32_t nd;
-  int32_t pad_;
+  int32_t variant;                        // kernel variant the autotuner picked (0: the default rule)
   int64_t nbatch;
   int64_t dims[MAXK];
   int64_t inc[3][MAXK];

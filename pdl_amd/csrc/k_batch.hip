@@ -277,6 +277,177 @@ __global__ __launch_bounds__(256) void list_mfma_one(const MmItem *__restrict__ 
   list_mfma_tile<T>(items[idx], blockIdx.x / ntn, blockIdx.x % ntn, As, Bs);
 }
 
+// ---------------------------------------------------------------- float on BF16 matrix cores
+// Float lists can also run as three-way BF16 splits: x = hi + mid + lo,
+// hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi - mid), each rounded
+// to nearest even; the residuals are exact and lo takes the last 8 bits, so
+// hi + mid + lo == x exactly for every normal float (down to ~2^-110,
+// where lo would be subnormal).  BF16 keeps FP32's exponent range: no
+// operand scaling, rows of any dynamic range.  Six
+// v_mfma_f32_32x32x16_bf16 products per 16-deep k step (hl, lh, mm, hm, mh,
+// hh: small terms first) with FP32 accumulation; the dropped ml, lm, ll
+// terms are at most 2^-23 of |x y|.  An Inf or NaN enters as hi (mid = lo =
+// 0), so non-finite values propagate as in FP32 arithmetic.  Each element
+// is split once, when its k chunk is staged: LDS holds
+// [stage][hi/mid/lo][row][32 k] as 16-byte segments, segment s of row r at
+// s ^ ((r >> 1) & 3), so the 8 lanes of a 16-byte access hit distinct bank
+// groups.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint32_t bf16_rne(uint32_t u) { return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16; }
+
+__device__ __forceinline__ void split3_bf16(float x, uint32_t &h, uint32_t &m, uint32_t &l) {
+  const uint32_t u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) {                 // Inf keeps its bits, NaN becomes a quiet NaN
+    h = (u & 0x7fffffu) ? (0x7fc0u | ((u >> 16) & 0x8000u)) : (u >> 16);
+    m = 0;
+    l = 0;
+    return;
+  }
+  h = bf16_rne(u);
+  if ((h & 0x7f80u) == 0x7f80u) h = u >> 16;             // above BF16's largest finite: truncate instead
+  const float r = x - __uint_as_float(h << 16);            // exact
+  m = bf16_rne(__float_as_uint(r));
+  const float r2 = r - __uint_as_float(m << 16);           // exact
+  l = bf16_rne(__float_as_uint(r2));                       // exact for normal x
+}
+
+constexpr int BKS = 32;   // k per chunk: two 16-deep MFMA steps
+
+__device__ __forceinline__ void list_bf16x3_tile(const MmItem &it, int64_t tm, int64_t tn, uint4 (*As)[3][64][4],
+                                                 uint4 (*Bs)[3][64][4]) {
+  const float *a = (const float *)it.a + it.oa, *b = (const float *)it.b + it.ob;
+  float *c = (float *)it.c;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+  const int H = (int)it.H, W = (int)it.W, K = (int)it.T;
+  const int h0 = (int)tm * 64, w0 = (int)tn * 64;
+  const int a_t = (int)it.a_t, a_h = (int)it.a_h, b_w = (int)it.b_w, b_t = (int)it.b_t;
+  const bool active = h0 + 32 * wr < H && w0 + 32 * wc < W;
+  // A: thread (row tid/4, 8 consecutive k of segment tid%4); B: thread
+  // (column tid%64, 8 consecutive k of segment tid/64)
+  const int arow = tid >> 2, aseg = tid & 3, bcol = tid & 63, bseg = tid >> 6;
+  const int ah = h0 + arow < H ? h0 + arow : H - 1, bw = w0 + bcol < W ? w0 + bcol : W - 1;
+  const bool avec = a_t == 1 && (((uintptr_t)(a + (int64_t)ah * a_h)) % 16 == 0);
+  float ra[8], rb[8];
+  auto load = [&](int k0) {
+    const int ka = k0 + aseg * 8, kb = k0 + bseg * 8;
+    if (avec && ka + 8 <= K) {
+      const Vec<float, 4> x = *reinterpret_cast<const Vec<float, 4> *>(a + (int64_t)ah * a_h + ka);
+      const Vec<float, 4> y = *reinterpret_cast<const Vec<float, 4> *>(a + (int64_t)ah * a_h + ka + 4);
+#pragma unroll
+      for (int e = 0; e < 4; e++) { ra[e] = x.v[e]; ra[4 + e] = y.v[e]; }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; e++) ra[e] = a[(int64_t)(ka + e < K ? ka + e : K - 1) * a_t + (int64_t)ah * a_h];
+    }
+#pragma unroll
+    for (int e = 0; e < 8; e++) rb[e] = b[(int64_t)(kb + e < K ? kb + e : K - 1) * b_t + (int64_t)bw * b_w];
+  };
+  auto stage = [&](int buf, int k0) {
+    uint32_t ph[4], pm[4], pl[4], qh[4], qm[4], ql[4];
+    const bool arow_ok = h0 + arow < H, bcol_ok = w0 + bcol < W;
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+      uint32_t h0_, m0_, l0_, h1_, m1_, l1_;
+      const float x0 = (arow_ok && k0 + aseg * 8 + e < K) ? ra[e] : 0.0f;
+      const float x1 = (arow_ok && k0 + aseg * 8 + e + 1 < K) ? ra[e + 1] : 0.0f;
+      split3_bf16(x0, h0_, m0_, l0_);
+      split3_bf16(x1, h1_, m1_, l1_);
+      ph[e / 2] = h0_ | (h1_ << 16); pm[e / 2] = m0_ | (m1_ << 16); pl[e / 2] = l0_ | (l1_ << 16);
+      const float y0 = (bcol_ok && k0 + bseg * 8 + e < K) ? rb[e] : 0.0f;
+      const float y1 = (bcol_ok && k0 + bseg * 8 + e + 1 < K) ? rb[e + 1] : 0.0f;
+      split3_bf16(y0, h0_, m0_, l0_);
+      split3_bf16(y1, h1_, m1_, l1_);
+      qh[e / 2] = h0_ | (h1_ << 16); qm[e / 2] = m0_ | (m1_ << 16); ql[e / 2] = l0_ | (l1_ << 16);
+    }
+    const int sa = aseg ^ ((arow >> 1) & 3), sb = bseg ^ ((bcol >> 1) & 3);
+    As[buf][0][arow][sa] = make_uint4(ph[0], ph[1], ph[2], ph[3]);
+    As[buf][1][arow][sa] = make_uint4(pm[0], pm[1], pm[2], pm[3]);
+    As[buf][2][arow][sa] = make_uint4(pl[0], pl[1], pl[2], pl[3]);
+    Bs[buf][0][bcol][sb] = make_uint4(qh[0], qh[1], qh[2], qh[3]);
+    Bs[buf][1][bcol][sb] = make_uint4(qm[0], qm[1], qm[2], qm[3]);
+    Bs[buf][2][bcol][sb] = make_uint4(ql[0], ql[1], ql[2], ql[3]);
+  };
+  lf32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+  load(0);
+  int buf = 0;
+  const int fr = lane & 31, fseg = lane >> 5;
+  const int ar_ = 32 * wr + fr, br_ = 32 * wc + fr;
+  for (int k0 = 0; k0 < K; k0 += BKS) {
+    stage(buf, k0);
+    __syncthreads();
+    if (k0 + BKS < K) load(k0 + BKS);
+    if (active) {
+#pragma unroll
+      for (int ks = 0; ks < 2; ks++) {
+        const int sa = (ks * 2 + fseg) ^ ((ar_ >> 1) & 3), sb = (ks * 2 + fseg) ^ ((br_ >> 1) & 3);
+        bf16x8 fa[3], fb[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          fa[q] = __builtin_bit_cast(bf16x8, As[buf][q][ar_][sa]);
+          fb[q] = __builtin_bit_cast(bf16x8, Bs[buf][q][br_][sb]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0], fb[2], acc, 0, 0, 0);   // hi * lo
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[2], fb[0], acc, 0, 0, 0);   // lo * hi
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1], fb[1], acc, 0, 0, 0);   // mid * mid
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0], fb[1], acc, 0, 0, 0);   // hi * mid
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1], fb[0], acc, 0, 0, 0);   // mid * hi
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0], fb[0], acc, 0, 0, 0);   // hi * hi
+      }
+    }
+    buf ^= 1;
+  }
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int64_t gh = h0 + 32 * wr + 8 * (r / 4) + 4 * (lane >> 5) + (r & 3);
+      const int64_t gw = w0 + 32 * wc + (lane & 31);
+      if (gh < H && gw < W) c[it.oc + gw * it.c_w + gh * it.c_h] = acc[r];
+    }
+  }
+  __syncthreads();   // the LDS buffers are refilled by the next work item
+}
+
+__global__ __launch_bounds__(256) void list_bf16x3_queue(const MmItem *__restrict__ items,
+                                                         const int3 *__restrict__ work, unsigned int nwork,
+                                                         unsigned int *counter) {
+  __shared__ uint4 As[2][3][64][4];
+  __shared__ uint4 Bs[2][3][64][4];
+  __shared__ unsigned int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+    __syncthreads();
+    const unsigned int w = next;
+    __syncthreads();
+    if (w >= nwork) break;
+    const int3 job = work[w];
+    list_bf16x3_tile(items[job.x], job.y, job.z, As, Bs);
+  }
+}
+
+__global__ __launch_bounds__(256) void list_bf16x3_one(const MmItem *__restrict__ items, int idx, int64_t ntn) {
+  __shared__ uint4 As[2][3][64][4];
+  __shared__ uint4 Bs[2][3][64][4];
+  list_bf16x3_tile(items[idx], blockIdx.x / ntn, blockIdx.x % ntn, As, Bs);
+}
+
+static hipError_t list_bf16x3(const MmItem *d_items, const MmItem *h_items, int nitems, const int3 *d_work,
+                              unsigned int nwork, unsigned int *d_counter, bool each, int nblocks, hipStream_t s) {
+  if (!each) {
+    hipLaunchKernelGGL(list_reset, dim3(1), dim3(64), 0, s, d_counter);
+    hipLaunchKernelGGL(list_bf16x3_queue, dim3(nblocks), dim3(256), 0, s, d_items, d_work, nwork, d_counter);
+    return hipGetLastError();
+  }
+  for (int i = 0; i < nitems; i++) {
+    const int64_t ntm = (h_items[i].H + 63) / 64, ntn = (h_items[i].W + 63) / 64;
+    if (ntm * ntn == 0 || h_items[i].T == 0) continue;
+    hipLaunchKernelGGL(list_bf16x3_one, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, d_items, i, ntn);
+  }
+  return hipGetLastError();
+}
+
 template <class T>
 static hipError_t list_mfma_T(const MmItem *d_items, const MmItem *h_items, int nitems, const int3 *d_work,
                               unsigned int nwork, unsigned int *d_counter, bool each, int nblocks, hipStream_t s) {
@@ -321,7 +492,10 @@ hipError_t launch_matmult_list(int T, const MmItem *d_items, const MmItem *h_ite
                                unsigned int nwork, unsigned int *d_counter, int tile, bool fused, bool each,
                                int nblocks, hipStream_t s) {
   // tile 0: the matrix-core kernels (64x64 work items), float/double lists
-  // in the default modes
+  // in the default modes; tile 1: float on the BF16 matrix cores
+  if (tile == 1 && fused && T == PDLHIP_F)
+    return list_bf16x3(d_items, h_items, nitems, d_work, nwork, d_counter, each, nblocks, s);
+  if (tile == 1) tile = 0;
   if (tile == 0 && fused && T == PDLHIP_F)
     return list_mfma_T<float>(d_items, h_items, nitems, d_work, nwork, d_counter, each, nblocks, s);
   if (tile == 0 && fused && T == PDLHIP_D)

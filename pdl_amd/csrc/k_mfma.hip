@@ -1008,9 +1008,10 @@ hipError_t launch_mfma_f64(const MmPlan &p, const double *a, const double *b, do
   // one product per call, where a 128x128 block is mostly useful and there
   // are at least 64 of them (fewer leave most CUs idle: the SIMT kernel's
   // smaller tiles win below about 1024x1024)
-  if (p.nbatch != 1 || p.H < 64 || p.W < 64 || p.T < 16) return hipErrorNotSupported;
+  // variant 1: the SIMT kernel; 2: the matrix cores whatever the block count
+  if (p.variant == 1 || p.nbatch != 1 || p.H < 64 || p.W < 64 || p.T < 16) return hipErrorNotSupported;
   const int64_t ntm = (p.H + 127) / 128, ntn = (p.W + 127) / 128;
-  if (ntm * ntn < 64 || ntm * ntn >= ((int64_t)1 << 31)) return hipErrorNotSupported;
+  if ((ntm * ntn < 64 && p.variant != 2) || ntm * ntn >= ((int64_t)1 << 31)) return hipErrorNotSupported;
   hipLaunchKernelGGL((dgemm_mfma<128, 128, 2, 2>), dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, a, b, c, p, ntm, ntn);
   return hipGetLastError();
 }
@@ -1039,8 +1040,9 @@ hipError_t launch_mfma_f32(const MmPlan &p, const float *a, const float *b, floa
   // otherwise one matrix product per call (broadcast batches use the SIMT
   // kernel), and only where a 128x128 tile is mostly useful
   if (p.nbatch != 1 || p.H < 64 || p.W < 64 || p.T < 32) return hipErrorNotSupported;
-  // 256x256 blocks once both sides fill them (FP8 then steps k by 64)
-  const bool big = p.H >= 1024 && p.W >= 1024;
+  // 256x256 blocks once both sides fill them (FP8 then steps k by 64), or as
+  // the autotuner picked (variant 128 / 256)
+  const bool big = p.variant ? p.variant == 256 : (p.H >= 1024 && p.W >= 1024);
   const int64_t BMr = big ? 256 : MB, BNr = big ? 256 : NB, KS = (big && fp8) ? KB8 : KB;
   const int64_t Hp = (p.H + BMr - 1) / BMr * BMr, Wp = (p.W + BNr - 1) / BNr * BNr, Kp = (p.T + KS - 1) / KS * KS;
   const size_t esz = fp8 ? 1 : 2, ncopy = fp8 ? 1 : 2;

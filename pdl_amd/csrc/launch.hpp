@@ -41,6 +41,29 @@ hipError_t launch_matmult_list(int T, const MmItem *d_items, const MmItem *h_ite
 hipError_t launch_badop(int T, int op, const EwPlan &p, const void *a, const void *m, void *b, const BadInfo &ba,
                         int32_t mbv, const pdlhip_value &bbv, double dval, bool bad, int *flag, hipStream_t s);
 
+// elementwise ops on inputs of other types, converted in registers
+// (k_mixed.hip): a, b of types ta, tb; sbad_*: the source is bad-flagged
+// (its bad elements become T's default bad value, as in a converted copy)
+struct MixedArgs {
+  const void *a, *b;
+  int32_t ta, tb;
+  int32_t sbad_a, sbad_b;
+  BadInfo sba, sbb;
+};
+#define X(TC) hipError_t launch_mixed_t##TC(int op, const MixedArgs &m, const EwPlan &p, void *c, bool bad, \
+                                            const BadInfo *bi, hipStream_t s);
+PDLHIP_FOR_EACH_TYPE(X)
+#undef X
+inline hipError_t launch_mixed(int T, int op, const MixedArgs &m, const EwPlan &p, void *c, bool bad,
+                               const BadInfo *bi, hipStream_t s) {
+  switch (T) {
+#define X(TC) case TC: return launch_mixed_t##TC(op, m, p, c, bad, bi, s);
+    PDLHIP_FOR_EACH_TYPE(X)
+#undef X
+  default: return hipErrorInvalidValue;
+  }
+}
+
 // ragged lists (k_ragged.hip)
 hipError_t launch_ragged_reduce(int T, int op, const RagLaunch &L, hipStream_t s);
 hipError_t launch_ragged_scan(int T, int op, const RagLaunch &L, hipStream_t s);
@@ -49,6 +72,25 @@ hipError_t launch_ragged_scan(int T, int op, const RagLaunch &L, hipStream_t s);
 // PDL's bad-value mapping (pdlconv.c:163-174); also assgn (Ops.pd:382-397)
 hipError_t launch_convert(int stype, int dtype, const EwPlan &p, const void *src, void *dst, bool bad,
                           const BadInfo &sbi, const pdlhip_value &dbv, hipStream_t s);
+// PDL's random / randsym / grandom stream (k_rng.hip).  kind 0 random, 1
+// randsym, 2 grandom; the output's logical element e (dim 0 fastest) sits at
+// off + sum_d i_d * incs[d] (off + e when dense) and takes draw e
+struct RngPlan {
+  int32_t nd;
+  int32_t dense;
+  int64_t n;
+  int64_t off;
+  int64_t dims[PDLHIP_MAXDIMS];
+  int64_t incs[PDLHIP_MAXDIMS];
+};
+void rng_seed(uint64_t seed, uint64_t s[4]);             // pdl_srand's first state
+void rng_advance(uint64_t s[4], uint64_t n);             // s = the state n draws later
+void rng_chunk_states(const uint64_t s[4], int64_t nchunks, uint64_t *out);  // every 4096 draws
+const uint64_t *rng_jump_table(size_t *nwords);          // the in-chunk jumps, for the device
+hipError_t launch_rng(int T, int kind, const RngPlan &p, void *out, const uint64_t *d_states, const uint64_t *d_jumps,
+                      int64_t nchunks, int *d_flag, hipStream_t s);
+constexpr int64_t RNG_CHUNK = 4096;
+
 hipError_t launch_fill(int dtype, const EwPlan &p, void *dst, uint64_t seed, double a, double b, int normal,
                        hipStream_t s);
 

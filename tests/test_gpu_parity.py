@@ -239,6 +239,56 @@ def test_mixed_types_convert_on_device():
     same(oo, dd, "into double")
 
 
+MIXED_PAIRS = [(O.F, O.B), (O.B, O.S), (O.L, O.D), (O.SB, O.UL), (O.US, O.F), (O.LL, O.F), (O.D, O.IND),
+               (O.ULL, O.L)]
+
+
+@pytest.mark.parametrize("ta,tb", MIXED_PAIRS, ids=["%s-%s" % (NAMES[a], NAMES[b]) for a, b in MIXED_PAIRS])
+def test_mixed_types_fused_conversion_bit_exact(ta, tb):
+    """inputs of other types than the op's are converted in registers
+    (one pass, no copies): bit-exact with the oracle's converted copies for
+    arithmetic, comparisons, bit ops and a ufunc, through a transposed view
+    and a broadcast operand, with bad-flagged sources (their bad elements
+    become the computing type's bad value) and an inplace output that
+    still takes a copy"""
+    rng = np.random.default_rng(300 + 16 * ta + tb)
+    a = rand(ta, (24, 70), rng, 0 if np.dtype(O.NP_TYPES[ta]).kind == "u" else -50, 50)
+    b = rand(tb, (70,), rng, 1, 40)
+    ops = ["plus", "mult", "minus", "gt", "eq", "divide"]
+    if ta < O.F and tb < O.F:
+        ops += ["or2", "xor", "shiftleft"]
+    for op in ops:
+        for lay in ((), (("xchg", 0, 1),)):
+            A = pair(a, ta)
+            A = (view(A[0], lay), view(A[1], lay))
+            B = pair(b, tb) if not lay else pair(rng.integers(1, 40, (24,)).astype(O.NP_TYPES[tb]), tb)
+            (o, d), = run_both(op, [A, B])
+            same(o, d, "%s %s %s %s" % (op, NAMES[ta], NAMES[tb], lay))
+    # bad-flagged sources: a bad element of either becomes a bad output
+    a2, b2 = a.copy(), b.copy()
+    a2.reshape(-1)[::9] = O.default_badval(ta)
+    b2[::7] = O.default_badval(tb)
+    for fa, fb in ((True, False), (False, True), (True, True)):
+        for op in ("plus", "modulo" if ta < O.F and tb < O.F else "power"):
+            (o, d), = run_both(op, [pair(a2, ta, fa), pair(b2, tb, fb)])
+            what = "%s %s %s bad %s %s" % (op, NAMES[ta], NAMES[tb], fa, fb)
+            if op == "power":
+                x, y = o.gather(), d.numpy()
+                assert x.dtype == y.dtype and ulp_distance(x, y).max() <= 2, what
+                assert bool(o.state & O.BADVAL) == d.badflag(), what
+            else:
+                same(o, d, what)
+    # a ufunc of another type's input (sqrt of an integer input computes in its type; exp in double)
+    (o, d), = run_both("exp", [pair(np.abs(a) % 20, ta)])
+    x, y = o.gather(), d.numpy()
+    assert o.dtype == d.dtype and ulp_distance(x, y).max() <= 2
+    # inplace: the input that is also the output still converts through a copy
+    oa, da = pair(a.copy(), ta)
+    O.run("plus", oa, O.OPdl.from_numpy(b, tb), oa)
+    C._run("plus", [da, C.ndarray.from_numpy(b, tb), da])
+    same(oa, da, "inplace %s %s" % (NAMES[ta], NAMES[tb]))
+
+
 def test_inplace_through_slice():
     rng = np.random.default_rng(4)
     x = rand(O.L, (6, 50), rng)
@@ -872,3 +922,33 @@ def test_graph_capture_keeps_scratch_and_refuses_what_cannot_be_captured():
     # the library still works eagerly afterwards
     C._run("matmult", [a, b, c])
     assert c.numpy().tobytes() == c_eager.tobytes()
+
+
+def test_device_allocation_failure_is_a_fatal_error_and_leaves_buffers_intact():
+    """an output the device cannot hold (a null output broadcast to 2^40
+    elements by a dummy dim) fails as PDL_EFATAL with PDL's kind of message
+    (pdl.h.PL:370-379; pdlmagic.c:288-296 / pdlutil.c:519-531 report such
+    failures as fatal, not user errors); the inputs keep their data and
+    bad flags, the library keeps working, and a direct allocation that
+    cannot succeed fails the same way"""
+    import ctypes
+    from pdl_amd import _lib as L
+    from pdl_amd._lib import PDLError
+    rng = np.random.default_rng(77)
+    ah = rng.standard_normal((3, 5))
+    a = C.ndarray.from_numpy(ah, O.D)
+    b = C.ndarray.from_numpy(ah[0], O.D, badflag=True)
+    huge = a.dummy(2, 1 << 40)
+    out = C.ndarray()
+    with pytest.raises(PDLError) as ei:
+        C._run("plus", [huge, b, out])
+    assert ei.value.kind == L.EFATAL and "out of device memory" in str(ei.value)
+    assert out.isnull()
+    np.testing.assert_array_equal(a.numpy(), ah)
+    np.testing.assert_array_equal(b.numpy(), ah[0])
+    assert b.badflag() and not a.badflag()
+    p = ctypes.c_void_p()
+    e = L.lib().pdlhip_alloc(1 << 50, ctypes.byref(p))
+    assert e.error == L.EFATAL and b"out of device memory" in e.message
+    # still working
+    np.testing.assert_array_equal((a + a).numpy(), ah + ah)

@@ -80,7 +80,7 @@ def rand(t, shape, rng, lo=None, hi=None):
 
 
 LAYOUTS = {"flat": ((37, 91), []), "xchg": ((130, 70), [("xchg", 0, 1)]),
-           "slice": ((40, 30), [("slice", (1, 38, 3), None)]), "dummy": ((50,), [("dummy", 1, 7)])}
+           "slice": ((40, 60), [("slice", (1, 38, 3), None)]), "dummy": ((50,), [("dummy", 1, 7)])}
 
 
 @pytest.mark.parametrize("layout", list(LAYOUTS))
@@ -98,9 +98,16 @@ def test_exact_ufuncs_bit_exact(t, layout):
         a.reshape(-1)[:2] = [0, np.iinfo(a.dtype).min]
     A = pair(a, t)
     A = (view(A[0], v), view(A[1], v))
-    for op in ("not", "abs", "bitnot"):
+    for op in ("not", "abs"):
         (o, d), = run_both(op, [A])
         same(o, d, "%s %s %s" % (op, NAMES[t], layout))
+    # bitnot of a float input computes in longlong: keep the converted values
+    # defined (NaN and infinities convert to integers by no rule of C)
+    f = np.nan_to_num(a, nan=1.0, posinf=2.0, neginf=-3.0) if t >= O.F else a
+    Fp = pair(f, t)
+    Fp = (view(Fp[0], v), view(Fp[1], v))
+    (o, d), = run_both("bitnot", [Fp])
+    same(o, d, "bitnot %s %s" % (NAMES[t], layout))
     s = np.abs(a) if a.dtype.kind != "u" else a
     if a.dtype.kind == "i":
         s = np.where(s < 0, 0, s).astype(a.dtype)       # |INT_MIN| wraps negative: keep sqrt's input >= 0

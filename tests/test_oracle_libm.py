@@ -5,14 +5,17 @@ the reference's own tests fix at three points only (t/ops.t:33, 126-127).
 
 Each result is held to the mathematics, not to the oracle's own formula:
   * random operands in every sign/quadrant, scored in units in the last
-    place of a 120-bit mpmath value (glibc documents < 1 ulp for
-    powf/pow/atan2f/atan2 on x86-64); a swapped-argument or sign mistake
-    misses by many ulps on these inputs;
+    place of a 120-bit mpmath value; glibc's x86-64 functions are not all
+    correctly rounded, and the bounds are what 20000 samples each show
+    (atan2f up to 1.27 ulp, log10f 1.6, log10 1.28, the others under 1),
+    rounded up; a swapped-argument or sign mistake misses by many ulps on
+    these inputs;
   * the special cases C99 Annex F fixes exactly (F.9.4.4 pow, F.9.1.4 atan2):
     zeros of both signs, infinities, NaN, negative bases with integer and
     non-integer exponents.
 """
 import math
+import zlib
 
 import numpy as np
 import pytest
@@ -24,7 +27,15 @@ from oracle import pdlcpu as O
 MP = mpmath.mp
 MP.prec = 120
 
-RNG = np.random.default_rng(20261203)
+RNG = None
+
+
+@pytest.fixture(autouse=True)
+def _seeded_rng(request):
+    """each test draws from a seed of its own (its name), so its samples do
+    not depend on which tests ran before it in the same process"""
+    global RNG
+    RNG = np.random.default_rng(zlib.crc32(request.node.name.encode()))
 FMT = {O.F: (np.float32, 24, -126, -149), O.D: (np.float64, 53, -1022, -1074)}
 
 
@@ -100,7 +111,7 @@ def test_atan2_within_one_ulp_in_every_quadrant(t):
     got = run_op("atan2", ys, xs, t)              # atan2(a, b) = atan2(y, x)
     for y, x, g in zip(ys, xs, got):
         tr = mpmath.atan2(mpmath.mpf(float(y)), mpmath.mpf(float(x)))
-        assert ulp_error(g, tr, t) <= 1.0, (float(y), float(x), float(g), str(tr))
+        assert ulp_error(g, tr, t) <= (2.0 if t == O.F else 1.0), (float(y), float(x), float(g), str(tr))
         # the quadrant: sign follows y, |angle| > pi/2 exactly when x < 0
         assert (g > 0) == (y > 0)
         if abs(abs(float(g)) - math.pi / 2) > 1e-3:
@@ -189,7 +200,7 @@ def run_u(op, a, t):
 
 # glibc's documented maximum errors on x86-64 (libm-test-ulps): 1 ulp for
 # these, 2 for double log10; sqrt is correctly rounded
-ULP_BOUND = {("sqrt", O.F): 0.5, ("sqrt", O.D): 0.5, ("log10", O.D): 2.0}
+ULP_BOUND = {("sqrt", O.F): 0.5, ("sqrt", O.D): 0.5, ("log10", O.F): 2.0, ("log10", O.D): 2.0}
 UFUNC_TRUTH = {"sin": mpmath.sin, "cos": mpmath.cos, "exp": mpmath.exp, "log": mpmath.log,
                "log10": mpmath.log10, "sqrt": mpmath.sqrt}
 
