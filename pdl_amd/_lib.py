@@ -107,6 +107,7 @@ BADOPS = ["isbad", "isgood", "setnantobad", "setinftobad", "setnonfinitetobad", 
 for _n in UFUNCS + BADOPS:
     EXPORTS["pdlhip_run_" + _n] = ([PP, PP], Error)
 EXPORTS["pdlhip_run_setbadif"] = ([PP, PP, PP], Error)
+EXPORTS["pdlhip_run_reduce_noalloc"] = ([ctypes.c_char_p, PP, PP], Error)
 EXPORTS["pdlhip_tune_stats"] = ([ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)], Error)
 # PDL's random-number stream
 EXPORTS["pdlhip_srandom"] = ([ctypes.c_uint64], Error)

@@ -41,6 +41,16 @@ namespace pdlhip {
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_msg;
 
+// ------------------------------------------------------------------ allocation-free reductions
+// The scratch of pdlhip_run_reduce_noalloc: the long-row reductions'
+// per-chunk partials (at most 2048 + 1023 of them, under 200 KiB) live in
+// this array instead of a cached hipMalloc block.  Each GPU holds its own
+// copy until the process ends; host code uses the device address from
+// hipGetSymbolAddress (pdlhip_init), never the array's name.
+constexpr size_t NOALLOC_BYTES = (size_t)256 << 10;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+static thread_local bool g_noalloc = false;   // inside pdlhip_run_reduce_noalloc
+
 static pdlhip_error ok() { return pdlhip_error{PDLHIP_ENONE, nullptr}; }
 
 static pdlhip_error make_err(int kind, const char *fmt, ...) {
@@ -59,6 +69,10 @@ static bool capturing();
 // blocks are handed out (no hipMalloc, no synchronisation), so an op whose
 // scratch is not cached yet must run once before the capture
 static pdlhip_error oom_err(const char *what) {
+  if (g_noalloc)
+    return make_err(PDLHIP_EUSERERROR,
+                    "pdlhip_run_reduce_noalloc: %s needs more scratch than the %zu bytes this entry point reserves",
+                    what, NOALLOC_BYTES);
   if (capturing())
     return make_err(PDLHIP_EUSERERROR,
                     "pdlhip: %s: needs device memory that is not cached while a graph is being captured "
@@ -101,6 +115,7 @@ struct Global {
   // them is destroyed
   bool capturing = false;
   std::vector<void *> pinned;
+  void *noalloc_scratch = nullptr;   // device address of `scratch`
   // PDL's random-number stream (one state: PDL with autopthread off)
   uint64_t rng[4] = {0, 0, 0, 0};
   bool rng_seeded = false;
@@ -161,6 +176,7 @@ static void release_cached() {
 }
 
 void *dev_alloc(size_t n) {
+  if (g_noalloc) return nullptr;
   size_t sz = round_size(n);
   std::lock_guard<std::mutex> lk(g_cache.mu);
   auto it = g_cache.freelist.lower_bound(sz);
@@ -813,6 +829,10 @@ static pdlhip_error readdata(const Sig &s, int T, bool bad, const Brc &b, pdlhip
       for (int q = 0; q < np; q++) p.inc[q][k] = d.inc[q][k];
     }
     for (int q = 0; q < np; q++) p.off[q] = b.offs[q];
+    if (g_noalloc) {
+      p.scratch = g.noalloc_scratch;
+      p.scratch_bytes = (int64_t)NOALLOC_BYTES;
+    }
     void *outs[4] = {nullptr, nullptr, nullptr, nullptr};
     pdlhip_value obv[4];
     std::memset(obv, 0, sizeof obv);
@@ -1754,6 +1774,10 @@ pdlhip_error pdlhip_init(int device) {
   if (e != hipSuccess) return hip_err(e, "hipStreamCreate");
   if ((e = hipEventCreate(&g.ev0)) != hipSuccess) return hip_err(e, "hipEventCreate");
   if ((e = hipEventCreate(&g.ev1)) != hipSuccess) return hip_err(e, "hipEventCreate");
+  // this GPU's copy of the allocation-free entry point's scratch
+  void *ptr = nullptr;
+  if ((e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch))) != hipSuccess) return hip_err(e, "hipGetSymbolAddress");
+  g.noalloc_scratch = ptr;
   g.device = device;
   return ok();
 }
@@ -1904,6 +1928,31 @@ pdlhip_error pdlhip_tune_stats(int64_t *measured, int64_t *reused) {
   *measured = g_tuner.measured;
   *reused = g_tuner.reused;
   return ok();
+}
+
+pdlhip_error pdlhip_run_reduce_noalloc(const char *op, pdlhip_pdl *a, pdlhip_pdl *b) {
+  const Sig *s = op ? find_sig(op) : nullptr;
+  if (!s || s->family != FAM_RED || s->npdls != 2 || s->realdims[1] != 0)
+    return make_err(PDLHIP_EUSERERROR,
+                    "pdlhip_run_reduce_noalloc: '%s' is not one of sumover, prodover, dsumover, dprodover, "
+                    "average, daverage, minimum, maximum, minimum_ind, maximum_ind",
+                    op ? op : "(null)");
+  if (!a || !b) return make_err(PDLHIP_EUSERERROR, "pdlhip_run_reduce_noalloc: %s: NULL ndarray", op);
+  if (b->state & PDLHIP_NOMYDIMS)
+    return make_err(PDLHIP_EUSERERROR, "pdlhip_run_reduce_noalloc: %s: the output must exist (none is created)", op);
+  RETURN_IF(ensure_init());
+  if (g.plan_only) return make_err(PDLHIP_EUSERERROR, "pdlhip_run_reduce_noalloc: not available when only planning");
+  pdlhip_pdl *v[2] = {a, b};
+  const int T = select_transtype(*s, v);
+  const int tout = adjust_from_trans(T, s->flags[1], s->par_types[1]);
+  if (a->datatype != T || b->datatype != tout)
+    return make_err(PDLHIP_EUSERERROR,
+                    "pdlhip_run_reduce_noalloc: %s: a must be %s and the output %s (no converted copy is made)", op,
+                    type_name(T), type_name(tout));
+  g_noalloc = true;
+  pdlhip_error e = run(*s, v, 2, nullptr);
+  g_noalloc = false;
+  return e;
 }
 
 UNARY_ENTRY(bitnot)

@@ -67,6 +67,8 @@ struct RedPlan {
   int64_t off[MAXP];
   int64_t m;                  // *_n_ind: m_size; scans: b's n extent
   int64_t inc_m;              // *_n_ind: c's stride along m
+  void *scratch;              // caller-provided scratch for the long-row split (null: from the cache)
+  int64_t scratch_bytes;      // its size
 };
 
 // Matmult launch plan (Primitive.pd:211-245): a(t,h) b(w,t) c(w,h) with

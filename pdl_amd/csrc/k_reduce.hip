@@ -342,8 +342,16 @@ static hipError_t red_launch(const RedPlan &p, const Tin *a, const Outs &o, cons
         if (clen < 16384) clen = 16384;
         clen = ((clen + VEC * 256 - 1) / (VEC * 256)) * (VEC * 256);
         int64_t nchunk = (p.n + clen - 1) / clen;
-        Part<Tin, Tacc> *parts = (Part<Tin, Tacc> *)scratch_alloc(sizeof(Part<Tin, Tacc>) * p.nout * nchunk);
-        if (!parts) return hipErrorOutOfMemory;
+        // partials in the caller's scratch when it brings one, else from the cache
+        const size_t need = sizeof(Part<Tin, Tacc>) * (size_t)p.nout * (size_t)nchunk;
+        Part<Tin, Tacc> *parts;
+        if (p.scratch) {
+          if (need > (size_t)p.scratch_bytes) return hipErrorOutOfMemory;
+          parts = (Part<Tin, Tacc> *)p.scratch;
+        } else {
+          parts = (Part<Tin, Tacc> *)scratch_alloc(need);
+          if (!parts) return hipErrorOutOfMemory;
+        }
         if (aligned)
           hipLaunchKernelGGL((red_split1<OP, Tin, Tout, Tacc, BAD, VEC>), dim3((unsigned)(p.nout * nchunk)), dim3(RT),
                              0, s, a, p, nchunk, clen, parts, ba, stream);
@@ -353,7 +361,7 @@ static hipError_t red_launch(const RedPlan &p, const Tin *a, const Outs &o, cons
         hipLaunchKernelGGL((red_split2<OP, Tin, Tout, Tacc, BAD>), dim3((unsigned)((p.nout + 3) / 4)), dim3(RT), 0, s,
                            p, nchunk, parts, o);
         hipError_t err = hipGetLastError();
-        scratch_free(parts);
+        if (!p.scratch) scratch_free(parts);
         return err;
       }
       int64_t blocks = (p.nout + 3) / 4;
