@@ -208,6 +208,8 @@ typedef struct trans {
   int badval_isnan[MAXP];
   volatile char setbad[MAXP];  /* $PDLSTATESETBAD requests */
   volatile char setgood[MAXP]; /* $PDLSTATESETGOOD requests */
+  volatile int croaked;        /* $CROAK from the loop body (PP.pm:371) */
+  char croak[512];
 } trans;
 
 /* per-thread dims/offsets (PDL_BRC_THR_OFFSET, pdlbroadcast.h:65-70;
@@ -844,6 +846,74 @@ static void rd_append(trans *tr, int thr) {
 }
 
 /* ------------------------------------------------------------------ */
+/* Slices.pd index / index1d / index2d (Slices.pd:142-220): Code       */
+/* gathers c() = a(n => ind), BackCode (the dataflow writeback,        */
+/* pdl.h.PL writebackdata) scatters a(n => ind) = c().  An index out of */
+/* range, or bad in bad mode, $CROAKs "Error in <op>:invalid index ..."  */
+/* except index1d, which sets c bad for a bad index (and skips it when  */
+/* writing back).  Opcodes: 0 index, 1 index1d, 2 index2d; +10 BackCode */
+/* ------------------------------------------------------------------ */
+enum { X_INDEX = 0, X_INDEX1D = 1, X_INDEX2D = 2, X_BACK = 10 };
+
+#define CROAK_AT(...) do { \
+  if (!tr->croaked) { tr->croaked = 1; snprintf(tr->croak, sizeof tr->croak, __VA_ARGS__); } \
+  goto croak_out; } while (0)
+
+#define INDEX_LOOP(ctype, ppsym, BACK) do { \
+  DECL(0, ctype, ppsym, a) DECL(2, ctype, ppsym, c) \
+  const int64_t *ind = (const int64_t *)tr->pdls[1]->data; const int64_t ind_bv = tr->badval[1].value.N; \
+  const int64_t ns = tr->ind_sizes[0], ia = tr->inc_sizes[0][0]; const int bad = tr->bvalflag; \
+  BLOOP(tr, thr, { const int64_t this_ind = ind[O1]; \
+    if ((bad && this_ind == ind_bv) || this_ind < 0 || this_ind >= ns) \
+      CROAK_AT("Error in %s:invalid index %lld (valid range 0..%lld)", name, (long long)this_ind, (long long)(ns - 1)); \
+    if (BACK) a[O0 + this_ind * ia] = c[O2]; else c[O2] = a[O0 + this_ind * ia]; }); \
+} while (0)
+
+#define INDEX1D_LOOP(ctype, ppsym, BACK) do { \
+  DECL(0, ctype, ppsym, a) DECL(2, ctype, ppsym, c) \
+  const int64_t *ind = (const int64_t *)tr->pdls[1]->data; const int64_t ind_bv = tr->badval[1].value.N; \
+  const int64_t ns = tr->ind_sizes[0], ms = tr->ind_sizes[1], ia = tr->inc_sizes[0][0]; \
+  const int64_t ii = tr->inc_sizes[1][0], ic = tr->inc_sizes[2][0]; const int bad = tr->bvalflag; \
+  BLOOP(tr, thr, { for (int64_t m = 0; m < ms; m++) { const int64_t this_ind = ind[O1 + m * ii]; \
+    if (bad && this_ind == ind_bv) { if (!(BACK)) c[O2 + m * ic] = c_bv; continue; } \
+    if (this_ind < 0 || this_ind >= ns) \
+      CROAK_AT("Error in %s:invalid index %lld at pos %lld (valid range 0..%lld)", name, (long long)this_ind, \
+               (long long)m, (long long)(ns - 1)); \
+    if (BACK) a[O0 + this_ind * ia] = c[O2 + m * ic]; else c[O2 + m * ic] = a[O0 + this_ind * ia]; } }); \
+} while (0)
+
+#define INDEX2D_LOOP(ctype, ppsym, BACK) do { \
+  DECL(0, ctype, ppsym, a) DECL(3, ctype, ppsym, c) \
+  const int64_t *inda = (const int64_t *)tr->pdls[1]->data, *indb = (const int64_t *)tr->pdls[2]->data; \
+  const int64_t bva = tr->badval[1].value.N, bvb = tr->badval[2].value.N; \
+  const int64_t na = tr->ind_sizes[0], nb = tr->ind_sizes[1]; \
+  const int64_t ia = tr->inc_sizes[0][0], ib = tr->inc_sizes[0][1]; const int bad = tr->bvalflag; \
+  BLOOP(tr, thr, { const int64_t xa = inda[O1], xb = indb[O2]; \
+    if ((bad && xa == bva) || xa < 0 || xa >= na) \
+      CROAK_AT("Error in %s:invalid x-index %lld (valid range 0..%lld)", name, (long long)xa, (long long)(na - 1)); \
+    if ((bad && xb == bvb) || xb < 0 || xb >= nb) \
+      CROAK_AT("Error in %s:invalid y-index %lld (valid range 0..%lld)", name, (long long)xb, (long long)(nb - 1)); \
+    if (BACK) a[O0 + xa * ia + xb * ib] = c[O3]; else c[O3] = a[O0 + xa * ia + xb * ib]; }); \
+} while (0)
+
+static void rd_index(trans *tr, int thr) {
+  const int op = tr->vt->opcode % X_BACK, back = tr->vt->opcode >= X_BACK;
+  const char *name = op == X_INDEX ? "index" : op == X_INDEX1D ? "index1d" : "index2d";
+  switch (tr->datatype) {
+#define X(code, ctype, ppsym) case code: \
+    if (op == X_INDEX) { if (back) INDEX_LOOP(ctype, ppsym, 1); else INDEX_LOOP(ctype, ppsym, 0); } \
+    else if (op == X_INDEX1D) { if (back) INDEX1D_LOOP(ctype, ppsym, 1); else INDEX1D_LOOP(ctype, ppsym, 0); } \
+    else { if (back) INDEX2D_LOOP(ctype, ppsym, 1); else INDEX2D_LOOP(ctype, ppsym, 0); } \
+    break;
+    ARITH_CASES(X)
+#undef X
+  default: break;
+  }
+croak_out:
+  return;
+}
+
+/* ------------------------------------------------------------------ */
 /* The vtables                                                         */
 /* ------------------------------------------------------------------ */
 #define BIOP_VT(nm, opc, gt, rd, mode) \
@@ -928,6 +998,20 @@ static vtable vtables[] = {
    {-1, PDLC_IND}, GT_REAL, rd_nind, 1, 0},
   {"append", 3, 2, {"a", "b", "c"}, {1, 1, 1}, {{0}, {1}, {2}}, {"n", "m", "mn"},
    {0, 0, P_ISOUT | P_ISCREAT}, {-1, -1, -1}, GT_ALL, rd_append, 0, 0},
+  /* Slices.pd:149-220 (GenericTypes ppdefs_all; ind typed indx) */
+  {"index", 3, 2, {"a", "ind", "c"}, {1, 0, 0}, {{0}}, {"n"}, {0, P_ISTYPED, P_ISOUT | P_ISCREAT},
+   {-1, PDLC_IND, -1}, GT_ALL, rd_index, X_INDEX, 0},
+  {"index1d", 3, 2, {"a", "ind", "c"}, {1, 1, 1}, {{0}, {1}, {1}}, {"n", "m"}, {0, P_ISTYPED, P_ISOUT | P_ISCREAT},
+   {-1, PDLC_IND, -1}, GT_ALL, rd_index, X_INDEX1D, 0},
+  {"index2d", 4, 3, {"a", "inda", "indb", "c"}, {2, 0, 0, 0}, {{0, 1}}, {"na", "nb"},
+   {0, P_ISTYPED, P_ISTYPED, P_ISOUT | P_ISCREAT}, {-1, PDLC_IND, PDLC_IND, -1}, GT_ALL, rd_index, X_INDEX2D, 0},
+  /* their BackCode: every ndarray exists, a is written */
+  {"index_back", 3, 3, {"a", "ind", "c"}, {1, 0, 0}, {{0}}, {"n"}, {0, P_ISTYPED, 0},
+   {-1, PDLC_IND, -1}, GT_ALL, rd_index, X_INDEX + X_BACK, 0},
+  {"index1d_back", 3, 3, {"a", "ind", "c"}, {1, 1, 1}, {{0}, {1}, {1}}, {"n", "m"}, {0, P_ISTYPED, 0},
+   {-1, PDLC_IND, -1}, GT_ALL, rd_index, X_INDEX1D + X_BACK, 0},
+  {"index2d_back", 4, 4, {"a", "inda", "indb", "c"}, {2, 0, 0, 0}, {{0, 1}}, {"na", "nb"},
+   {0, P_ISTYPED, P_ISTYPED, 0}, {-1, PDLC_IND, PDLC_IND, -1}, GT_ALL, rd_index, X_INDEX2D + X_BACK, 0},
 };
 
 static void init_vtables(void) {
@@ -1474,7 +1558,16 @@ int pdlc_run(const char *op, pdlc_pdl **user_pdls, int npdls, const int64_t *oth
   if ((vt->badmode_biop == 1 || vt->badmode_biop == 2) && otherpars) tr.swap = otherpars[0];
   if (vt->badmode_biop == 3 && otherpars) memcpy(&tr.dpar, &otherpars[0], sizeof tr.dpar);
   capture_plan(&tr);
+  /* a BackCode writes its first parameter: it must already have the trans type */
+  if (vt->readdata == rd_index && vt->opcode >= X_BACK && converted[0]) {
+    rc = make_error(err, 1, "pdlcpu: %s: a must have the type of c", vt->name);
+    goto cleanup;
+  }
   run_readdata(&tr);
+  if (tr.croaked) {
+    rc = make_error(err, 1, "%s", tr.croak);
+    goto cleanup;
+  }
   for (i = 0; i < npdls; i++) {
     if (tr.setgood[i]) tr.pdls[i]->state &= ~PDLC_BADVAL;
     if (tr.setbad[i]) tr.pdls[i]->state |= PDLC_BADVAL;

@@ -109,6 +109,12 @@ for _n in UFUNCS + BADOPS:
 EXPORTS["pdlhip_run_setbadif"] = ([PP, PP, PP], Error)
 EXPORTS["pdlhip_run_reduce_noalloc"] = ([ctypes.c_char_p, PP, PP], Error)
 EXPORTS["pdlhip_tune_stats"] = ([ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)], Error)
+EXPORTS["pdlhip_op_info"] = ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], Error)
+# Slices.pd index family and the dataflow writeback
+EXPORTS["pdlhip_run_index"] = ([PP, PP, PP], Error)
+EXPORTS["pdlhip_run_index1d"] = ([PP, PP, PP], Error)
+EXPORTS["pdlhip_run_index2d"] = ([PP, PP, PP, PP], Error)
+EXPORTS["pdlhip_writebackdata"] = ([ctypes.c_char_p, ctypes.POINTER(PP), ctypes.c_int], Error)
 # PDL's random-number stream
 EXPORTS["pdlhip_srandom"] = ([ctypes.c_uint64], Error)
 EXPORTS["pdlhip_rand_skip"] = ([ctypes.c_uint64], Error)

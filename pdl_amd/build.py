@@ -38,6 +38,7 @@ def units():
     u.append(("k_misc.hip", "misc", []))
     u.append(("k_badops.hip", "badops", []))
     u.append(("k_rng.hip", "rng", []))
+    u.append(("k_index.hip", "index", []))
     u.append(("k_mfma.hip", "mfma", []))
     u.append(("k_batch.hip", "batch", []))
     u.append(("k_ragged.hip", "ragged", []))

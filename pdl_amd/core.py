@@ -15,6 +15,7 @@ Perl-level wrapper logic restated here (cited where defined):
   operator overloads    lib/PDL/PP.pm:1420-1484 (swap for reversed operands)
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -74,12 +75,31 @@ def typecode(t):
 
 
 class _Buf:
-    """An HBM allocation owned by this process (released to the cache on GC)."""
-    __slots__ = ("ptr", "nbytes")
+    """An HBM allocation owned by this process (released to the cache on GC),
+    with the state PDL keeps for an ndarray's data (pdlapi.c:871-945):
+
+    version       bumped whenever an op writes the buffer (pdl_changed)
+    host          a host mirror of the whole buffer, filled by one download
+                  when an element is read (the Core.xs sync points: at, set,
+                  sclr), current while host_version == version
+    dev_stale     the mirror holds writes (set) not yet in HBM; they go up
+                  before any device op uses the buffer
+    flow          the dataflow of a child made by index / index1d / index2d
+                  (PDL keeps that trans): recomputed from its parents when
+                  they changed, written back into them when it is written
+    """
+    __slots__ = ("ptr", "nbytes", "version", "host", "host_version", "host_epoch", "dev_stale", "flow",
+                 "__weakref__")
 
     def __init__(self, ptr, nbytes):
         self.ptr = ptr
         self.nbytes = nbytes
+        self.version = 0
+        self.host = None
+        self.host_version = -1
+        self.host_epoch = -1
+        self.dev_stale = False
+        self.flow = None
 
     def __del__(self):
         if self.ptr and L._lib is not None and _inited["device"] is not None:
@@ -95,6 +115,107 @@ def _alloc(nbytes):
     p = ctypes.c_void_p()
     L.check(L.lib().pdlhip_alloc(max(int(nbytes), 1), ctypes.byref(p)))
     return _Buf(p.value, nbytes)
+
+
+# ---------------------------------------------------------------- mirrors and dataflow
+_MIRROR_MAX = 64 << 20          # buffers up to this size get a whole host mirror
+_EPOCH = [0]                    # bumped by launches that do not say which buffers they write
+_STALE = weakref.WeakSet()      # buffers whose mirror holds writes not yet uploaded
+
+
+def _launched():
+    """a list / chain / graph launch wrote buffers this module does not track:
+    every host mirror is out of date"""
+    _EPOCH[0] += 1
+
+
+def _flush(buf):
+    if buf is not None and buf.dev_stale:
+        if buf.nbytes:
+            L.check(L.lib().pdlhip_upload(buf.ptr, buf.host.ctypes.data, buf.nbytes))
+        buf.dev_stale = False
+        _STALE.discard(buf)
+
+
+def _flush_all():
+    for b in list(_STALE):
+        _flush(b)
+
+
+def _mirror(buf):
+    """the buffer's host mirror, current (one download when it is not)"""
+    if buf.dev_stale:
+        return buf.host
+    if buf.host is None or buf.host_version != buf.version or buf.host_epoch != _EPOCH[0]:
+        if buf.host is None:
+            buf.host = np.empty(buf.nbytes, np.uint8)
+        if buf.nbytes:
+            L.check(L.lib().pdlhip_download(buf.host.ctypes.data, buf.ptr, buf.nbytes))
+        buf.host_version, buf.host_epoch = buf.version, _EPOCH[0]
+    return buf.host
+
+
+class _Flow:
+    """a child's dataflow: op, its parents (a and the index ndarrays, in
+    signature order), the child over its whole buffer, and the parents'
+    buffer versions its data reflects"""
+    __slots__ = ("op", "parents", "child", "seen")
+
+    def __init__(self, op, parents, child):
+        self.op, self.parents, self.child = op, list(parents), child
+        self.seen = self.versions()
+
+    def versions(self):
+        return tuple((p._buf.version if p._buf is not None else 0) for p in self.parents)
+
+
+_OPINFO = {}
+
+
+def _op_info(op):
+    if op not in _OPINFO:
+        n, k = ctypes.c_int(0), ctypes.c_int(0)
+        L.check(L.lib().pdlhip_op_info(op.encode(), ctypes.byref(n), ctypes.byref(k)))
+        _OPINFO[op] = (n.value, k.value)
+    return _OPINFO[op]
+
+
+def _refresh(buf):
+    """before a read: a child whose parents changed is computed again
+    (PDL's make_physical of a dataflow child)"""
+    f = buf.flow if buf is not None else None
+    if f is None:
+        return
+    for p in f.parents:
+        _refresh(p._buf)
+        _flush(p._buf)
+    if f.versions() != f.seen:
+        _call(f.op, f.parents + [f.child])
+        buf.version += 1
+        f.seen = f.versions()
+
+
+def _written(buf):
+    """after a write: the buffer's version moves on and a dataflow child
+    writes itself back into its parent (writebackdata)"""
+    if buf is None:
+        return
+    buf.version += 1
+    _writeback(buf)
+
+
+def _writeback(buf):
+    f = buf.flow
+    if f is None:
+        return
+    _flush(buf)
+    for p in f.parents:
+        _flush(p._buf)
+    cs = [p._to_c() for p in f.parents + [f.child]]
+    arr = (L.PP * len(cs))(*[ctypes.pointer(c) for c in cs])
+    L.check(L.lib().pdlhip_writebackdata(f.op.encode(), arr, len(cs)))
+    _written(f.parents[0]._buf)
+    f.seen = f.versions()
 
 
 def _dense_incs(dims):
@@ -272,6 +393,62 @@ class ndarray:
         _run("assgn", [self, out])
         return out
 
+    def assign(self, other):
+        """PDL's `.=`: other's values into this ndarray (through a view into
+        its parent; a dataflow child flows back into its own)"""
+        _run("assgn", [topdl(other), self])
+        return self
+
+    def _alias(self):
+        v = ndarray()
+        v._buf, v.dtype, v.dims, v.incs, v.offs = self._buf, self.dtype, list(self.dims), list(self.incs), self.offs
+        v.state, v.badvalue_ = self.state, self.badvalue_
+        return v
+
+    def _flowing(self, op, parents):
+        c = ndarray()
+        _run(op, parents + [c])
+        c._buf.flow = _Flow(op, parents, c._alias())
+        return c
+
+    # Slices.pd:142-220 -- non-affine slices: a new ndarray with dataflow
+    # both ways (recomputed when a parent changes, written back when written)
+    def index(self, ind):
+        return self._flowing("index", [self, topdl(ind)])
+
+    def index1d(self, ind):
+        return self._flowing("index1d", [self, topdl(ind)])
+
+    def index2d(self, inda, indb):
+        return self._flowing("index2d", [self, topdl(inda), topdl(indb)])
+
+    def dice_axis(self, axis, idx):
+        """Slices.pd:2077-2082: mv(axis,0)->index1d(idx)->mv(0,axis)"""
+        ix = topdl(idx)
+        if ix.ndims > 1:
+            raise PDLError(L.EUSERERROR, "dice_axis: index must be <=1D")
+        axis = axis + self.ndims if axis < 0 else axis
+        return self.mv(axis, 0).index1d(ix).mv(0, axis)
+
+    def dice(self, *dim_indexes):
+        """Slices.pd:2001-2026: dice_axis along each dim in turn ('X' keeps a dim)"""
+        if len(dim_indexes) > self.ndims:
+            raise PDLError(L.EUSERERROR, "PDL::dice: Number of index arrays (%d) not equal to the dimensions of the "
+                           "PDL (%d" % (len(dim_indexes), self.ndims))
+        out = self
+        for k, ix in enumerate(dim_indexes):
+            if isinstance(ix, str) and ix == "X":
+                continue
+            out = out.dice_axis(k, ix)
+        return out
+
+    def sever(self):
+        """detach a dataflow child from its parents (pdlapi.c:1043-1050)"""
+        if self._buf is not None and self._buf.flow is not None:
+            _refresh(self._buf)
+            self._buf.flow = None
+        return self
+
     def make_physical(self):
         return self
 
@@ -302,6 +479,9 @@ class ndarray:
         """make_physical + copy to host; numpy shape = reversed dims"""
         if self.isnull():
             raise PDLError(L.EUSERERROR, "numpy(): ndarray is null")
+        if self._buf is not None:
+            _refresh(self._buf)
+            _flush(self._buf)
         src = self if self._is_dense() else self.copy()
         out = np.empty(src.dims[::-1], dtype=NP_TYPES[src.dtype])
         if out.nbytes:
@@ -309,13 +489,45 @@ class ndarray:
         return out
 
     def at(self, *idx):
+        """one element (a Core.xs sync point): read from the buffer's host
+        mirror, which one download fills while the device has not written
+        the buffer since (buffers above 64 MiB: the element alone)"""
         if len(idx) != self.ndims or any(not 0 <= i < d for i, d in zip(idx, self.dims)):
             raise PDLError(L.EUSERERROR, "Position %s out of range" % (list(idx),))
         off = self.offs + sum(i * c for i, c in zip(idx, self.incs))
-        out = np.empty(1, dtype=NP_TYPES[self.dtype])
-        size = out.itemsize
-        L.check(L.lib().pdlhip_download(out.ctypes.data, self._buf.ptr + off * size, size))
+        dt = np.dtype(NP_TYPES[self.dtype])
+        _refresh(self._buf)
+        if self._buf.nbytes <= _MIRROR_MAX:
+            return np.frombuffer(_mirror(self._buf), dtype=dt, count=1, offset=off * dt.itemsize)[0]
+        _flush(self._buf)
+        out = np.empty(1, dtype=dt)
+        L.check(L.lib().pdlhip_download(out.ctypes.data, self._buf.ptr + off * dt.itemsize, dt.itemsize))
         return out[0]
+
+    def set(self, *args):
+        """PDL's set(i, j, ..., value): one element written through the host
+        mirror (uploaded before the next device op reads the buffer); a
+        dataflow child writes itself back into its parent at once"""
+        *idx, v = args
+        if len(idx) != self.ndims or any(not 0 <= i < d for i, d in zip(idx, self.dims)):
+            raise PDLError(L.EUSERERROR, "Position %s out of range" % (list(idx),))
+        off = self.offs + sum(i * c for i, c in zip(idx, self.incs))
+        dt = np.dtype(NP_TYPES[self.dtype])
+        val = np.array([v]).astype(dt)
+        buf = self._buf
+        _refresh(buf)
+        if buf.nbytes <= _MIRROR_MAX:
+            h = _mirror(buf)
+            h[off * dt.itemsize:(off + 1) * dt.itemsize] = val.view(np.uint8)
+            buf.version += 1
+            buf.host_version, buf.host_epoch = buf.version, _EPOCH[0]
+            buf.dev_stale = True
+            _STALE.add(buf)
+        else:
+            L.check(L.lib().pdlhip_upload(buf.ptr + off * dt.itemsize, val.ctypes.data, dt.itemsize))
+            buf.version += 1
+        _writeback(buf)
+        return self
 
     def sclr(self):
         if self.nelem < 1:
@@ -559,6 +771,7 @@ def null():
 def _fill_const(p, v):
     c = p._to_c()
     L.check(L.lib().pdlhip_fill_uniform(ctypes.byref(c), 0, float(v), float(v)))
+    _written(p._buf)
     return p
 
 
@@ -603,8 +816,11 @@ def rand_state():
 
 
 def _rand_into(x, entry):
+    _refresh(x._buf)
+    _flush(x._buf)
     c = x._to_c()
     L.check(getattr(L.lib(), entry)(ctypes.byref(c)))
+    _written(x._buf)
     return x
 
 
@@ -651,10 +867,28 @@ def grandom_into(x):
 
 def _run(op, pdls, swap=0, m_size=None, dpar=None):
     """One pdl_run_<op> call through the C ABI (dpar: a double OtherPar,
-    passed as its IEEE bit pattern)."""
+    passed as its IEEE bit pattern), with the data kept in step: inputs that
+    are dataflow children are recomputed if their parents changed, host
+    writes go up first, written buffers move on and write dataflow children
+    back into their parents."""
     _need()
     if op == "assgn_convert":
         op = "assgn"
+    _npdls, npar = _op_info(op)
+    for p in pdls:
+        if p._buf is not None:
+            _refresh(p._buf)
+            _flush(p._buf)
+    _call(op, pdls, swap, m_size, dpar)
+    seen = set()
+    for p in pdls[npar:]:
+        if p._buf is not None and id(p._buf) not in seen:
+            seen.add(id(p._buf))
+            _written(p._buf)
+
+
+def _call(op, pdls, swap=0, m_size=None, dpar=None):
+    """the bare C ABI call (no dataflow bookkeeping)"""
     uniq, cs = {}, []
     for p in pdls:
         if id(p) not in uniq:
@@ -718,7 +952,9 @@ def matmult_list(As, Bs, Cs=None, mode="queue"):
     pa = (L.PP * n)(*[ctypes.pointer(x) for x in ca])
     pb = (L.PP * n)(*[ctypes.pointer(x) for x in cb])
     pc = (L.PP * n)(*[ctypes.pointer(x) for x in cc])
+    _flush_all()
     L.check(L.lib().pdlhip_run_matmult_list(n, pa, pb, pc, {"queue": 0, "each": 1}[mode]))
+    _launched()
     for c, s, cr in zip(Cs, cc, created):
         c._from_c(s, cr)
     return Cs
@@ -736,7 +972,9 @@ class _Prepared:
     _h = None
 
     def run(self):
+        _flush_all()
         L.check(L.lib().pdlhip_ragged_run(self._h))
+        _launched()
         return self
 
     def _flags(self):
@@ -888,7 +1126,9 @@ class Graph:
         self._g = g
 
     def launch(self):
+        _flush_all()
         L.check(L.lib().pdlhip_graph_launch(self._g))
+        _launched()
 
     def destroy(self):
         if self._g:
@@ -930,7 +1170,9 @@ class MatmultList:
             o._from_c(s, cr)
 
     def launch(self, mode="queue"):
+        _flush_all()
         L.check(L.lib().pdlhip_matmult_list_launch(self._h, {"queue": 0, "each": 1}[mode]))
+        _launched()
         return self
 
     def free(self):

@@ -50,6 +50,7 @@ static thread_local std::string g_msg;
 constexpr size_t NOALLOC_BYTES = (size_t)256 << 10;
 static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
 static thread_local bool g_noalloc = false;   // inside pdlhip_run_reduce_noalloc
+static thread_local bool g_writeback = false; // inside pdlhip_writebackdata: run the op's BackCode
 
 static pdlhip_error ok() { return pdlhip_error{PDLHIP_ENONE, nullptr}; }
 
@@ -223,7 +224,7 @@ void *scratch_alloc(size_t bytes) { return dev_alloc(bytes); }
 void scratch_free(void *p) { dev_free(p); }
 
 // ------------------------------------------------------------------ signatures
-enum Family { FAM_BIOP, FAM_BIFUNC, FAM_ASSGN, FAM_RED, FAM_NIND, FAM_MM, FAM_UFUNC, FAM_BADOP };
+enum Family { FAM_BIOP, FAM_BIFUNC, FAM_ASSGN, FAM_RED, FAM_NIND, FAM_MM, FAM_UFUNC, FAM_BADOP, FAM_INDEX };
 enum ParFlag { P_OUT = 1, P_CREAT = 2, P_TYPED = 4, P_TPLUS = 8 };
 
 struct Sig {
@@ -335,6 +336,15 @@ static std::vector<Sig> build_sigs() {
   bo("setbadtoval", B_SETBADTOVAL, GT_ALL, 0, -1);
   v.push_back(Sig{"setbadif", 3, 2, {"a", "mask", "b"}, {0, 0, 0}, {{}, {}, {}}, {}, {0, P_TYPED, P_OUT | P_CREAT},
                   {-1, PDLHIP_L, -1}, GT_ALL, FAM_BADOP, B_SETBADIF});
+  // Slices.pd:149-220: index a(n); indx ind(); [oca] c()  /  index1d a(n);
+  // indx ind(m); [oca] c(m)  /  index2d a(na,nb); indx inda(); indx indb();
+  // [oca] c()  (GenericTypes ppdefs_all)
+  v.push_back(Sig{"index", 3, 2, {"a", "ind", "c"}, {1, 0, 0}, {{0}, {}, {}}, {"n"}, {0, P_TYPED, P_OUT | P_CREAT},
+                  {-1, PDLHIP_IND, -1}, GT_ALL, FAM_INDEX, 0});
+  v.push_back(Sig{"index1d", 3, 2, {"a", "ind", "c"}, {1, 1, 1}, {{0}, {1}, {1}}, {"n", "m"},
+                  {0, P_TYPED, P_OUT | P_CREAT}, {-1, PDLHIP_IND, -1}, GT_ALL, FAM_INDEX, 1});
+  v.push_back(Sig{"index2d", 4, 3, {"a", "inda", "indb", "c"}, {2, 0, 0, 0}, {{0, 1}, {}, {}, {}}, {"na", "nb"},
+                  {0, P_TYPED, P_TYPED, P_OUT | P_CREAT}, {-1, PDLHIP_IND, PDLHIP_IND, -1}, GT_ALL, FAM_INDEX, 2});
   return v;
 }
 
@@ -686,11 +696,14 @@ __global__ void zero_i32(int *p, int n) {
   if (i < n) p[i] = 0;
 }
 
+static pdlhip_error readdata_index(const Sig &s, int T, bool bad, const Brc &b, pdlhip_pdl **pdls);
+
 // the readdata of one op over a planned broadcast (the pdlhip_readdata body)
 static pdlhip_error readdata(const Sig &s, int T, bool bad, const Brc &b, pdlhip_pdl **pdls, const int64_t *other) {
   if (g.plan_only) return ok();
   if (!type_on_device(T)) return make_err(PDLHIP_EUSERERROR, "pdlhip: %s: type %s is not supported on the device",
                                           s.name.c_str(), type_name(T));
+  if (s.family == FAM_INDEX) return readdata_index(s, T, bad, b, pdls);
   hipStream_t st = g.stream;
   hipError_t e = hipSuccess;
   const bool mixed = (s.family == FAM_BIOP || s.family == FAM_BIFUNC || s.family == FAM_UFUNC) &&
@@ -725,6 +738,11 @@ static pdlhip_error readdata(const Sig &s, int T, bool bad, const Brc &b, pdlhip
     std::memset(&m, 0, sizeof m);
     m.a = pa->data; m.ta = pa->datatype; m.sbad_a = (pa->state & PDLHIP_BADVAL) != 0; m.sba = badinfo(pa, 1);
     m.b = pb->data; m.tb = pb->datatype; m.sbad_b = (pb->state & PDLHIP_BADVAL) != 0; m.sbb = badinfo(pb, 1);
+    m.a_scalar = m.b_scalar = 1;
+    for (int k = 0; k < p.nd; k++) {
+      if (p.inc[0][k] != 0 && p.dims[k] > 1) m.a_scalar = 0;
+      if (p.inc[1][k] != 0 && p.dims[k] > 1) m.b_scalar = 0;
+    }
     // the op's bad test on the converted values: the copies' bad value is
     // the default of T; a biop tests an input only if it is bad-flagged
     BadInfo bi[3];
@@ -945,6 +963,100 @@ static pdlhip_error readdata(const Sig &s, int T, bool bad, const Brc &b, pdlhip
   return ok();
 }
 
+__global__ void fill_u64(unsigned long long *p, unsigned long long v) { p[threadIdx.x] = v; }
+
+// Slices.pd index / index1d / index2d: Code (gather), or BackCode (scatter
+// into a) under pdlhip_writebackdata; a $CROAK position found by the kernel
+// becomes PDL's message, worded from the index read back at that position
+static pdlhip_error readdata_index(const Sig &s, int T, bool bad, const Brc &b, pdlhip_pdl **pdls) {
+  if (capturing())
+    return make_err(PDLHIP_EUSERERROR, "pdlhip: %s checks its indices on the host and cannot be captured in a graph",
+                    s.name.c_str());
+  IxPlan ip;
+  std::memset(&ip, 0, sizeof ip);
+  ip.op = s.op;
+  ip.back = g_writeback ? 1 : 0;
+  ip.bad = bad ? 1 : 0;
+  const int ci = s.npdls - 1;
+  const int slot[4] = {0, 1, s.op == 2 ? 2 : -1, ci};
+  ip.npos = 1;
+  for (int k = 0; k < b.ndims; k++) {
+    if (b.dims[k] == 1) continue;
+    ip.dims[ip.nd] = b.dims[k];
+    for (int q = 0; q < 4; q++) ip.inc[q][ip.nd] = slot[q] >= 0 ? b.incs[k * b.npdls + slot[q]] : 0;
+    ip.npos *= b.dims[k];
+    ip.nd++;
+  }
+  for (int q = 0; q < 4; q++) ip.off[q] = slot[q] >= 0 ? b.offs[slot[q]] : 0;
+  ip.m = s.op == 1 ? b.ind_sizes[1] : 1;
+  ip.na = b.ind_sizes[0];
+  ip.nb2 = s.op == 2 ? b.ind_sizes[1] : 0;
+  ip.ia0 = b.inc_sizes[0][0];
+  ip.ia1 = s.op == 2 ? b.inc_sizes[0][1] : 0;
+  ip.iim = s.op == 1 ? b.inc_sizes[1][0] : 0;
+  ip.icm = s.op == 1 ? b.inc_sizes[2][0] : 0;
+  for (int k = 0; k < (s.op == 2 ? 2 : 1); k++) {
+    const pdlhip_value v = badinfo(pdls[1 + k], 1).bv;
+    std::memcpy(&ip.ind_bv[k], &v, sizeof(int64_t));
+  }
+  if (ip.npos == 0 || ip.m == 0) return ok();
+  const pdlhip_pdl *a = pdls[0];
+  long long *owner = nullptr;
+  if (ip.back) {
+    int64_t lo = a->offs, hi = a->offs;
+    for (int d = 0; d < a->ndims; d++) {
+      if (a->dims[d] <= 0) return ok();
+      const int64_t ext = (a->dims[d] - 1) * a->incs[d];
+      if (ext < 0) lo += ext; else hi += ext;
+    }
+    ip.a_lo = lo;
+    ip.a_span = hi - lo + 1;
+    owner = (long long *)dev_alloc((size_t)ip.a_span * sizeof(long long));
+    if (!owner) return oom_err(s.name.c_str());
+  }
+  unsigned long long *err = (unsigned long long *)dev_alloc(sizeof(unsigned long long));
+  if (!err) {
+    dev_free(owner);
+    return oom_err(s.name.c_str());
+  }
+  hipStream_t st = g.stream;
+  hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, st, err, ~0ull);
+  hipError_t e = launch_index(T, ip, pdls[0]->data, (const int64_t *)pdls[1]->data,
+                              s.op == 2 ? (const int64_t *)pdls[2]->data : nullptr, pdls[ci]->data,
+                              badinfo(pdls[ci], 1).bv, err, owner, st);
+  unsigned long long pos = ~0ull;
+  if (e == hipSuccess) e = hipMemcpyAsync(&pos, err, sizeof pos, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  dev_free(err);
+  dev_free(owner);
+  if (e != hipSuccess) return hip_err(e, s.name.c_str());
+  if (pos == ~0ull) return ok();
+  // the loop position PDL croaks at: read its index (or indices)
+  int64_t m = (int64_t)pos % ip.m, r = (int64_t)pos / ip.m, o1 = ip.off[1], o2 = ip.off[2];
+  for (int d = 0; d < ip.nd; d++) {
+    const int64_t i = r % ip.dims[d];
+    r /= ip.dims[d];
+    o1 += i * ip.inc[1][d];
+    o2 += i * ip.inc[2][d];
+  }
+  int64_t x[2] = {0, 0};
+  e = hipMemcpy(&x[0], (const int64_t *)pdls[1]->data + o1 + m * ip.iim, sizeof(int64_t), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && s.op == 2)
+    e = hipMemcpy(&x[1], (const int64_t *)pdls[2]->data + o2, sizeof(int64_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_err(e, s.name.c_str());
+  if (s.op == 0)
+    return make_err(PDLHIP_EUSERERROR, "Error in index:invalid index %lld (valid range 0..%lld)", (long long)x[0],
+                    (long long)(ip.na - 1));
+  if (s.op == 1)
+    return make_err(PDLHIP_EUSERERROR, "Error in index1d:invalid index %lld at pos %lld (valid range 0..%lld)",
+                    (long long)x[0], (long long)m, (long long)(ip.na - 1));
+  if ((bad && x[0] == ip.ind_bv[0]) || x[0] < 0 || x[0] >= ip.na)
+    return make_err(PDLHIP_EUSERERROR, "Error in index2d:invalid x-index %lld (valid range 0..%lld)",
+                    (long long)x[0], (long long)(ip.na - 1));
+  return make_err(PDLHIP_EUSERERROR, "Error in index2d:invalid y-index %lld (valid range 0..%lld)", (long long)x[1],
+                  (long long)(ip.nb2 - 1));
+}
+
 // dense temp of type `t` shaped like `p` (the converted pdl of
 // pdl__get_convertedpdl_recprotect)
 static pdlhip_error make_converted(const pdlhip_pdl *p, int t, pdlhip_pdl *out, bool copy_in) {
@@ -1052,7 +1164,9 @@ static pdlhip_error run(const Sig &s, pdlhip_pdl **user, int npdls, const int64_
   for (int i = 0; i < npdls; i++) orig[i] = pdls[i];
   pdlhip_error err = ok();
   // elementwise ops read an input of another type directly and convert it
-  // in registers (k_mixed.hip): no copy, unless it is also an output
+  // in registers (k_mixed.hip): no copy, unless it is also an output; an
+  // input of a few elements (a Perl scalar such as the 2 in x * 2) is
+  // converted into a tiny copy instead, so the op keeps the same-type kernels
   const bool ew = s.family == FAM_BIOP || s.family == FAM_BIFUNC || s.family == FAM_UFUNC;
   for (int i = npdls - 1; i >= 0 && !err.error; i--) {
     if (creating[i] || pdls[i]->datatype == req[i]) continue;
@@ -1064,7 +1178,9 @@ static pdlhip_error run(const Sig &s, pdlhip_pdl **user, int npdls, const int64_
     if (ew && i < s.nparents) {
       bool inout = false;
       for (int j = s.nparents; j < npdls; j++) inout = inout || orig[j] == orig[i];
-      if (!inout) continue;
+      int64_t nel = 1;
+      for (int d = 0; d < pdls[i]->ndims; d++) nel *= pdls[i]->dims[d];
+      if (!inout && nel > 64) continue;
     }
     int dup = -1;
     for (int j = i + 1; j < npdls; j++)
@@ -1846,6 +1962,14 @@ pdlhip_error pdlhip_synchronize(void) {
 
 void *pdlhip_stream(void) { return (void *)g.stream; }
 
+pdlhip_error pdlhip_op_info(const char *op, int *npdls, int *nparents) {
+  const Sig *s = op ? find_sig(op) : nullptr;
+  if (!s) return make_err(PDLHIP_EUSERERROR, "pdlhip: unknown op '%s'", op ? op : "(null)");
+  if (npdls) *npdls = s->npdls;
+  if (nparents) *nparents = s->nparents;
+  return ok();
+}
+
 pdlhip_error pdlhip_run(const char *op, pdlhip_pdl **pdls, int npdls, const int64_t *otherpars) {
   const Sig *s = op ? find_sig(op) : nullptr;
   if (!s) return make_err(PDLHIP_EUSERERROR, "pdlhip: unknown op '%s'", op ? op : "(null)");
@@ -1970,6 +2094,45 @@ UNARY_ENTRY(setnantobad)
 UNARY_ENTRY(setinftobad)
 UNARY_ENTRY(setnonfinitetobad)
 UNARY_ENTRY(setbadtonan)
+
+pdlhip_error pdlhip_run_index(pdlhip_pdl *a, pdlhip_pdl *ind, pdlhip_pdl *c) {
+  pdlhip_pdl *v[3] = {a, ind, c};
+  return pdlhip_run("index", v, 3, nullptr);
+}
+pdlhip_error pdlhip_run_index1d(pdlhip_pdl *a, pdlhip_pdl *ind, pdlhip_pdl *c) {
+  pdlhip_pdl *v[3] = {a, ind, c};
+  return pdlhip_run("index1d", v, 3, nullptr);
+}
+pdlhip_error pdlhip_run_index2d(pdlhip_pdl *a, pdlhip_pdl *inda, pdlhip_pdl *indb, pdlhip_pdl *c) {
+  pdlhip_pdl *v[4] = {a, inda, indb, c};
+  return pdlhip_run("index2d", v, 4, nullptr);
+}
+
+// the vtable's writebackdata for the ops with a BackCode (Slices.pd index,
+// index1d, index2d): every ndarray exists, a and c already have the type the
+// op computes in; a is written, the others are left as they are
+pdlhip_error pdlhip_writebackdata(const char *op, pdlhip_pdl **pdls, int npdls) {
+  const Sig *s = op ? find_sig(op) : nullptr;
+  if (!s || s->family != FAM_INDEX)
+    return make_err(PDLHIP_EUSERERROR, "pdlhip_writebackdata: '%s' has no BackCode here (index, index1d, index2d)",
+                    op ? op : "(null)");
+  if (!pdls || npdls != s->npdls)
+    return make_err(PDLHIP_EUSERERROR, "pdlhip_writebackdata: %s takes %d ndarrays", op, s->npdls);
+  for (int i = 0; i < npdls; i++)
+    if (!pdls[i] || (pdls[i]->state & PDLHIP_NOMYDIMS))
+      return make_err(PDLHIP_EUSERERROR, "pdlhip_writebackdata: %s: parameter '%s' is null", op,
+                      s->par_names[i].c_str());
+  const int T = select_transtype(*s, pdls);
+  pdlhip_pdl *a = pdls[0], *c = pdls[npdls - 1];
+  if (a->datatype != T || c->datatype != T)
+    return make_err(PDLHIP_EUSERERROR, "pdlhip_writebackdata: %s: a and c must both be %s", op, type_name(T));
+  const int32_t cstate = c->state;
+  g_writeback = true;
+  pdlhip_error e = run(*s, pdls, npdls, nullptr);
+  g_writeback = false;
+  c->state = cstate;
+  return e;
+}
 
 pdlhip_error pdlhip_run_setbadif(pdlhip_pdl *a, pdlhip_pdl *mask, pdlhip_pdl *b) {
   pdlhip_pdl *v[3] = {a, mask, b};

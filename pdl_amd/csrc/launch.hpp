@@ -49,6 +49,7 @@ struct MixedArgs {
   int32_t ta, tb;
   int32_t sbad_a, sbad_b;
   BadInfo sba, sbb;
+  int32_t a_scalar, b_scalar;   // every increment of the operand is 0: one element
 };
 #define X(TC) hipError_t launch_mixed_t##TC(int op, const MixedArgs &m, const EwPlan &p, void *c, bool bad, \
                                             const BadInfo *bi, hipStream_t s);
@@ -90,6 +91,25 @@ const uint64_t *rng_jump_table(size_t *nwords);          // the in-chunk jumps, 
 hipError_t launch_rng(int T, int kind, const RngPlan &p, void *out, const uint64_t *d_states, const uint64_t *d_jumps,
                       int64_t nchunks, int *d_flag, hipStream_t s);
 constexpr int64_t RNG_CHUNK = 4096;
+
+// Slices.pd index / index1d / index2d (k_index.hip): op 0/1/2, back = the
+// BackCode (scatter into a).  Pointer slots: 0 a, 1 first index, 2 second
+// index (index2d), 3 c; broadcast dims in PDL's loop order, m innermost.
+struct IxPlan {
+  int32_t op, back, nd, bad;
+  int64_t npos;                         // broadcast positions
+  int64_t m;                            // index1d: the m extent; 1 otherwise
+  int64_t dims[PDLHIP_MAXDIMS + 2];
+  int64_t inc[4][PDLHIP_MAXDIMS + 2];
+  int64_t off[4];
+  int64_t na, nb2;                      // a's core extents (n; na, nb)
+  int64_t ia0, ia1;                     // a's core strides
+  int64_t iim, icm;                     // index1d: ind's and c's strides along m
+  int64_t ind_bv[2];                    // the indices' bad values
+  int64_t a_lo, a_span;                 // BackCode: the owner table spans a's elements [a_lo, a_lo + a_span)
+};
+hipError_t launch_index(int T, const IxPlan &p, void *a, const int64_t *i1, const int64_t *i2, void *c,
+                        const pdlhip_value &cbv, unsigned long long *err, long long *owner, hipStream_t s);
 
 hipError_t launch_fill(int dtype, const EwPlan &p, void *dst, uint64_t seed, double a, double b, int normal,
                        hipStream_t s);
