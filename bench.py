@@ -516,6 +516,27 @@ def run_single(args):
     if args.only == "lists":
         print(json.dumps({"ragged_matmult": run_ragged_matmult()}))
         return
+    if args.only == "listq":
+        # the two prepared lists' queue launches alone, for counter runs
+        C.srandom(0)
+        ns = 8 + np.floor(C.random(4096).numpy() * 249).astype(np.int64)
+        tot = int(np.sum(ns ** 2))
+        for t in (C.L.F, C.L.D):
+            A, B = uniform_pm1(C, tot, dtype=t), uniform_pm1(C, tot, dtype=t)
+            Cb = C.ndarray.new_dense(t, [tot])
+            As, Bs, Cs, off = [], [], [], 0
+            for n in ns:
+                n = int(n)
+                As.append(C.view_into(A, [n, n], off)); Bs.append(C.view_into(B, [n, n], off))
+                Cs.append(C.view_into(Cb, [n, n], off))
+                off += n * n
+            ml = C.MatmultList(As, Bs, Cs)
+            for _ in range(args.steps):
+                ml.launch("queue")
+            C.synchronize()
+            ml.free()
+        print(json.dumps({"only": args.only, "steps": args.steps}))
+        return
     if args.only in ("mm16", "mm8"):
         C.set_matmult_precision("split16" if args.only == "mm16" else "fp8")
         C.srandom(21)

@@ -1395,17 +1395,32 @@ static pdlhip_error list_build(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlhip_pdl
   std::vector<int> order(nit);
   for (int i = 0; i < nit; i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return h.d.items[x].T > h.d.items[y].T; });
-  auto build_work = [&](int tile, std::vector<int3> &work) {
-    work.clear();
+  // 8 bins, one per XCD (the kernels' list_next): each product, largest
+  // inner dimension first, goes whole to the bin with the least work so far
+  // (tiles x k); the bins lie back to back, bounds[x] .. bounds[x + 1]
+  auto build_work = [&](int tile, std::vector<int3> &work, unsigned *bounds) {
+    std::vector<std::vector<int3>> bins(8);
+    std::vector<double> load(8, 0.0);
     for (int i : order) {
       const MmItem &it = h.d.items[i];
       int64_t ntm = (it.H + tile - 1) / tile, ntn = (it.W + tile - 1) / tile;
+      const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[x] += (double)(ntm * ntn) * (double)std::max<int64_t>(it.T, 1);
       for (int64_t tm = 0; tm < ntm; tm++)
-        for (int64_t tn = 0; tn < ntn; tn++) work.push_back(make_int3(i, (int)tm, (int)tn));
+        for (int64_t tn = 0; tn < ntn; tn++) bins[x].push_back(make_int3(i, (int)tm, (int)tn));
     }
+    work.clear();
+    for (int x = 0; x < 8; x++) {
+      bounds[x] = (unsigned)work.size();
+      work.insert(work.end(), bins[x].begin(), bins[x].end());
+    }
+    bounds[8] = (unsigned)work.size();
   };
-  build_work(32, h.work32);
-  build_work(64, h.work64);
+  // the device block: 8 bin counters 128 bytes apart, then the 64-tile
+  // list's bounds, then the 32-tile list's
+  unsigned qinit[256 + 9 + 9] = {0};
+  build_work(64, h.work64, qinit + 256);
+  build_work(32, h.work32, qinit + 265);
   // the matrix-core kernel indexes within a product with 32-bit offsets
   auto span = [](int64_t n1, int64_t i1, int64_t n2, int64_t i2) {
     return (n1 > 0 ? n1 - 1 : 0) * (i1 < 0 ? -i1 : i1) + (n2 > 0 ? n2 - 1 : 0) * (i2 < 0 ? -i2 : i2);
@@ -1416,7 +1431,7 @@ static pdlhip_error list_build(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlhip_pdl
         it.H >= ((int64_t)1 << 30) || it.W >= ((int64_t)1 << 30) || it.T >= ((int64_t)1 << 30))
       h.mfma = false;
   h.d_items = (MmItem *)dev_alloc(sizeof(MmItem) * nit);
-  h.d_counter = (unsigned int *)dev_alloc(sizeof(unsigned int));
+  h.d_counter = (unsigned int *)dev_alloc(sizeof qinit);
   h.d_w32 = (int3 *)dev_alloc(sizeof(int3) * std::max<size_t>(h.work32.size(), 1));
   h.d_w64 = (int3 *)dev_alloc(sizeof(int3) * std::max<size_t>(h.work64.size(), 1));
   if (!h.d_items || !h.d_counter || !h.d_w32 || !h.d_w64) {
@@ -1424,14 +1439,16 @@ static pdlhip_error list_build(int n, pdlhip_pdl **a, pdlhip_pdl **b, pdlhip_pdl
     return oom_err("matmult list");
   }
   hipError_t e = upload_now(h.d_items, h.d.items.data(), sizeof(MmItem) * nit);
+  if (e == hipSuccess) e = upload_now(h.d_counter, qinit, sizeof qinit);
   if (e == hipSuccess) e = upload_now(h.d_w32, h.work32.data(), sizeof(int3) * h.work32.size());
   if (e == hipSuccess) e = upload_now(h.d_w64, h.work64.data(), sizeof(int3) * h.work64.size());
   // tile by the autotuner: 32 / 64 SIMT tiles, the FP32/FP64 matrix-core
-  // kernel (0) and, for float, the BF16 three-way split (1); timed on a copy
+  // kernel (0; 2: half the k per LDS stage) and, for float, the BF16
+  // three-way split (1); timed on a copy
   // of the item table whose outputs all point into one scratch block
   if (e == hipSuccess) {
     std::vector<int> cands = {32, 64};
-    if (h.mfma) cands.push_back(0);
+    if (h.mfma) { cands.push_back(0); cands.push_back(2); }
     if (h.mfma && h.d.T == PDLHIP_F) cands.push_back(1);
     const int dflt = h.mfma ? 0 : 64;
     char fam[160];

@@ -81,27 +81,61 @@ __device__ inline void list_tile(const MmItem &it, int64_t tm, int64_t tn, T (*A
   }
 }
 
+// The work list holds LIST_BINS bins back to back, bin x = work[bounds[x]
+// .. bounds[x + 1]), one per XCD: the host spreads the products over the
+// bins by their work (largest first, each to the lightest bin), so all the
+// tiles of a product sit in one bin, in order.  Workgroups go to the 8 XCDs
+// round-robin, so block b starts on bin b % 8 and the tiles of a product,
+// taken by neighbouring blocks of one XCD at about the same time, re-read
+// its A and B panels from that XCD's L2 rather than from HBM; a block whose
+// bin is empty goes on to the next bins.  counter[x * LIST_CSTRIDE] is bin
+// x's counter, each in a 128-byte line of its own (reset by a kernel before
+// each launch); the bounds follow the counters.
+constexpr int LIST_BINS = 8, LIST_CSTRIDE = 32, LIST_BOUNDS = LIST_BINS * LIST_CSTRIDE;
+
 __global__ void list_reset(unsigned int *counter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
+  if (threadIdx.x < LIST_BINS && blockIdx.x == 0) counter[threadIdx.x * LIST_CSTRIDE] = 0u;
 }
 
-// persistent: take work items until the counter passes nwork
+// the block's next work item (false: every bin is empty); thread 0 takes it
+// and hands it on through LDS
+__device__ __forceinline__ bool list_next(unsigned int *counter, const unsigned int *__restrict__ bounds,
+                                          const int3 *__restrict__ work, int &bin, unsigned &done, unsigned *sh,
+                                          int3 &job) {
+  if (threadIdx.x == 0) {
+    unsigned w = ~0u;
+    while (done != (1u << LIST_BINS) - 1u) {
+      const unsigned i = atomicAdd(counter + bin * LIST_CSTRIDE, 1u);
+      if (bounds[bin] + i < bounds[bin + 1]) {
+        w = bounds[bin] + i;
+        break;
+      }
+      done |= 1u << bin;
+      bin = (bin + 1) % LIST_BINS;
+    }
+    *sh = w;
+  }
+  __syncthreads();
+  const unsigned w = *sh;
+  __syncthreads();
+  if (w == ~0u) return false;
+  job = work[w];
+  return true;
+}
+
+// persistent: take work items until every bin is empty
 template <class T, bool FUSED, int TS>
 __global__ __launch_bounds__(256) void list_queue(const MmItem *__restrict__ items, const int3 *__restrict__ work,
-                                                  unsigned int nwork, unsigned int *counter) {
+                                                  const unsigned int *__restrict__ bounds, unsigned int *counter) {
   constexpr int KS = 16;
   __shared__ T As[KS][TS + 1];
   __shared__ T Bs[KS][TS + 1];
   __shared__ unsigned int next;
-  for (;;) {
-    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
-    __syncthreads();
-    const unsigned int w = next;
-    __syncthreads();
-    if (w >= nwork) break;
-    const int3 job = work[w];
+  int bin = blockIdx.x % LIST_BINS;
+  unsigned done = 0;
+  int3 job;
+  while (list_next(counter, bounds, work, bin, done, &next, job))
     list_tile<T, FUSED, TS>(items[job.x], job.y, job.z, As, Bs);
-  }
 }
 
 // one launch per item (the comparison the list path is measured against)
@@ -133,10 +167,9 @@ template <> struct ListMfma<double> { static constexpr int KS = 16; };
 typedef float lf32x16 __attribute__((ext_vector_type(16)));
 typedef double ld4 __attribute__((ext_vector_type(4)));
 
-template <class T>
-__device__ __forceinline__ void list_mfma_tile(const MmItem &it, int64_t tm, int64_t tn,
-                                      T (*As)[ListMfma<T>::KS][65], T (*Bs)[ListMfma<T>::KS][65]) {
-  constexpr int KS = ListMfma<T>::KS;
+template <class T, int KS>
+__device__ __forceinline__ void list_mfma_tile(const MmItem &it, int64_t tm, int64_t tn, T (*As)[KS][65],
+                                               T (*Bs)[KS][65]) {
   constexpr int NE = KS / 4;            // staged elements per thread per operand (64 x KS / 256)
   constexpr int SK = 256 / KS;          // rows apart of a thread's elements when it runs along k
   // a product's own extent fits 32-bit offsets (checked by the host)
@@ -153,10 +186,24 @@ __device__ __forceinline__ void list_mfma_tile(const MmItem &it, int64_t tm, int
   const int fa = a_kfast ? tid % KS : tid % 64, sa = a_kfast ? tid / KS : tid / 64, da = a_kfast ? SK : 4;
   const int fb = b_wfast ? tid % 64 : tid % KS, sb = b_wfast ? tid / 64 : tid / KS, db = b_wfast ? 4 : SK;
   const bool active = h0 + 32 * wr < H && w0 + 32 * wc < W;
+  // a chunk wholly inside both operands loads without clamps or masks: one
+  // base address per operand, the thread's elements a constant stride apart
+  const bool inside_hw = h0 + 64 <= H && w0 + 64 <= W;
+  const int a_step = da * (a_kfast ? a_h : a_t), b_step = db * (b_wfast ? b_t : b_w);
   T ra[NE], rb[NE];
-  // loads clamped into the operand (the element is zeroed when staged), so
-  // no load sits behind a branch
+  // edge chunks: loads clamped into the operand (the element is zeroed when
+  // staged), so no load sits behind a branch
   auto load = [&](int k0) {
+    if (inside_hw && k0 + KS <= K) {
+      const T *pa = a + (a_kfast ? (k0 + fa) * a_t + (h0 + sa) * a_h : (k0 + sa) * a_t + (h0 + fa) * a_h);
+      const T *pb = b + (b_wfast ? (w0 + fb) * b_w + (k0 + sb) * b_t : (w0 + sb) * b_w + (k0 + fb) * b_t);
+#pragma unroll
+      for (int q = 0; q < NE; q++) {
+        ra[q] = pa[q * a_step];
+        rb[q] = pb[q * b_step];
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < NE; q++) {
       const int s1 = sa + da * q, s2 = sb + db * q;
@@ -167,13 +214,14 @@ __device__ __forceinline__ void list_mfma_tile(const MmItem &it, int64_t tm, int
     }
   };
   auto stage = [&](int buf, int k0) {
+    const bool inside = inside_hw && k0 + KS <= K;
 #pragma unroll
     for (int q = 0; q < NE; q++) {
       const int s1 = sa + da * q, s2 = sb + db * q;
       const int mm = a_kfast ? s1 : fa, kk = a_kfast ? fa : s1;
       const int nn = b_wfast ? fb : s2, kb = b_wfast ? s2 : fb;
-      As[buf][kk][mm] = (h0 + mm < H && k0 + kk < K) ? ra[q] : T(0);
-      Bs[buf][kb][nn] = (w0 + nn < W && k0 + kb < K) ? rb[q] : T(0);
+      As[buf][kk][mm] = (inside || (h0 + mm < H && k0 + kk < K)) ? ra[q] : T(0);
+      Bs[buf][kb][nn] = (inside || (w0 + nn < W && k0 + kb < K)) ? rb[q] : T(0);
     }
   };
   if constexpr (sizeof(T) == 4) {
@@ -186,13 +234,15 @@ __device__ __forceinline__ void list_mfma_tile(const MmItem &it, int64_t tm, int
       stage(buf, k0);
       __syncthreads();
       if (k0 + KS < K) load(k0 + KS);
-      if (active) {
+      // every wave multiplies, also one whose quarter lies outside the
+      // product (its LDS rows are zeros and it stores nothing): a branch
+      // here makes the compiler copy the accumulators out of AGPRs and back
+      // each chunk
 #pragma unroll
-        for (int kk = 0; kk < KS; kk += 2) {
-          const float av = As[buf][kk + (lane >> 5)][32 * wr + (lane & 31)];
-          const float bv = Bs[buf][kk + (lane >> 5)][32 * wc + (lane & 31)];
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
-        }
+      for (int kk = 0; kk < KS; kk += 2) {
+        const float av = As[buf][kk + (lane >> 5)][32 * wr + (lane & 31)];
+        const float bv = Bs[buf][kk + (lane >> 5)][32 * wc + (lane & 31)];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
       }
       buf ^= 1;
     }
@@ -216,20 +266,18 @@ __device__ __forceinline__ void list_mfma_tile(const MmItem &it, int64_t tm, int
       stage(buf, k0);
       __syncthreads();
       if (k0 + KS < K) load(k0 + KS);
-      if (active) {
 #pragma unroll
-        for (int kk = 0; kk < KS; kk += 4) {
-          double af[2], bf[2];
+      for (int kk = 0; kk < KS; kk += 4) {
+        double af[2], bf[2];
 #pragma unroll
-          for (int i = 0; i < 2; i++) af[i] = As[buf][kk + (lane >> 4)][32 * wr + 16 * i + (lane & 15)];
+        for (int i = 0; i < 2; i++) af[i] = As[buf][kk + (lane >> 4)][32 * wr + 16 * i + (lane & 15)];
 #pragma unroll
-          for (int j = 0; j < 2; j++) bf[j] = Bs[buf][kk + (lane >> 4)][32 * wc + 16 * j + (lane & 15)];
+        for (int j = 0; j < 2; j++) bf[j] = Bs[buf][kk + (lane >> 4)][32 * wc + 16 * j + (lane & 15)];
 #pragma unroll
-          for (int i = 0; i < 2; i++)
+        for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int j = 0; j < 2; j++)
-              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
-        }
+          for (int j = 0; j < 2; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
       }
       buf ^= 1;
     }
@@ -250,31 +298,31 @@ __device__ __forceinline__ void list_mfma_tile(const MmItem &it, int64_t tm, int
   __syncthreads();
 }
 
-template <class T>
-__global__ __launch_bounds__(256) void list_mfma_queue(const MmItem *__restrict__ items,
-                                                       const int3 *__restrict__ work, unsigned int nwork,
+// KS: k per LDS stage (the type's default, or half of it: half the LDS per
+// block, more resident blocks to cover load latency); the register budget
+// matches what the LDS allows (4 blocks of 4 waves per CU at the default
+// stage, 6 at half)
+template <class T, int KS>
+__global__ __launch_bounds__(256, KS == ListMfma<T>::KS ? 4 : 6) void list_mfma_queue(const MmItem *__restrict__ items,
+                                                       const int3 *__restrict__ work,
+                                                       const unsigned int *__restrict__ bounds,
                                                        unsigned int *counter) {
-  constexpr int KS = ListMfma<T>::KS;
   __shared__ T As[2][KS][65];
   __shared__ T Bs[2][KS][65];
   __shared__ unsigned int next;
-  for (;;) {
-    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
-    __syncthreads();
-    const unsigned int w = next;
-    __syncthreads();
-    if (w >= nwork) break;
-    const int3 job = work[w];
-    list_mfma_tile<T>(items[job.x], job.y, job.z, As, Bs);
-  }
+  int bin = blockIdx.x % LIST_BINS;
+  unsigned done = 0;
+  int3 job;
+  while (list_next(counter, bounds, work, bin, done, &next, job))
+    list_mfma_tile<T, KS>(items[job.x], job.y, job.z, As, Bs);
 }
 
-template <class T>
-__global__ __launch_bounds__(256) void list_mfma_one(const MmItem *__restrict__ items, int idx, int64_t ntn) {
-  constexpr int KS = ListMfma<T>::KS;
+template <class T, int KS>
+__global__ __launch_bounds__(256, KS == ListMfma<T>::KS ? 4 : 6) void list_mfma_one(const MmItem *__restrict__ items,
+                                                                                   int idx, int64_t ntn) {
   __shared__ T As[2][KS][65];
   __shared__ T Bs[2][KS][65];
-  list_mfma_tile<T>(items[idx], blockIdx.x / ntn, blockIdx.x % ntn, As, Bs);
+  list_mfma_tile<T, KS>(items[idx], blockIdx.x / ntn, blockIdx.x % ntn, As, Bs);
 }
 
 // ---------------------------------------------------------------- float on BF16 matrix cores
@@ -379,7 +427,7 @@ __device__ __forceinline__ void list_bf16x3_tile(const MmItem &it, int64_t tm, i
     stage(buf, k0);
     __syncthreads();
     if (k0 + BKS < K) load(k0 + BKS);
-    if (active) {
+    {   // every wave multiplies (see list_mfma_tile)
 #pragma unroll
       for (int ks = 0; ks < 2; ks++) {
         const int sa = (ks * 2 + fseg) ^ ((ar_ >> 1) & 3), sb = (ks * 2 + fseg) ^ ((br_ >> 1) & 3);
@@ -411,20 +459,16 @@ __device__ __forceinline__ void list_bf16x3_tile(const MmItem &it, int64_t tm, i
 }
 
 __global__ __launch_bounds__(256) void list_bf16x3_queue(const MmItem *__restrict__ items,
-                                                         const int3 *__restrict__ work, unsigned int nwork,
+                                                         const int3 *__restrict__ work,
+                                                         const unsigned int *__restrict__ bounds,
                                                          unsigned int *counter) {
   __shared__ uint4 As[2][3][64][4];
   __shared__ uint4 Bs[2][3][64][4];
   __shared__ unsigned int next;
-  for (;;) {
-    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
-    __syncthreads();
-    const unsigned int w = next;
-    __syncthreads();
-    if (w >= nwork) break;
-    const int3 job = work[w];
-    list_bf16x3_tile(items[job.x], job.y, job.z, As, Bs);
-  }
+  int bin = blockIdx.x % LIST_BINS;
+  unsigned done = 0;
+  int3 job;
+  while (list_next(counter, bounds, work, bin, done, &next, job)) list_bf16x3_tile(items[job.x], job.y, job.z, As, Bs);
 }
 
 __global__ __launch_bounds__(256) void list_bf16x3_one(const MmItem *__restrict__ items, int idx, int64_t ntn) {
@@ -437,7 +481,8 @@ static hipError_t list_bf16x3(const MmItem *d_items, const MmItem *h_items, int 
                               unsigned int nwork, unsigned int *d_counter, bool each, int nblocks, hipStream_t s) {
   if (!each) {
     hipLaunchKernelGGL(list_reset, dim3(1), dim3(64), 0, s, d_counter);
-    hipLaunchKernelGGL(list_bf16x3_queue, dim3(nblocks), dim3(256), 0, s, d_items, d_work, nwork, d_counter);
+    hipLaunchKernelGGL(list_bf16x3_queue, dim3(nblocks), dim3(256), 0, s, d_items, d_work,
+                       (const unsigned int *)d_counter + LIST_BOUNDS, d_counter);
     return hipGetLastError();
   }
   for (int i = 0; i < nitems; i++) {
@@ -448,18 +493,19 @@ static hipError_t list_bf16x3(const MmItem *d_items, const MmItem *h_items, int 
   return hipGetLastError();
 }
 
-template <class T>
+template <class T, int KS>
 static hipError_t list_mfma_T(const MmItem *d_items, const MmItem *h_items, int nitems, const int3 *d_work,
                               unsigned int nwork, unsigned int *d_counter, bool each, int nblocks, hipStream_t s) {
   if (!each) {
     hipLaunchKernelGGL(list_reset, dim3(1), dim3(64), 0, s, d_counter);
-    hipLaunchKernelGGL(list_mfma_queue<T>, dim3(nblocks), dim3(256), 0, s, d_items, d_work, nwork, d_counter);
+    hipLaunchKernelGGL((list_mfma_queue<T, KS>), dim3(nblocks), dim3(256), 0, s, d_items, d_work,
+                       (const unsigned int *)d_counter + LIST_BOUNDS, d_counter);
     return hipGetLastError();
   }
   for (int i = 0; i < nitems; i++) {
     const int64_t ntm = (h_items[i].H + 63) / 64, ntn = (h_items[i].W + 63) / 64;
     if (ntm * ntn == 0 || h_items[i].T == 0) continue;
-    hipLaunchKernelGGL(list_mfma_one<T>, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, d_items, i, ntn);
+    hipLaunchKernelGGL((list_mfma_one<T, KS>), dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, d_items, i, ntn);
   }
   return hipGetLastError();
 }
@@ -468,7 +514,8 @@ template <class T>
 static hipError_t list_T(const MmItem *d_items, const MmItem *h_items, int nitems, const int3 *d_work,
                          unsigned int nwork, unsigned int *d_counter, int tile, bool fused, bool each, int nblocks,
                          hipStream_t s) {
-#define LQ(F, TS) hipLaunchKernelGGL((list_queue<T, F, TS>), dim3(nblocks), dim3(256), 0, s, d_items, d_work, nwork, \
+#define LQ(F, TS) hipLaunchKernelGGL((list_queue<T, F, TS>), dim3(nblocks), dim3(256), 0, s, d_items, d_work, \
+                                     (const unsigned int *)d_counter + LIST_BOUNDS + (TS == 32 ? LIST_BINS + 1 : 0), \
                                      d_counter)
 #define L1(F, TS) hipLaunchKernelGGL((list_one<T, F, TS>), dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, d_items, i, ntn)
   if (!each) {
@@ -496,10 +543,16 @@ hipError_t launch_matmult_list(int T, const MmItem *d_items, const MmItem *h_ite
   if (tile == 1 && fused && T == PDLHIP_F)
     return list_bf16x3(d_items, h_items, nitems, d_work, nwork, d_counter, each, nblocks, s);
   if (tile == 1) tile = 0;
+  constexpr int KF = ListMfma<float>::KS, KD = ListMfma<double>::KS;
+  if (tile == 2 && fused && T == PDLHIP_F)
+    return list_mfma_T<float, KF / 2>(d_items, h_items, nitems, d_work, nwork, d_counter, each, nblocks, s);
+  if (tile == 2 && fused && T == PDLHIP_D)
+    return list_mfma_T<double, KD / 2>(d_items, h_items, nitems, d_work, nwork, d_counter, each, nblocks, s);
+  if (tile == 2) tile = 0;
   if (tile == 0 && fused && T == PDLHIP_F)
-    return list_mfma_T<float>(d_items, h_items, nitems, d_work, nwork, d_counter, each, nblocks, s);
+    return list_mfma_T<float, KF>(d_items, h_items, nitems, d_work, nwork, d_counter, each, nblocks, s);
   if (tile == 0 && fused && T == PDLHIP_D)
-    return list_mfma_T<double>(d_items, h_items, nitems, d_work, nwork, d_counter, each, nblocks, s);
+    return list_mfma_T<double, KD>(d_items, h_items, nitems, d_work, nwork, d_counter, each, nblocks, s);
   if (tile == 0) tile = 64;
   switch (T) {
 #define X(TC) case TC: return list_T<CType<TC>::T>(d_items, h_items, nitems, d_work, nwork, d_counter, tile, \
