@@ -1738,3 +1738,200 @@ void pdlc_truth_matmult_ld(const double *a, const double *b, double *c, int64_t 
   }
   free(row);
 }
+
+/* ------------------------------------------------------------------ */
+/* Slices.pd rangeb (Slices.pd:561-1015), the engine of range and      */
+/* indexND: MakeComp (index and size conditioning, boundary letters,   */
+/* the corners), RedoDims (the child's dims) and the EquivCPOffs loop   */
+/* with PP's $EQUIVCPTRUNC (PP.pm:1272-1298): forward, a truncated or   */
+/* (bad mode) bad source element gives the child BAD / 0; back, the    */
+/* child's elements go to the parent unless truncated.  `size` is NULL */
+/* for an undefined size, else nsize values; ind must be indx.         */
+/* The child is created (calloc'd, dense).  back: run the BackCode into */
+/* parent from an existing child of the same dims.                     */
+/* ------------------------------------------------------------------ */
+int pdlc_rangeb(pdlc_pdl *parent, const pdlc_pdl *ind, const int64_t *size, int64_t nsize, const char *boundary,
+                pdlc_pdl *child, int back, pdlc_error *err) {
+  memset(err, 0, sizeof *err);
+  if (ind->datatype != PDLC_IND) return make_error(err, 1, "pdlcpu: rangeb: the index must be indx");
+  if (!type_supported(parent->datatype)) return make_error(err, 1, "pdlcpu: rangeb: type not supported");
+  if (ind->ndims && ind->dims[0] == 0)
+    return make_error(err, 1, "Error in rangeb:can't handle Empty indices -- call range instead");
+  const int64_t rdim = ind->ndims ? ind->dims[0] : 1;
+  int64_t nitems = 1;
+  for (int i = 1; i < ind->ndims; i++) nitems *= ind->dims[i];
+  const int itdim = ind->ndims ? ind->ndims - 1 : 0;
+  if (rdim > 64 || itdim > PDLC_MAXDIMS) return make_error(err, 1, "pdlcpu: rangeb: too many dims");
+  /* boundary letters (MakeComp) */
+  char bnd[64];
+  const size_t blen = boundary ? strlen(boundary) : 0;
+  for (int64_t i = 0; i < rdim; i++) {
+    if (!blen) { bnd[i] = 0; continue; }
+    switch (boundary[i < (int64_t)blen - 1 ? i : (int64_t)blen - 1]) {
+    case '0': case 'f': case 'F': bnd[i] = 0; break;
+    case '1': case 't': case 'T': bnd[i] = 1; break;
+    case '2': case 'e': case 'E': bnd[i] = 2; break;
+    case '3': case 'p': case 'P': bnd[i] = 3; break;
+    case '4': case 'm': case 'M': bnd[i] = 4; break;
+    default:
+      return make_error(err, 1, "Error in rangeb:Unknown boundary condition '%c' in range", boundary[i]);
+    }
+  }
+  /* sizes */
+  int64_t sizes[64];
+  for (int64_t i = 0; i < rdim; i++) sizes[i] = 0;
+  if (size && nsize > 0) {
+    if (nsize == 1) {
+      for (int64_t i = 0; i < rdim; i++) sizes[i] = size[0];
+      if (rdim > 0 && sizes[0] < 0) return make_error(err, 1, "Error in rangeb:Negative range size is not allowed\n");
+    } else if (nsize <= rdim) {
+      for (int64_t i = 0; i < rdim; i++) {
+        sizes[i] = i < nsize ? size[i] : 0;
+        if (sizes[i] < 0) return make_error(err, 1, "Error in rangeb:Negative range sizes are not allowed\n");
+      }
+    } else {
+      return make_error(err, 1, "Error in rangeb:Size must match index's 0th dim\n");
+    }
+  }
+  int ntsize = 0;
+  for (int64_t i = 0; i < rdim; i++) ntsize += sizes[i] != 0;
+  /* corners[item][k] */
+  int64_t *corners = malloc(sizeof(int64_t) * (size_t)(rdim * (nitems ? nitems : 1)));
+  {
+    int64_t it[PDLC_MAXDIMS] = {0};
+    for (int64_t item = 0; item < nitems; item++) {
+      int64_t ioff = ind->offs;
+      for (int k = 0; k < itdim; k++) ioff += it[k] * ind->incs[k + 1];
+      for (int64_t j = 0; j < rdim; j++) corners[item * rdim + j] = ((const int64_t *)ind->data)[ioff + (ind->ndims ? ind->incs[0] : 0) * j];
+      for (int k = 0; k < itdim && ++it[k] >= ind->dims[k + 1]; k++) it[k] = 0;
+    }
+  }
+  /* RedoDims */
+  const int pdim = parent->ndims;
+  if (rdim > pdim + 5 && nsize != rdim) {
+    free(corners);
+    return make_error(err, 1,
+                      "Error in rangeb:Ludicrous number of extra dims in range index; leaving child null.\n"
+                      "  (%lld implicit dims is > 5; index has %lld dims; source has %d dim%s.)\n"
+                      "  This often means that your index PDL is incorrect.\n"
+                      "  To avoid this message, allocate dummy dims in\n"
+                      "  the source or use %lld dims in range's size field.\n",
+                      (long long)(rdim - pdim), (long long)rdim, pdim, pdim > 1 ? "s" : "", (long long)rdim);
+  }
+  const int stdim = pdim - rdim > 0 ? (int)(pdim - rdim) : 0;
+  const int cnd = itdim + ntsize + stdim;
+  if (cnd > PDLC_MAXDIMS) { free(corners); return make_error(err, 1, "pdlcpu: rangeb: too many child dims"); }
+  /* the child's dims (RedoDims): index broadcast dims, nonzero sizes,
+     source broadcast dims */
+  int64_t cdims[PDLC_MAXDIMS], inc = 1;
+  {
+    int dim = itdim;
+    for (int64_t i = 0; i < rdim; i++)
+      if (sizes[i]) inc *= (cdims[dim++] = sizes[i]);
+    for (dim = 0; dim < itdim; dim++) inc *= (cdims[dim] = ind->dims[dim + 1]);
+    dim = itdim + ntsize;
+    for (int i = 0; i < stdim; i++) inc *= (cdims[dim++] = parent->dims[i + rdim]);
+  }
+  if (parent->ndims && parent->dims[0] == 0)
+    for (int64_t k = 0; k < rdim; k++) if (bnd[k]) bnd[k] = 1;
+  const int64_t nchild = inc;
+  if (!back) {
+    memset(child, 0, sizeof *child);
+    child->datatype = parent->datatype;
+    child->ndims = cnd;
+    int64_t dinc = 1;
+    for (int d = 0; d < cnd; d++) { child->dims[d] = cdims[d]; child->incs[d] = dinc; dinc *= cdims[d]; }
+    child->state = parent->state & PDLC_BADVAL;
+    child->data = calloc(nchild ? nchild : 1, type_sizes[child->datatype]);
+  }
+  /* child element (coords cc) -> offset in the child as the caller holds it */
+  const int bad = (parent->state & PDLC_BADVAL) != 0;
+  pdlc_anyval pbv, cbv;
+  int pbn, cbn;
+  effective_badval(parent, &pbv, &pbn);
+  effective_badval(child, &cbv, &cbn);
+  /* EquivCPOffsCode */
+  const int prdim = (int)(rdim < pdim ? rdim : pdim);
+  int64_t szv[64 + PDLC_MAXDIMS];
+  for (int64_t k = 0; k < rdim; k++) szv[k] = sizes[k];
+  for (int64_t k = rdim; k < pdim; k++) szv[k] = parent->dims[k];
+  int rc = 0;
+  for (int64_t item = 0; item < nitems && !rc; item++) {
+    int64_t iter[64 + PDLC_MAXDIMS];
+    const int64_t nit = rdim > pdim ? rdim : pdim;
+    for (int64_t k = 0; k < nit; k++) iter[k] = 0;
+    int64_t k;
+    do {
+      int64_t coords[64];
+      int trunc = 0;
+      for (k = 0; k < prdim; k++) {
+        int64_t ck = iter[k] + corners[item * rdim + k];
+        const int64_t pd = parent->dims[k];
+        if (ck < 0 || ck >= pd) {
+          switch (bnd[k]) {
+          case 0: rc = make_error(err, 1, "Error in rangeb:index out-of-bounds in range (index vector #%ld)", (long)item); break;
+          case 1: trunc = 1; break;
+          case 2: ck = (ck >= pd) ? pd - 1 : 0; break;
+          case 3: ck %= pd; if (ck < 0) ck += pd; break;
+          case 4:
+            ck += pd; ck %= pd * 2; if (ck < 0) ck += pd * 2; ck -= pd;
+            if (ck < 0) { ck *= -1; ck -= 1; }
+            break;
+          }
+          if (rc) break;
+        }
+        coords[k] = ck;
+      }
+      if (rc) break;
+      for (; k < rdim; k++) {
+        int64_t ck = iter[k] + corners[item * rdim + k];
+        switch (bnd[k]) {
+        case 0:
+          if (ck != 0) rc = make_error(err, 1, "Error in rangeb:Too many dims in range index (and you've forbidden boundary violations)");
+          break;
+        case 1: trunc |= (ck != 0); break;
+        default: break;
+        }
+        if (rc) break;
+      }
+      if (rc) break;
+      /* the child's coordinates: the item over the index broadcast dims,
+         the in-range position over the nonzero sizes, then the source
+         broadcast dims (RedoDims' order) */
+      int64_t cbase = child->offs, poff = 0;
+      {
+        int64_t r = item;
+        for (int d = 0; d < itdim; d++) { cbase += (r % cdims[d]) * child->incs[d]; r /= cdims[d]; }
+        int k2 = itdim;
+        for (k = 0; k < rdim; k++) {
+          if (k < prdim) poff += coords[k] * parent->incs[k];
+          if (sizes[k]) cbase += iter[k] * child->incs[k2++];
+        }
+      }
+      do {
+        int64_t poff1 = parent->offs + poff, cpos = cbase;
+        int k2 = itdim + ntsize;
+        for (k = rdim; k < pdim; k++) {
+          poff1 += iter[k] * parent->incs[k];
+          cpos += iter[k] * child->incs[k2++];
+        }
+        if (!back) {
+          if (trunc || (bad && elem_isbad(parent, poff1, &pbv, pbn))) {
+            if (bad) set_badval_elem(child->data, child->datatype, cpos, &cbv);
+            else set_badval_elem(child->data, child->datatype, cpos, &(pdlc_anyval){child->datatype, 0, {0}});
+          } else {
+            convert_one(parent->data, parent->datatype, poff1, child->data, child->datatype, cpos);
+          }
+        } else if (!trunc) {
+          if (bad && elem_isbad(child, cpos, &cbv, cbn)) set_badval_elem(parent->data, parent->datatype, poff1, &pbv);
+          else convert_one(child->data, child->datatype, cpos, parent->data, parent->datatype, poff1);
+        }
+        for (k = rdim; k < pdim && ++iter[k] >= parent->dims[k]; k++) iter[k] = 0;
+      } while (k < pdim);
+      for (k = 0; k < rdim && ++iter[k] >= szv[k]; k++) iter[k] = 0;
+    } while (k < rdim);
+  }
+  free(corners);
+  if (rc && !back) { free(child->data); child->data = NULL; }
+  return rc;
+}

@@ -128,6 +128,8 @@ void pdlc_random_fill(double *out, int64_t n, uint64_t seed);
 void pdlc_random_fill_from(double *out, int64_t n, uint64_t seed, uint64_t skip);
 void pdlc_rand_state_after(uint64_t *out, uint64_t seed, uint64_t skip);
 double pdlc_ndtri(double y);
+int pdlc_rangeb(pdlc_pdl *parent, const pdlc_pdl *ind, const int64_t *size, int64_t nsize, const char *boundary,
+                pdlc_pdl *child, int back, pdlc_error *err);
 void pdlc_ndtri_n(const double *y, double *x, int64_t n);
 
 /* FP64 / extended-precision truth helpers (not PDL functions: the "PDL

@@ -75,6 +75,8 @@ def lib():
         l.pdlc_random_fill.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64]
         l.pdlc_random_fill_from.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint64]
         l.pdlc_rand_state_after.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64]
+        l.pdlc_rangeb.argtypes = [ctypes.POINTER(CPdl), ctypes.POINTER(CPdl), ctypes.c_void_p, ctypes.c_int64,
+                                  ctypes.c_char_p, ctypes.POINTER(CPdl), ctypes.c_int, ctypes.POINTER(CError)]
         l.pdlc_ndtri.argtypes = [ctypes.c_double]
         l.pdlc_ndtri.restype = ctypes.c_double
         l.pdlc_ndtri_n.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
@@ -331,6 +333,26 @@ def random_uniform(n, seed):
     out = np.empty(n, dtype=np.float64)
     lib().pdlc_random_fill(out.ctypes.data, n, seed)
     return out
+
+
+def rangeb(parent, ind, size=None, boundary="", back_child=None):
+    """Slices.pd rangeb (the engine of range / indexND): the child as a new
+    OPdl; with back_child, its BackCode writes that child into parent.
+    size: None (undefined) or a list of sizes; ind must be indx."""
+    pc, ic = parent.to_c(), ind.to_c()
+    cc = back_child.to_c() if back_child is not None else CPdl()
+    sz = None if size is None else np.ascontiguousarray(np.atleast_1d(np.asarray(size, dtype=np.int64)))
+    err = CError()
+    rc = lib().pdlc_rangeb(ctypes.byref(pc), ctypes.byref(ic), sz.ctypes.data if sz is not None and sz.size else None,
+                           0 if sz is None else sz.size, (boundary or "").encode(), ctypes.byref(cc),
+                           1 if back_child is not None else 0, ctypes.byref(err))
+    if rc:
+        raise OracleError(err.message.decode())
+    if back_child is not None:
+        return back_child
+    child = OPdl.null()
+    child.from_c(cc, True)
+    return child
 
 
 def random_from(n, seed, skip=0):

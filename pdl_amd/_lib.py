@@ -115,6 +115,8 @@ EXPORTS["pdlhip_run_index"] = ([PP, PP, PP], Error)
 EXPORTS["pdlhip_run_index1d"] = ([PP, PP, PP], Error)
 EXPORTS["pdlhip_run_index2d"] = ([PP, PP, PP, PP], Error)
 EXPORTS["pdlhip_writebackdata"] = ([ctypes.c_char_p, ctypes.POINTER(PP), ctypes.c_int], Error)
+for _n in ("pdlhip_run_rangeb", "pdlhip_writeback_rangeb"):
+    EXPORTS[_n] = ([PP, PP, ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, ctypes.c_char_p, PP], Error)
 # PDL's random-number stream
 EXPORTS["pdlhip_srandom"] = ([ctypes.c_uint64], Error)
 EXPORTS["pdlhip_rand_skip"] = ([ctypes.c_uint64], Error)

@@ -156,13 +156,15 @@ def _mirror(buf):
 
 
 class _Flow:
-    """a child's dataflow: op, its parents (a and the index ndarrays, in
-    signature order), the child over its whole buffer, and the parents'
-    buffer versions its data reflects"""
-    __slots__ = ("op", "parents", "child", "seen")
+    """a child's dataflow: its parents (the source first, then the index
+    ndarrays), the child over its whole buffer, the parents' buffer versions
+    its data reflects, and the two calls PDL's trans would make: fwd
+    (readdata: compute the child again) and back (writebackdata: the child
+    into the source)"""
+    __slots__ = ("parents", "child", "seen", "fwd", "back")
 
-    def __init__(self, op, parents, child):
-        self.op, self.parents, self.child = op, list(parents), child
+    def __init__(self, parents, child, fwd, back):
+        self.parents, self.child, self.fwd, self.back = list(parents), child, fwd, back
         self.seen = self.versions()
 
     def versions(self):
@@ -190,7 +192,7 @@ def _refresh(buf):
         _refresh(p._buf)
         _flush(p._buf)
     if f.versions() != f.seen:
-        _call(f.op, f.parents + [f.child])
+        f.fwd(f)
         buf.version += 1
         f.seen = f.versions()
 
@@ -211,11 +213,35 @@ def _writeback(buf):
     _flush(buf)
     for p in f.parents:
         _flush(p._buf)
-    cs = [p._to_c() for p in f.parents + [f.child]]
-    arr = (L.PP * len(cs))(*[ctypes.pointer(c) for c in cs])
-    L.check(L.lib().pdlhip_writebackdata(f.op.encode(), arr, len(cs)))
+    f.back(f)
     _written(f.parents[0]._buf)
     f.seen = f.versions()
+
+
+def _index_flow(op):
+    """the forward and back calls of an index-family child"""
+    def fwd(f):
+        _call(op, f.parents + [f.child])
+
+    def back(f):
+        cs = [p._to_c() for p in f.parents + [f.child]]
+        arr = (L.PP * len(cs))(*[ctypes.pointer(c) for c in cs])
+        L.check(L.lib().pdlhip_writebackdata(op.encode(), arr, len(cs)))
+    return fwd, back
+
+
+def _range_call(entry, parent, ind, size, boundary, child):
+    """pdlhip_run_rangeb / pdlhip_writeback_rangeb; size None (undefined) or
+    a list of ints; returns the child's C struct"""
+    cp, ci, cc = parent._to_c(), ind._to_c(), child._to_c()
+    if size is None:
+        sz, n = None, -1
+    else:
+        arr = np.ascontiguousarray(np.asarray(size, dtype=np.int64).reshape(-1))
+        sz, n = (ctypes.c_int64 * max(arr.size, 1))(*arr.tolist()), arr.size
+    L.check(getattr(L.lib(), entry)(ctypes.byref(cp), ctypes.byref(ci), sz, n,
+                                    (boundary or "").encode(), ctypes.byref(cc)))
+    return cc
 
 
 def _dense_incs(dims):
@@ -408,8 +434,67 @@ class ndarray:
     def _flowing(self, op, parents):
         c = ndarray()
         _run(op, parents + [c])
-        c._buf.flow = _Flow(op, parents, c._alias())
+        c._buf.flow = _Flow(parents, c._alias(), *_index_flow(op))
         return c
+
+    def rangeb(self, ind, size=None, boundary=""):
+        """Slices.pd rangeb (Slices.pd:561-1015): the chunks of this ndarray at
+        the coordinates in ind (dim 0: a coordinate vector), each of `size`
+        (None: single elements), with per-dim boundary letters (f forbid, t
+        truncate, e extend, p periodic, m mirror); a child with dataflow"""
+        ind = topdl(ind)
+        for p in (self, ind):
+            _refresh(p._buf)
+            _flush(p._buf)
+        c = ndarray()
+        cc = _range_call("pdlhip_run_rangeb", self, ind, size, boundary, c)
+        c._from_c(cc, True)
+        _written(c._buf)
+
+        def fwd(f):
+            _range_call("pdlhip_run_rangeb", f.parents[0], f.parents[1], size, boundary, f.child)
+
+        def back(f):
+            _range_call("pdlhip_writeback_rangeb", f.parents[0], f.parents[1], size, boundary, f.child)
+        if c._buf is not None:
+            c._buf.flow = _Flow([self, ind], c._alias(), fwd, back)
+        return c
+
+    def range(self, ind, size=None, boundary=None):
+        """PDL::range (Slices.pd:272-304): the index as indx (a 0-dim index
+        gets a dummy dim 0), an empty index gives an empty result of the
+        right dims, boundary lists pack to their first letters"""
+        index = ind if isinstance(ind, ndarray) and ind.dtype == L.IND else topdl(ind).convert(L.IND) \
+            if isinstance(ind, ndarray) else ndarray.from_numpy(np.asarray(ind, dtype=np.int64), L.IND)
+        sz = None if size is None else [int(v) for v in np.asarray(size.numpy() if isinstance(size, ndarray)
+                                                                    else size).reshape(-1)]
+        if index.nelem == 0:
+            sdims = list(self.dims)
+            n0 = index.dims[0] if index.ndims else 0
+            del sdims[:n0 + (1 if n0 == 0 else 0)]
+            if sz is not None:
+                sdims = sz + sdims
+            idims = [0] * (index.ndims - 1) or [0]
+            return ndarray.new_dense(self.dtype, idims + sdims)
+        if index.ndims == 0:
+            index = index.dummy(0, 1)
+        bound = boundary
+        if isinstance(boundary, (list, tuple)):
+            bound = ""
+            for el in boundary:
+                el = str(el)
+                if not el or el[0] not in "0123fFtTeEpPmM":
+                    raise PDLError(L.EUSERERROR, "Illegal boundary value '%s' in range" % el)
+                bound += el[0]
+        elif boundary is not None:
+            import re
+            if not re.match(r"^[0123ftepx]+$", boundary) and re.match(r"^([0123ftepx])", boundary, re.I):
+                bound = boundary[0]
+        return self.rangeb(index, sz, bound or "")
+
+    def indexND(self, ind, boundary=None):
+        """Slices.pd:265-268: range with no size"""
+        return self.range(ind, None, boundary)
 
     # Slices.pd:142-220 -- non-affine slices: a new ndarray with dataflow
     # both ways (recomputed when a parent changes, written back when written)

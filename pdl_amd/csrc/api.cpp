@@ -2151,6 +2151,178 @@ pdlhip_error pdlhip_writebackdata(const char *op, pdlhip_pdl **pdls, int npdls) 
   return e;
 }
 
+// Slices.pd rangeb (the engine of range and indexND): MakeComp's index and
+// size conditioning and boundary letters, RedoDims' child dims, then the
+// EquivCPOffs loop on the device (k_index.hip); back: its BackCode
+static pdlhip_error rangeb(pdlhip_pdl *parent, pdlhip_pdl *ind, const int64_t *size, int64_t nsize,
+                           const char *boundary, pdlhip_pdl *child, bool back) {
+  RETURN_IF(ensure_init());
+  if (!parent || !ind || !child) return make_err(PDLHIP_EUSERERROR, "pdlhip: rangeb: NULL ndarray");
+  if ((parent->state & PDLHIP_NOMYDIMS) || (ind->state & PDLHIP_NOMYDIMS))
+    return make_err(PDLHIP_EUSERERROR, "pdlhip: rangeb: the source and the index must exist");
+  if (!type_on_device(parent->datatype) || !type_on_device(ind->datatype))
+    return make_err(PDLHIP_EUSERERROR, "pdlhip: rangeb: type not supported on the device");
+  if (capturing())
+    return make_err(PDLHIP_EUSERERROR, "pdlhip: rangeb checks its indices on the host and cannot be captured");
+  if (ind->ndims && ind->dims[0] == 0)
+    return make_err(PDLHIP_EUSERERROR, "Error in rangeb:can't handle Empty indices -- call range instead");
+  RgPlan p;
+  std::memset(&p, 0, sizeof p);
+  const int64_t rdim = ind->ndims ? ind->dims[0] : 1;
+  if (rdim > RG_MAXR) return make_err(PDLHIP_EUSERERROR, "pdlhip: rangeb: more than %d index coordinates", RG_MAXR);
+  p.rdim = (int)rdim;
+  p.itdim = ind->ndims ? ind->ndims - 1 : 0;
+  p.nitems = 1;
+  for (int i = 1; i < ind->ndims; i++) p.nitems *= ind->dims[i];
+  const size_t blen = boundary ? strlen(boundary) : 0;
+  for (int64_t i = 0; i < rdim; i++) {
+    if (!blen) continue;
+    switch (boundary[i < (int64_t)blen - 1 ? i : (int64_t)blen - 1]) {
+    case '0': case 'f': case 'F': p.bnd[i] = 0; break;
+    case '1': case 't': case 'T': p.bnd[i] = 1; break;
+    case '2': case 'e': case 'E': p.bnd[i] = 2; break;
+    case '3': case 'p': case 'P': p.bnd[i] = 3; break;
+    case '4': case 'm': case 'M': p.bnd[i] = 4; break;
+    default:
+      return make_err(PDLHIP_EUSERERROR, "Error in rangeb:Unknown boundary condition '%c' in range",
+                      i < (int64_t)blen ? boundary[i] : '\0');
+    }
+  }
+  if (size && nsize > 0) {
+    if (nsize == 1) {
+      for (int64_t i = 0; i < rdim; i++) p.sizes[i] = size[0];
+      if (rdim > 0 && p.sizes[0] < 0)
+        return make_err(PDLHIP_EUSERERROR, "Error in rangeb:Negative range size is not allowed\n");
+    } else if (nsize <= rdim) {
+      for (int64_t i = 0; i < rdim; i++) {
+        p.sizes[i] = i < nsize ? size[i] : 0;
+        if (p.sizes[i] < 0) return make_err(PDLHIP_EUSERERROR, "Error in rangeb:Negative range sizes are not allowed\n");
+      }
+    } else {
+      return make_err(PDLHIP_EUSERERROR, "Error in rangeb:Size must match index's 0th dim\n");
+    }
+  }
+  for (int64_t i = 0; i < rdim; i++) p.ntsize += p.sizes[i] != 0;
+  p.pdim = parent->ndims;
+  if (rdim > p.pdim + 5 && nsize != rdim)
+    return make_err(PDLHIP_EUSERERROR,
+                    "Error in rangeb:Ludicrous number of extra dims in range index; leaving child null.\n"
+                    "  (%lld implicit dims is > 5; index has %lld dims; source has %d dim%s.)\n"
+                    "  This often means that your index PDL is incorrect.\n"
+                    "  To avoid this message, allocate dummy dims in\n"
+                    "  the source or use %lld dims in range's size field.\n",
+                    (long long)(rdim - p.pdim), (long long)rdim, p.pdim, p.pdim > 1 ? "s" : "", (long long)rdim);
+  p.stdim = p.pdim - (int)rdim > 0 ? p.pdim - (int)rdim : 0;
+  p.cnd = p.itdim + p.ntsize + p.stdim;
+  if (p.cnd > PDLHIP_MAXDIMS) return make_err(PDLHIP_EUSERERROR, "pdlhip: rangeb: the child would have %d dims", p.cnd);
+  int64_t cd[PDLHIP_MAXDIMS];
+  {
+    int dim = p.itdim;
+    for (int64_t i = 0; i < rdim; i++) if (p.sizes[i]) cd[dim++] = p.sizes[i];
+    for (dim = 0; dim < p.itdim; dim++) cd[dim] = ind->dims[dim + 1];
+    dim = p.itdim + p.ntsize;
+    for (int i = 0; i < p.stdim; i++) cd[dim++] = parent->dims[i + rdim];
+  }
+  // an empty source: the boundaries that do not croak truncate (RedoDims);
+  // a zero extent under extend / periodic / mirror would divide by zero
+  for (int64_t k = 0; k < rdim && k < p.pdim; k++)
+    if (p.bnd[k] >= 2 && (parent->dims[k] == 0 || (p.pdim && parent->dims[0] == 0))) p.bnd[k] = 1;
+  if (p.pdim && parent->dims[0] == 0)
+    for (int64_t k = 0; k < rdim; k++) if (p.bnd[k]) p.bnd[k] = 1;
+  p.nchild = 1;
+  for (int d = 0; d < p.cnd; d++) p.nchild *= cd[d];
+  p.ninr = 1;
+  for (int64_t k = 0; k < rdim; k++) p.ninr *= p.sizes[k] ? p.sizes[k] : 1;
+  p.nst = 1;
+  for (int i = 0; i < p.stdim; i++) p.nst *= parent->dims[i + rdim];
+  if (back) {
+    if (child->state & PDLHIP_NOMYDIMS) return make_err(PDLHIP_EUSERERROR, "pdlhip: rangeb: no child to write back");
+    bool same = child->ndims == p.cnd && child->datatype == parent->datatype;
+    for (int d = 0; same && d < p.cnd; d++) same = child->dims[d] == cd[d];
+    if (!same) return make_err(PDLHIP_EUSERERROR, "pdlhip: rangeb: the child does not have the range's dims and type");
+  } else if (child->state & PDLHIP_NOMYDIMS) {
+    child->datatype = parent->datatype;
+    child->ndims = p.cnd;
+    int64_t inc = 1;
+    for (int d = 0; d < p.cnd; d++) { child->dims[d] = cd[d]; child->incs[d] = inc; inc *= cd[d]; }
+    child->offs = 0;
+    child->has_badvalue = 0;
+    child->data = g.plan_only ? nullptr : dev_alloc((size_t)std::max<int64_t>(p.nchild, 1) * type_size(child->datatype));
+    if (!g.plan_only && !child->data) return oom_err("rangeb");
+    child->state = parent->state & PDLHIP_BADVAL;
+  } else {
+    bool same = child->ndims == p.cnd && child->datatype == parent->datatype;
+    for (int d = 0; same && d < p.cnd; d++) same = child->dims[d] == cd[d];
+    if (!same) return make_err(PDLHIP_EUSERERROR, "pdlhip: rangeb: the output does not have the range's dims and type");
+  }
+  if (g.plan_only || p.nchild == 0 || p.nitems == 0) return ok();
+  // the index as indx (MakeComp converts it)
+  pdlhip_pdl iconv;
+  pdlhip_pdl *ix = ind;
+  bool conv = false;
+  if (ind->datatype != PDLHIP_IND) {
+    RETURN_IF(make_converted(ind, PDLHIP_IND, &iconv, true));
+    ix = &iconv;
+    conv = true;
+  }
+  p.ioff = ix->offs;
+  p.iincs[0] = ix->ndims ? ix->incs[0] : 0;
+  for (int d = 0; d < p.itdim; d++) p.iincs[d + 1] = ix->incs[d + 1];
+  for (int d = 0; d < p.pdim; d++) { p.pdims[d] = parent->dims[d]; p.pincs[d] = parent->incs[d]; }
+  p.poff = parent->offs;
+  for (int d = 0; d < p.cnd; d++) { p.cdims[d] = cd[d]; p.cincs[d] = child->incs[d]; }
+  p.coff = child->offs;
+  p.bad = (parent->state & PDLHIP_BADVAL) != 0;
+  p.back = back ? 1 : 0;
+  long long *owner = nullptr;
+  if (back) {
+    int64_t lo = parent->offs, hi = parent->offs;
+    for (int d = 0; d < parent->ndims; d++) {
+      const int64_t ext = (parent->dims[d] - 1) * parent->incs[d];
+      if (ext < 0) lo += ext; else hi += ext;
+    }
+    p.a_lo = lo;
+    p.a_span = hi - lo + 1;
+    owner = (long long *)dev_alloc((size_t)p.a_span * sizeof(long long));
+    if (!owner) {
+      if (conv) dev_free(iconv.data);
+      return oom_err("rangeb");
+    }
+  }
+  unsigned long long *err = (unsigned long long *)dev_alloc(sizeof(unsigned long long));
+  if (!err) {
+    dev_free(owner);
+    if (conv) dev_free(iconv.data);
+    return oom_err("rangeb");
+  }
+  hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, g.stream, err, ~0ull);
+  const BadInfo pb = badinfo(parent, 1), cb = badinfo(child, 1);
+  hipError_t e = launch_range(parent->datatype, p, parent->data, (const int64_t *)ix->data, child->data, pb.bv, cb.bv,
+                              pb.isnan, cb.isnan, err, owner, g.stream);
+  unsigned long long key = ~0ull;
+  if (e == hipSuccess) e = hipMemcpyAsync(&key, err, sizeof key, hipMemcpyDeviceToHost, g.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+  dev_free(err);
+  dev_free(owner);
+  if (conv) dev_free(iconv.data);
+  if (e != hipSuccess) return hip_err(e, "rangeb");
+  if (key == ~0ull) return ok();
+  if (key & 1)
+    return make_err(PDLHIP_EUSERERROR,
+                    "Error in rangeb:Too many dims in range index (and you've forbidden boundary violations)");
+  return make_err(PDLHIP_EUSERERROR, "Error in rangeb:index out-of-bounds in range (index vector #%ld)",
+                  (long)((key >> 1) / p.ninr));
+}
+
+pdlhip_error pdlhip_run_rangeb(pdlhip_pdl *parent, pdlhip_pdl *ind, const int64_t *size, int64_t nsize,
+                               const char *boundary, pdlhip_pdl *child) {
+  return rangeb(parent, ind, size, nsize, boundary, child, false);
+}
+pdlhip_error pdlhip_writeback_rangeb(pdlhip_pdl *parent, pdlhip_pdl *ind, const int64_t *size, int64_t nsize,
+                                     const char *boundary, pdlhip_pdl *child) {
+  return rangeb(parent, ind, size, nsize, boundary, child, true);
+}
+
 pdlhip_error pdlhip_run_setbadif(pdlhip_pdl *a, pdlhip_pdl *mask, pdlhip_pdl *b) {
   pdlhip_pdl *v[3] = {a, mask, b};
   return pdlhip_run("setbadif", v, 3, nullptr);

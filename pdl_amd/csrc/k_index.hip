@@ -135,6 +135,158 @@ static hipError_t index_go(const IxPlan &p, void *a, const int64_t *i1, const in
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- rangeb
+// Slices.pd rangeb (EquivCPOffsCode, Slices.pd:900-1013, with PP's
+// $EQUIVCPTRUNC): a child element at index-broadcast position `item`,
+// in-range position iter[] and source-broadcast position st reads the
+// parent at corner + iter with each dim's boundary rule (0 forbid: $CROAK;
+// 1 truncate: the child gets BAD in bad mode, else 0; 2 extend: clamp; 3
+// periodic; 4 mirror); dims of the index past the parent's take rule 0/1 on
+// a nonzero coordinate and ignore it otherwise.  A failing position keeps
+// the smallest (item, in-range position, kind) in PDL's loop order for the
+// host's message.  BackCode: the parent element gets the child of the last
+// position in PDL's loop order (items, in-range positions, source
+// broadcast) that maps to it and is not truncated.
+struct RgElem {
+  int64_t poff, coff, key;   // key: the position in PDL's loop order
+  int trunc;
+};
+
+__device__ __forceinline__ bool rg_element(const RgPlan &p, int64_t e, const int64_t *__restrict__ ind,
+                                           unsigned long long *err, RgElem &r) {
+  int64_t c[PDLHIP_MAXDIMS];
+  int64_t rem = e;
+  for (int d = 0; d < p.cnd; d++) {
+    c[d] = rem % p.cdims[d];
+    rem /= p.cdims[d];
+  }
+  int64_t item = 0, istride = 1, ioff = p.ioff;
+  for (int d = 0; d < p.itdim; d++) {
+    item += c[d] * istride;
+    istride *= p.cdims[d];
+    ioff += c[d] * p.iincs[d + 1];
+  }
+  r.coff = p.coff;
+  for (int d = 0; d < p.cnd; d++) r.coff += c[d] * p.cincs[d];
+  // the in-range position (zero sizes stay at 0) and its place in PDL's loop
+  int64_t it[RG_MAXR], inr = 0, inr_stride = 1;
+  int j = p.itdim;
+  for (int k = 0; k < p.rdim; k++) {
+    it[k] = p.sizes[k] ? c[j++] : 0;
+    inr += it[k] * inr_stride;
+    inr_stride *= p.sizes[k] ? p.sizes[k] : 1;
+  }
+  const int64_t pos = item * p.ninr + inr;
+  int64_t poff = p.poff;
+  int trunc = 0;
+  const int prdim = p.rdim < p.pdim ? p.rdim : p.pdim;
+  for (int k = 0; k < p.rdim; k++) {
+    int64_t ck = it[k] + ind[ioff + k * p.iincs[0]];
+    if (k < prdim) {
+      const int64_t pd = p.pdims[k];
+      if (ck < 0 || ck >= pd) {
+        switch (p.bnd[k]) {
+        case 0:
+          atomicMin(err, (unsigned long long)(pos * 2));        // "index out-of-bounds in range"
+          return false;
+        case 1: trunc = 1; break;
+        case 2: ck = ck >= pd ? pd - 1 : 0; break;
+        case 3: ck %= pd; if (ck < 0) ck += pd; break;
+        default:
+          ck += pd; ck %= pd * 2; if (ck < 0) ck += pd * 2; ck -= pd;
+          if (ck < 0) ck = -ck - 1;
+          break;
+        }
+      }
+      if (!trunc) poff += ck * p.pincs[k];
+    } else if (ck != 0) {
+      if (p.bnd[k] == 0) {
+        atomicMin(err, (unsigned long long)(pos * 2 + 1));      // "Too many dims in range index"
+        return false;
+      }
+      if (p.bnd[k] == 1) trunc = 1;
+    }
+  }
+  int64_t st = 0, st_stride = 1;
+  for (int k = 0; k < p.stdim; k++) {
+    const int64_t x = c[p.itdim + p.ntsize + k];
+    poff += x * p.pincs[p.rdim + k];
+    st += x * st_stride;
+    st_stride *= p.cdims[p.itdim + p.ntsize + k];
+  }
+  r.poff = poff;
+  r.trunc = trunc;
+  r.key = (item * p.ninr + inr) * p.nst + st;
+  return true;
+}
+
+template <class T>
+__global__ __launch_bounds__(IX_THREADS) void k_range_gather(RgPlan p, const T *__restrict__ a,
+                                                             const int64_t *__restrict__ ind, T *__restrict__ c,
+                                                             T pbv, T cbv, int pnan, unsigned long long *err) {
+  for (int64_t e = (int64_t)blockIdx.x * IX_THREADS + threadIdx.x; e < p.nchild; e += (int64_t)gridDim.x * IX_THREADS) {
+    RgElem r;
+    if (!rg_element(p, e, ind, err, r)) continue;
+    if (r.trunc) {
+      c[r.coff] = p.bad ? cbv : T(0);
+      continue;
+    }
+    const T v = a[r.poff];
+    c[r.coff] = (p.bad && isbad(v, pbv, pnan)) ? cbv : v;
+  }
+}
+
+__global__ __launch_bounds__(IX_THREADS) void k_range_owner(RgPlan p, const int64_t *__restrict__ ind,
+                                                            unsigned long long *err, long long *owner) {
+  for (int64_t e = (int64_t)blockIdx.x * IX_THREADS + threadIdx.x; e < p.nchild; e += (int64_t)gridDim.x * IX_THREADS) {
+    RgElem r;
+    if (!rg_element(p, e, ind, err, r) || r.trunc) continue;
+    atomicMax(owner + (r.poff - p.a_lo), (long long)r.key);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(IX_THREADS) void k_range_scatter(RgPlan p, T *__restrict__ a,
+                                                              const int64_t *__restrict__ ind, const T *__restrict__ c,
+                                                              T pbv, T cbv, int cnan, const long long *owner) {
+  for (int64_t e = (int64_t)blockIdx.x * IX_THREADS + threadIdx.x; e < p.nchild; e += (int64_t)gridDim.x * IX_THREADS) {
+    RgElem r;
+    unsigned long long dummy = ~0ull;
+    if (!rg_element(p, e, ind, &dummy, r) || r.trunc) continue;
+    if (owner[r.poff - p.a_lo] != (long long)r.key) continue;
+    const T v = c[r.coff];
+    a[r.poff] = (p.bad && isbad(v, cbv, cnan)) ? pbv : v;
+  }
+}
+
+template <class T>
+static hipError_t range_go(const RgPlan &p, void *a, const int64_t *ind, void *c, const pdlhip_value &pbv,
+                           const pdlhip_value &cbv, int pnan, int cnan, unsigned long long *err, long long *owner,
+                           hipStream_t s) {
+  if (p.nchild == 0) return hipSuccess;
+  if (!p.back) {
+    hipLaunchKernelGGL(k_range_gather<T>, dim3(ix_grid(p.nchild)), dim3(IX_THREADS), 0, s, p, (const T *)a, ind,
+                       (T *)c, bv_as<T>(pbv), bv_as<T>(cbv), pnan, err);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(k_fill_i64, dim3(ix_grid(p.a_span)), dim3(IX_THREADS), 0, s, owner, p.a_span, -1ll);
+  hipLaunchKernelGGL(k_range_owner, dim3(ix_grid(p.nchild)), dim3(IX_THREADS), 0, s, p, ind, err, owner);
+  hipLaunchKernelGGL(k_range_scatter<T>, dim3(ix_grid(p.nchild)), dim3(IX_THREADS), 0, s, p, (T *)a, ind,
+                     (const T *)c, bv_as<T>(pbv), bv_as<T>(cbv), cnan, (const long long *)owner);
+  return hipGetLastError();
+}
+
+hipError_t launch_range(int T, const RgPlan &p, void *parent, const int64_t *ind, void *child,
+                        const pdlhip_value &pbv, const pdlhip_value &cbv, int pbv_nan, int cbv_nan,
+                        unsigned long long *err, long long *owner, hipStream_t s) {
+  switch (T) {
+#define X(TC) case TC: return range_go<CType<TC>::T>(p, parent, ind, child, pbv, cbv, pbv_nan, cbv_nan, err, owner, s);
+    PDLHIP_FOR_EACH_TYPE(X)
+#undef X
+  default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_index(int T, const IxPlan &p, void *a, const int64_t *i1, const int64_t *i2, void *c,
                         const pdlhip_value &cbv, unsigned long long *err, long long *owner, hipStream_t s) {
   switch (T) {

@@ -111,6 +111,24 @@ struct IxPlan {
 hipError_t launch_index(int T, const IxPlan &p, void *a, const int64_t *i1, const int64_t *i2, void *c,
                         const pdlhip_value &cbv, unsigned long long *err, long long *owner, hipStream_t s);
 
+// Slices.pd rangeb (k_index.hip): one thread per child element; the
+// child's dims are the index broadcast dims, the nonzero sizes, then the
+// source's dims past rdim (RedoDims' order)
+constexpr int RG_MAXR = 32;
+struct RgPlan {
+  int32_t rdim, pdim, itdim, ntsize, stdim, bad, back, cnd;
+  int64_t nitems, nchild, ninr, nst;
+  int64_t sizes[RG_MAXR];
+  int32_t bnd[RG_MAXR];
+  int64_t pdims[PDLHIP_MAXDIMS], pincs[PDLHIP_MAXDIMS], poff;
+  int64_t iincs[PDLHIP_MAXDIMS + 1], ioff;       // the index: coordinate stride, then its broadcast dims
+  int64_t cdims[PDLHIP_MAXDIMS], cincs[PDLHIP_MAXDIMS], coff;
+  int64_t a_lo, a_span;                           // BackCode: the owner table spans the parent's elements
+};
+hipError_t launch_range(int T, const RgPlan &p, void *parent, const int64_t *ind, void *child,
+                        const pdlhip_value &pbv, const pdlhip_value &cbv, int pbv_nan, int cbv_nan,
+                        unsigned long long *err, long long *owner, hipStream_t s);
+
 hipError_t launch_fill(int dtype, const EwPlan &p, void *dst, uint64_t seed, double a, double b, int normal,
                        hipStream_t s);
 

@@ -191,3 +191,104 @@ def test_host_mirror_reads_once_and_writes_before_the_next_op():
     np.testing.assert_array_equal(s.numpy(), [5 + 2 + 4 + 6, 8 + 10 + 12 + 14, 16 + 18 + 20 + 22])
     b += 1                                       # the device writes b: the mirror is out of date
     assert b.at(0, 0) == 6.0
+
+
+# ------------------------------------------------------------ range / rangeb / indexND
+def source10():
+    return (10 * np.arange(10)[None, :] + np.arange(10)[:, None]).astype(np.float64)   # x(i,j) = 10 i + j
+
+
+IDX = np.array([[[2, 3], [4, 5]], [[6, 7], [8, 9]]], np.int64)
+
+
+def test_indexND_known_answers():
+    """t/slice.t:179-195"""
+    s = C.ndarray.from_numpy(source10())
+    np.testing.assert_array_equal(s.indexND(IDX).numpy(), [[23, 45], [67, 89]])
+    z = np.zeros((2, 10, 10))
+    src3 = (100 * np.arange(10)[None, None, :] + 10 * np.arange(10)[None, :, None] + np.arange(2)[:, None, None] + z)
+    x = C.ndarray.from_numpy(src3).indexND(IDX)
+    np.testing.assert_array_equal(x.numpy(), [[[230, 450], [670, 890]], [[231, 451], [671, 891]]])
+    e = C.ndarray.from_numpy(source10()).indexND(C.ndarray.new_dense(L.IND, [2, 0]))
+    assert e.dims == [0]
+
+
+@pytest.mark.parametrize("size,bound,sl,want", [
+    (None, None, None, [[23, 45], [67, 89]]),
+    (3, "t", (1, 1), [[89, 99, 0], [0, 0, 0], [0, 0, 0]]),
+    (3, "tp", (1, 1), [[89, 99, 0], [80, 90, 0], [81, 91, 0]]),
+    (3, ["e", "p"], (1, 1), [[89, 99, 99], [80, 90, 90], [81, 91, 91]]),
+])
+def test_range_known_answers(size, bound, sl, want):
+    """t/slice.t:196-233"""
+    s = C.ndarray.from_numpy(source10())
+    y = s.range(IDX, size, bound)
+    assert y.dims == ([2, 2] if size is None else [2, 2, 3, 3])
+    got = y.numpy() if sl is None else y.numpy()[:, :, sl[1], sl[0]]
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(s.numpy(), source10())       # the source is not mutated
+
+
+def test_range_out_of_bounds_and_periodic_writeback():
+    s = C.ndarray.from_numpy(source10())
+    with pytest.raises(L.PDLError) as e:
+        s.range(IDX, 3)
+    assert str(e.value) == "Error in rangeb:index out-of-bounds in range (index vector #3)"
+    s3 = C.ndarray.from_numpy((10 * np.arange(3)[None, :] + np.arange(3)[:, None]).astype(np.float64))
+    y = s3.range([-1, -1], [2, 2], "p")
+    np.testing.assert_array_equal(y.numpy(), [[22, 2], [20, 0]])
+    y.assign(6)                                                   # flows back through the periodic corners
+    np.testing.assert_array_equal(s3.numpy(), [[6, 10, 6], [1, 11, 21], [6, 12, 6]])
+
+
+BOUNDS = ["f", "t", "e", "p", "m", "tp", "em", "pe"]
+
+
+@pytest.mark.parametrize("bound", BOUNDS)
+@pytest.mark.parametrize("t", [O.S, O.F, O.D], ids=["short", "float", "double"])
+def test_range_against_the_oracle(t, bound):
+    """random corners (in and out of range), sizes with a zero (a dropped
+    dim), source broadcast dims past the index length, bad values; forward
+    and BackCode bit-exact with the oracle's rangeb"""
+    rng = np.random.default_rng(900 + t + 7 * BOUNDS.index(bound))
+    src = rand(t, (3, 7, 9), rng)                                # parent dims (9,7,3): dims past rdim=2 broadcast
+    lo, hi = (0, 4) if bound == "f" else (-12, 14)
+    ind = rng.integers(lo, hi, (5, 4, 2)).astype(np.int64)        # index dims (2,4,5): 20 vectors
+    for size in ([3, 0], [4, 2], [1]):
+        for badflag in (False, True):
+            s2 = src.copy()
+            if badflag:
+                s2.reshape(-1)[::11] = O.default_badval(t)
+            op_, dp_ = pair(s2, t, badflag)
+            oi, di = pair(ind, O.IND)
+            oc = O.rangeb(op_, oi, size, bound)
+            dc = dp_.rangeb(di, size, bound)
+            same(oc, dc, "%s %s %s" % (bound, size, badflag))
+            # BackCode: new child values into copies of the sources
+            newc = rand(t, tuple(oc.dims[::-1]), rng)
+            op2, dp2 = pair(s2, t, badflag)
+            oc2 = O.OPdl.from_numpy(newc, t, badflag=badflag)
+            O.rangeb(op2, oi, size, bound, back_child=oc2)
+            dc2 = C.ndarray.from_numpy(newc, t, badflag=badflag)
+            cp, ci, cc = dp2._to_c(), di._to_c(), dc2._to_c()
+            arr = np.asarray(size, np.int64)
+            L.check(L.lib().pdlhip_writeback_rangeb(C.ctypes.byref(cp), C.ctypes.byref(ci),
+                                                     (C.ctypes.c_int64 * arr.size)(*arr.tolist()), arr.size,
+                                                     bound.encode(), C.ctypes.byref(cc)))
+            np.testing.assert_array_equal(op2.gather().view(np.uint8), dp2.numpy().view(np.uint8))
+
+
+def test_range_size_and_boundary_errors():
+    s = C.ndarray.from_numpy(source10())
+    o = O.OPdl.from_numpy(source10(), O.D)
+    idx = np.array([[1, 2], [3, 4]], np.int64)
+    for size, bound, msg in (([-1], "", "Error in rangeb:Negative range size is not allowed\n"),
+                             ([2, -1], "", "Error in rangeb:Negative range sizes are not allowed\n"),
+                             ([1, 2, 3], "", "Error in rangeb:Size must match index's 0th dim\n"),
+                             ([1], "q", "Error in rangeb:Unknown boundary condition 'q' in range")):
+        with pytest.raises(O.OracleError) as eo:
+            O.rangeb(o, O.OPdl.from_numpy(idx, O.IND), size, bound)
+        assert str(eo.value) == msg
+        with pytest.raises(L.PDLError) as ed:
+            s.rangeb(C.ndarray.from_numpy(idx, L.IND), size, bound)
+        assert str(ed.value) == msg
