@@ -597,18 +597,20 @@ __device__ __forceinline__ float pdl_dot_f32(const float *a, int64_t at, const f
   return cc;
 }
 
-// Recomputes with PDL's loop the outputs the GEMM epilogue marked: for each
-// marked row, its columns are re-tested with mm_needs_fix on the stored
-// values, 256 columns at a time.  A chunk with 64 or more marked outputs (a
-// row with an Inf or NaN) has each thread compute its own; fewer, and each
-// wave computes one output at a time, its 64 lanes forming 64 products per
-// step (the next step's loads in flight) that the wave adds in ascending t.
+// Recomputes with PDL's loop the outputs the GEMM epilogue marked: a marked
+// row's columns are re-tested with mm_needs_fix on the stored values,
+// FX_S x 256 columns at a time with all of their loads in flight, and the
+// marked ones listed in LDS.  Many marked (a row with an Inf or NaN): each
+// thread computes its own.  Few: each wave computes one output at a time,
+// its 64 lanes loading 64 products per step, FX_U steps at a time, which
+// the wave adds in ascending t (constant lane indices for full steps).
 // Blocks stride over the rows, so every block reaches the end.
+constexpr int FX_S = 16, FX_U = 16;
 __global__ __launch_bounds__(256) void mm_fixup(const float *a, const float *b, float *c, MmPlan p,
                                                 const float *__restrict__ sA, const float *__restrict__ sB,
                                                 const float *__restrict__ dA, const float *__restrict__ dB,
                                                 const int *__restrict__ rowflag) {
-  __shared__ int list[256];
+  __shared__ int list[256 * FX_S];
   __shared__ int nlist;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   for (int64_t h = blockIdx.x; h < p.H; h += gridDim.x) {
@@ -616,17 +618,35 @@ __global__ __launch_bounds__(256) void mm_fixup(const float *a, const float *b, 
     const float *arow = a + p.off[0] + h * p.a_h;
     const int eA = pow2_exp(sA[h]);
     const float da = dA[h];
-    for (int64_t w0 = 0; w0 < p.W; w0 += 256) {
+    float *crow = c + p.off[2] + h * p.c_h;
+    for (int64_t w0 = 0; w0 < p.W; w0 += 256 * FX_S) {
       if (tid == 0) nlist = 0;
       __syncthreads();
-      const int64_t wc = w0 + tid;
-      bool mark = false;
-      if (wc < p.W) mark = mm_needs_fix(c[p.off[2] + wc * p.c_w + h * p.c_h], da + dB[wc], eA + pow2_exp(sB[wc]));
-      if (mark) list[atomicAdd(&nlist, 1)] = tid;
+      float cv[FX_S], dv[FX_S], sv[FX_S];
+#pragma unroll
+      for (int k = 0; k < FX_S; k++) {
+        const int64_t col = w0 + k * 256 + tid;
+        const bool in = col < p.W;
+        cv[k] = in ? crow[col * p.c_w] : 0.0f;
+        dv[k] = in ? dB[col] : 0.0f;
+        sv[k] = in ? sB[col] : 1.0f;
+      }
+      unsigned marks = 0;
+#pragma unroll
+      for (int k = 0; k < FX_S; k++)
+        if (w0 + k * 256 + tid < p.W && mm_needs_fix(cv[k], da + dv[k], eA + pow2_exp(sv[k]))) marks |= 1u << k;
+      const int mine = __builtin_popcount(marks);
+      if (mine) {
+        int at = atomicAdd(&nlist, mine);
+        for (unsigned m = marks; m; m &= m - 1) list[at++] = __builtin_ctz(m) * 256 + tid;
+      }
       __syncthreads();
       const int n = nlist;
-      if (n >= 64) {
-        if (mark) c[p.off[2] + wc * p.c_w + h * p.c_h] = pdl_dot_f32(arow, p.a_t, b + p.off[1] + wc * p.b_w, p.b_t, p.T);
+      if (n >= 128) {
+        for (unsigned m = marks; m; m &= m - 1) {
+          const int64_t col = w0 + __builtin_ctz(m) * 256 + tid;
+          crow[col * p.c_w] = pdl_dot_f32(arow, p.a_t, b + p.off[1] + col * p.b_w, p.b_t, p.T);
+        }
       } else {
         for (int i = w; i < n; i += 4) {
           const int64_t col = w0 + list[i];
@@ -635,17 +655,29 @@ __global__ __launch_bounds__(256) void mm_fixup(const float *a, const float *b, 
             const int64_t t = t0 + lane;
             return t < p.T ? arow[t * p.a_t] * bcol[t * p.b_t] : 0.0f;
           };
-          float cc = 0.0f, cur = prod_at(0);
-          for (int64_t t0 = 0; t0 < p.T; t0 += 64) {
-            const float nxt = t0 + 64 < p.T ? prod_at(t0 + 64) : 0.0f;
-            const int m = p.T - t0 < 64 ? (int)(p.T - t0) : 64;
-            for (int l = 0; l < m; l++) cc = cc + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur), l));
-            cur = nxt;
+          float cc = 0.0f;
+          for (int64_t t0 = 0; t0 < p.T; t0 += 64 * FX_U) {
+            float v[FX_U];
+#pragma unroll
+            for (int u = 0; u < FX_U; u++) v[u] = prod_at(t0 + 64 * u);
+            if (t0 + 64 * FX_U <= p.T) {
+#pragma unroll
+              for (int u = 0; u < FX_U; u++)
+#pragma unroll
+                for (int l = 0; l < 64; l++) cc = cc + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[u]), l));
+            } else {
+#pragma unroll
+              for (int u = 0; u < FX_U; u++) {
+                const int64_t tb = t0 + 64 * u;
+                const int m = p.T - tb < 64 ? (p.T - tb > 0 ? (int)(p.T - tb) : 0) : 64;
+                for (int l = 0; l < m; l++) cc = cc + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[u]), l));
+              }
+            }
           }
-          if (lane == 0) c[p.off[2] + col * p.c_w + h * p.c_h] = cc;
+          if (lane == 0) crow[col * p.c_w] = cc;
         }
       }
-      __syncthreads();   // the list is refilled for the next chunk
+      __syncthreads();   // the list is refilled for the next columns
     }
   }
 }
