@@ -1186,6 +1186,20 @@ static int dim_checks(trans *tr, const int *creating, int load_only, int nimpl, 
   return 0;
 }
 
+/* pdl->broadcastids[k]: a plain ndarray (nbroadcastids 0 here) has the one
+ * entry ndims; slots past the list are the zeroed def_broadcastids of
+ * pdl_pdlnew (pdlapi.c:211-214, 226-228) */
+static int64_t bid_at(const pdlc_pdl *p, int k) {
+  int n = p->nbroadcastids > 0 ? p->nbroadcastids : 1;
+  if (n > PDLC_MAXBIDS) n = PDLC_MAXBIDS;
+  if (k >= n) return 0;
+  return p->nbroadcastids > 0 ? p->broadcastids[k] : p->ndims;
+}
+static int nbids(const pdlc_pdl *p) {
+  int n = p->nbroadcastids > 0 ? p->nbroadcastids : 1;
+  return n > PDLC_MAXBIDS ? PDLC_MAXBIDS : n;
+}
+
 static int redodims(trans *tr, const int *creating, int nthreads, int pthread_size_mi, pdlc_error *err) {
   const vtable *vt = tr->vt;
   brc_t *b = &tr->brc;
@@ -1194,13 +1208,37 @@ static int redodims(trans *tr, const int *creating, int nthreads, int pthread_si
   memset(b, 0, sizeof *b);
   b->npdls = npdls;
   b->mag_nth = b->mag_nthpdl = b->mag_nthr = -1;
-  int nimpl = 0;
+  /* pdl_initbroadcaststruct (pdlbroadcast.c:354-395) */
+  int nimpl = 0, nids = 0;
   for (j = 0; j < npdls; j++) {
     if (creating[j]) continue;
-    int nb = pdls[j]->ndims - vt->par_realdims[j];
+    /* lists that do not ascend to ndims let one dim serve two broadcast
+     * dims (walked past the ndarray's end): refused, as the device does */
+    pdlc_pdl *p = pdls[j];
+    if (p->nbroadcastids > PDLC_MAXBIDS)
+      return make_error(err, 1, "pdlhip: %s: parameter '%s' has more than %d broadcast ids", vt->name,
+                        vt->par_names[j], PDLC_MAXBIDS);
+    for (int k = 0; k < nbids(p); k++)
+      if (bid_at(p, k) < (k ? bid_at(p, k - 1) : 0) || bid_at(p, k) > p->ndims ||
+          (k == nbids(p) - 1 && bid_at(p, k) != p->ndims))
+        return make_error(err, 1, "pdlhip: %s: parameter '%s': broadcast ids not ascending to ndims %d", vt->name,
+                          vt->par_names[j], p->ndims);
+    if (nbids(pdls[j]) > nids) nids = nbids(pdls[j]);
+    int nb = (int)bid_at(pdls[j], 0) - vt->par_realdims[j];
     if (nb > nimpl) nimpl = nb;
   }
   b->nimpl = nimpl;
+  /* explicit dims per id: every ndarray's counts add to ndims, the last
+   * ndarray's counts are the ones looped over (pdlbroadcast.c:389-392) */
+  int64_t nbroadcastids[PDLC_MAXBIDS] = {0}, nexpl = 0;
+  for (j = 0; j < npdls; j++) {
+    if (creating[j]) continue;
+    for (int i = 0; i < nids; i++) {
+      int64_t d = bid_at(pdls[j], i + 1) - bid_at(pdls[j], i);
+      nbroadcastids[i] = d > 0 ? d : 0;
+      nexpl += nbroadcastids[i];
+    }
+  }
   /* pdl_autopthreadmagic (pdlbroadcast.c:168-209) */
   int mag_pdl = -1, mag_dim = -1, mag_thr = 0;
   if (nthreads > 0) {
@@ -1216,7 +1254,7 @@ static int redodims(trans *tr, const int *creating, int nthreads, int pthread_si
       if (maxp > 1) mag_thr = maxp;
     }
   }
-  int ndims = nimpl;
+  int ndims = nimpl + (int)nexpl;
   b->nextra = 2 - ndims > 0 ? 2 - ndims : 0;
   ndims += b->nextra;
   if (ndims > MAXB) return make_error(err, 2, "too many broadcast dims");
@@ -1234,7 +1272,7 @@ static int redodims(trans *tr, const int *creating, int nthreads, int pthread_si
       int isoutput = j >= vt->nparents;
       int rd = vt->par_realdims[j];
       pdlc_pdl *p = pdls[j];
-      if (nth >= p->ndims - rd) {
+      if (nth >= bid_at(p, 0) - rd) {
         if (isoutput && b->dims[nth] != 1)
           return param_error(err, vt, nth, pdls, nimpl, creating,
                              "implicit dim %d size %lld, can't broadcast over output ndarray with size > 1",
@@ -1261,6 +1299,34 @@ static int redodims(trans *tr, const int *creating, int nthreads, int pthread_si
       }
     }
   }
+  /* the explicit ids' dims (pdlbroadcast.c:437-467) */
+  nth = nimpl;
+  for (int id = 0; id < nids; id++) {
+    for (int64_t i = 0; i < nbroadcastids[id]; i++, nth++) {
+      for (j = 0; j < npdls; j++) {
+        if (creating[j]) continue;
+        pdlc_pdl *p = pdls[j];
+        if (nbids(p) < id) continue;
+        if (bid_at(p, id + 1) - bid_at(p, id) <= i) continue;
+        int64_t which = i + bid_at(p, id);
+        if (which < 0 || which >= p->ndims) return make_error(err, 1, "broadcast id names a dim past the ndarray");
+        int64_t mydim = p->dims[which];
+        if (mydim == 1) continue;
+        if (which < vt->par_realdims[j])
+          return make_error(err, 1, "pdlhip: %s: parameter '%s': dim %lld is both an active dim and in broadcast id %d",
+                            vt->name, vt->par_names[j], (long long)which, id + 1);
+        if (b->dims[nth] == 1) {
+          b->dims[nth] = mydim;
+        } else if (b->dims[nth] != mydim) {
+          int64_t at = i + vt->par_realdims[j];
+          return param_error(err, vt, j, pdls, nimpl, creating,
+                             "Mismatched implicit broadcast dimension %lld: should be %lld, is %lld", (long long)i,
+                             (long long)b->dims[nth], (long long)(at < p->ndims ? p->dims[at] : 0));
+        }
+        b->incs[nth * npdls + j] = p->incs[which];
+      }
+    }
+  }
   /* thread split of the magicked dim (pdlbroadcast.c:469-484) */
   if (b->mag_nthr > 0) {
     int64_t md = b->dims[b->mag_nth];
@@ -1279,9 +1345,13 @@ static int redodims(trans *tr, const int *creating, int nthreads, int pthread_si
   /* pdl_broadcast_create_parameter (pdlbroadcast.c:490-523) */
   for (int i = 0; i < npdls; i++) {
     if (!creating[i]) continue;
+    if (nimpl != b->ndims - b->nextra)
+      return make_error(err, 1, "%s: trying to create parameter '%s' while explicitly broadcasting.\n"
+                        "See the manual for why this is impossible", vt->name, vt->par_names[i]);
     pdlc_pdl *p = pdls[i];
     int rd = vt->par_realdims[i];
     if (rd + nimpl > PDLC_MAXDIMS) return make_error(err, 2, "too many dims");
+    p->nbroadcastids = 0;
     p->ndims = rd + nimpl;
     for (j = 0; j < rd; j++) p->dims[j] = tr->ind_sizes[vt->par_ind[i][j]];
     for (j = 0; j < nimpl; j++) p->dims[rd + j] = b->dims[j];
@@ -1509,6 +1579,9 @@ int pdlc_run(const char *op, pdlc_pdl **user_pdls, int npdls, const int64_t *oth
     c->ndims = p->ndims;
     int64_t inc = 1;
     for (j = 0; j < p->ndims; j++) { c->dims[j] = p->dims[j]; c->incs[j] = inc; inc *= p->dims[j]; }
+    /* the converted child keeps the broadcast ids (pdlconv.c:156-158) */
+    c->nbroadcastids = p->nbroadcastids;
+    memcpy(c->broadcastids, p->broadcastids, sizeof c->broadcastids);
     c->data = calloc(inc ? inc : 1, type_sizes[c->datatype]);
     conv_ctx cx;
     cx.src = p; cx.dst = c->data; cx.dsttype = c->datatype; cx.bad = (p->state & PDLC_BADVAL) != 0;

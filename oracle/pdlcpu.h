@@ -45,6 +45,7 @@ enum {
 #define PDLC_VAFFINE  0x4   /* this is a virtual-affine view (has vafftrans) */
 
 #define PDLC_MAXDIMS 16
+#define PDLC_MAXBIDS 8
 
 typedef union {
   int8_t A; uint8_t B; int16_t S; uint16_t U; int32_t L; uint32_t K;
@@ -70,6 +71,10 @@ typedef struct {
   int64_t offs;
   void *data;
   pdlc_anyval badvalue;
+  /* pdl->broadcastids / nbroadcastids (pdl.h.PL:569-570); 0 = one id, ndims */
+  int32_t nbroadcastids;
+  int32_t pad2_;
+  int64_t broadcastids[PDLC_MAXBIDS];
 } pdlc_pdl;
 
 typedef struct {

@@ -40,7 +40,9 @@ class CPdl(ctypes.Structure):
     _fields_ = [("datatype", ctypes.c_int32), ("state", ctypes.c_int32), ("ndims", ctypes.c_int32),
                 ("has_badvalue", ctypes.c_int32), ("dims", ctypes.c_int64 * MAXDIMS),
                 ("incs", ctypes.c_int64 * MAXDIMS), ("offs", ctypes.c_int64),
-                ("data", ctypes.c_void_p), ("badvalue", Anyval)]
+                ("data", ctypes.c_void_p), ("badvalue", Anyval),
+                ("nbroadcastids", ctypes.c_int32), ("pad2_", ctypes.c_int32),
+                ("broadcastids", ctypes.c_int64 * 8)]
 
 
 class CPlan(ctypes.Structure):
@@ -124,6 +126,7 @@ class OPdl:
         self.state = state
         self.badvalue = badvalue    # None -> type default
         self.root_dims = list(self.dims)
+        self.bids = None            # pdl->broadcastids when set by broadcastI
 
     # ---- constructors ------------------------------------------------------
     @classmethod
@@ -215,6 +218,66 @@ class OPdl:
                 offs += a * self.incs[d]
         return self._view(dims, incs, offs)
 
+    # ---- explicit broadcast ids --------------------------------------------------
+    def broadcastids(self):
+        return list(self.bids) if self.bids else [len(self.dims)]
+
+    def broadcastI(self, bid, whichdims):
+        """Slices.pd:1819-1898 RedoDims, step for step: the child keeps the
+        parent's ndims; the dims not named go in order, leaving room for
+        the named ones at broadcastids[bid] (or the end), which then go
+        there; broadcastids shift by the number moved."""
+        wd = list(whichdims)
+        for i in range(len(wd)):
+            for j in range(i + 1, len(wd)):
+                if wd[i] == wd[j] and wd[i] != -1:
+                    raise OracleError("Error in broadcastI:duplicate arg %d %d %d" % (i, j, wd[i]))
+        nrealwhichdims = sum(1 for w in wd if w != -1)
+        pbids, nd = self.broadcastids(), len(self.dims)
+        dims, incs = [None] * nd, [None] * nd
+        nthc = 0
+        for i in range(nd):
+            if len(pbids) > bid and bid >= 0 and i == pbids[bid]:
+                nthc += len(wd)
+            if i in wd:
+                continue
+            dims[nthc], incs[nthc] = self.dims[i], self.incs[i]
+            nthc += 1
+        for i in range(len(wd)):
+            cdim = i + (pbids[bid] if len(pbids) > bid and bid >= 0 else nd) - nrealwhichdims
+            if wd[i] == -1:
+                dims[cdim], incs[cdim] = 1, 0
+            else:
+                dims[cdim], incs[cdim] = self.dims[wd[i]], self.incs[wd[i]]
+        if None in dims:
+            raise OracleError("broadcastI leaves a dim unset")
+        n = max(bid + 1, len(pbids))
+        cb = []
+        for i in range(n - 1):
+            cb.append((pbids[i] if len(pbids) > i else nd) +
+                      (-nrealwhichdims if i <= bid else len(wd) - nrealwhichdims))
+        cb.append(nd)
+        v = self._view(dims, incs, self.offs)
+        v.bids = cb
+        return v
+
+    def unbroadcast(self, atind):
+        """Slices.pd:1900-1925"""
+        nd, b0 = len(self.dims), self.broadcastids()[0]
+        dims, incs = [None] * nd, [None] * nd
+        for i in range(nd):
+            corc = i if i < atind else (i + nd - b0 if i < b0 else i - b0 + atind)
+            dims[corc], incs[corc] = self.dims[i], self.incs[i]
+        return self._view(dims, incs, self.offs)
+
+    def dimstr(self):
+        """Core.pm:2105-2117"""
+        ids = self.broadcastids()
+        out = "D [%s]" % ",".join(map(str, self.dims[:ids[0]]))
+        for i in range(1, len(ids)):
+            out += " T%d [%s]" % (i, ",".join(map(str, self.dims[ids[i - 1]:ids[i]])))
+        return out
+
     # ---- data ------------------------------------------------------------------
     def gather(self):
         """dense numpy copy, shape = reversed dims"""
@@ -252,6 +315,10 @@ class OPdl:
             c.dims[i] = d
             c.incs[i] = self.incs[i]
         c.offs = self.offs
+        if self.bids:
+            c.nbroadcastids = len(self.bids)
+            for i, v in enumerate(self.bids):
+                c.broadcastids[i] = v
         c.data = self.root.ctypes.data if self.root is not None else None
         if self.badvalue is not None:
             c.has_badvalue = 1

@@ -12,6 +12,7 @@ LIBPATH = os.path.join(HERE, "libpdlhip.so")
 SB, B, S, US, L, UL, IND, ULL, LL, F, D, LD, CF, CD, CLD = range(15)
 NOMYDIMS, BADVAL, VAFFINE = 1, 2, 4
 MAXDIMS = 16
+MAXBIDS = 8
 ENONE, EUSERERROR, EFATAL = 0, 1, 2
 MM_SPLIT16, MM_FP32, MM_PDLORDER, MM_FP8 = 0, 1, 2, 3
 
@@ -32,7 +33,9 @@ class Pdl(ctypes.Structure):
     _fields_ = [("datatype", ctypes.c_int32), ("state", ctypes.c_int32), ("ndims", ctypes.c_int32),
                 ("has_badvalue", ctypes.c_int32), ("dims", ctypes.c_int64 * MAXDIMS),
                 ("incs", ctypes.c_int64 * MAXDIMS), ("offs", ctypes.c_int64),
-                ("data", ctypes.c_void_p), ("badvalue", Anyval)]
+                ("data", ctypes.c_void_p), ("badvalue", Anyval),
+                ("nbroadcastids", ctypes.c_int32), ("pad2_", ctypes.c_int32),
+                ("broadcastids", ctypes.c_int64 * MAXBIDS)]
 
 
 class Error(ctypes.Structure):

@@ -24,6 +24,9 @@ from ._lib import PDLError
 
 TYPE_NAMES = ["sbyte", "byte", "short", "ushort", "long", "ulong", "indx", "ulonglong", "longlong",
               "float", "double", "ldouble", "cfloat", "cdouble", "cldouble"]
+# the types' ctype names without PDL_ (Types.pm:31-200, shortctype at 265)
+SHORT_CTYPES = ["SByte", "Byte", "Short", "Ushort", "Long", "ULong", "Indx", "ULongLong", "LongLong",
+                "Float", "Double", "LDouble", "CFloat", "CDouble", "CLDouble"]
 NP_TYPES = {L.SB: np.int8, L.B: np.uint8, L.S: np.int16, L.US: np.uint16, L.L: np.int32, L.UL: np.uint32,
             L.IND: np.int64, L.ULL: np.uint64, L.LL: np.int64, L.F: np.float32, L.D: np.float64}
 _FROM_NP = {np.dtype(np.int8): L.SB, np.dtype(np.uint8): L.B, np.dtype(np.int16): L.S,
@@ -265,6 +268,7 @@ class ndarray:
         self.state = L.NOMYDIMS
         self.badvalue_ = None
         self._parent = None
+        self.bids = None          # explicit broadcast ids (broadcastI); None: a plain ndarray
 
     __hash__ = object.__hash__
 
@@ -379,6 +383,98 @@ class ndarray:
 
     def transpose(self):
         return self.dummy(0) if self.ndims < 2 else self.xchg(0, 1)
+
+    # ------------------------------------------------------------ explicit broadcasting
+    def broadcastids(self):
+        """pdl->broadcastids (Core.xs:1278-1290): [ndims] for a plain ndarray"""
+        return list(self.bids) if self.bids else [self.ndims]
+
+    def broadcastI(self, bid, whichdims):
+        """Slices.pd:1819-1898: a view whose dims `whichdims` (in that order)
+        form explicit broadcast id `bid`, placed after the dims of the ids
+        below it; the other dims keep their order.  Every view op other than
+        broadcastI and unbroadcast gives a plain ndarray again (its dims are
+        set with one broadcast id, pdlapi.c:613)."""
+        wd = [int(x) for x in whichdims]
+        for i in range(len(wd)):
+            for j in range(i + 1, len(wd)):
+                if wd[i] == wd[j] and wd[i] != -1:
+                    raise PDLError(L.EUSERERROR, "Error in broadcastI:duplicate arg %d %d %d" % (i, j, wd[i]))
+        nd = self.ndims
+        if any(w == -1 for w in wd):
+            raise PDLError(L.EUSERERROR, "broadcastI: a whichdims entry of -1 is not supported")
+        for w in wd:
+            if not 0 <= w < nd:
+                raise PDLError(L.EUSERERROR, "broadcastI: dim %d out of range (ndarray has %d dims)" % (w, nd))
+        pb = self.broadcastids()
+        pn = len(pb)
+        nreal, count = len(wd), len(wd)
+        dims, incs = [0] * nd, [0] * nd
+        nthc = 0
+        for i in range(nd):
+            if pn > bid >= 0 and i == pb[bid]:
+                nthc += count
+            if i in wd:
+                continue
+            dims[nthc], incs[nthc] = self.dims[i], self.incs[i]
+            nthc += 1
+        base = pb[bid] if pn > bid >= 0 else nd
+        for i, w in enumerate(wd):
+            dims[i + base - nreal], incs[i + base - nreal] = self.dims[w], self.incs[w]
+        # the moved dims land where the others left room only when none of
+        # them lies past broadcastids[bid]; PDL leaves a hole otherwise
+        if pn > bid >= 0 and base < nd and any(w >= base for w in wd):
+            raise PDLError(L.EUSERERROR, "broadcastI: dims %s lie past broadcast id %d's start %d" % (wd, bid, base))
+        cn = max(bid + 1, pn)
+        if cn > L.MAXBIDS:
+            raise PDLError(L.EUSERERROR, "broadcastI: at most %d broadcast ids" % L.MAXBIDS)
+        bids = [(pb[i] if pn > i else nd) + (-nreal if i <= bid else count - nreal) for i in range(cn - 1)]
+        v = self._view(dims, incs, self.offs)
+        v.bids = bids + [nd]
+        return v
+
+    def broadcast(self, *dims):
+        """explicit broadcasting over dims with broadcast id 1 (Core.pm:1995-1998)"""
+        return self.broadcastI(1, dims)
+
+    broadcast1 = broadcast
+
+    def broadcast2(self, *dims):
+        """Core.pm:2035-2038"""
+        return self.broadcastI(2, dims)
+
+    def broadcast3(self, *dims):
+        """Core.pm:2055-2058"""
+        return self.broadcastI(3, dims)
+
+    def unbroadcast(self, atind):
+        """Slices.pd:1900-1925: the broadcast-id dims (from broadcastids[0]
+        on) move back to position atind; a plain ndarray comes out"""
+        nd, b0 = self.ndims, self.broadcastids()[0]
+        dims, incs = [0] * nd, [0] * nd
+        for i in range(nd):
+            if i < atind:
+                corc = i
+            elif i < b0:
+                corc = i + nd - b0
+            else:
+                corc = i - b0 + atind
+            dims[corc], incs[corc] = self.dims[i], self.incs[i]
+        return self._view(dims, incs, self.offs)
+
+    def dimstr(self):
+        """Core.pm:2105-2117: 'D [..]' then ' Tk [..]' per explicit id"""
+        ids = self.broadcastids()
+        out = "D [" + ",".join(str(d) for d in self.dims[:ids[0]]) + "]"
+        for i in range(1, len(ids)):
+            out += " T%d [" % i + ",".join(str(d) for d in self.dims[ids[i - 1]:ids[i]]) + "]"
+        return out
+
+    def info(self):
+        """PDL::info with the default format '%C: %T %D' (Core.pm:56, 2226-2241)"""
+        if self.isnull():
+            return "PDL->null"
+        return "PDL: %s %s" % (SHORT_CTYPES[self.dtype], self.dimstr())
 
     def slice(self, *specs):
         """per dim: None (all), int k ('(k)', drops the dim) or (start, stop, step) with an
@@ -636,6 +732,10 @@ class ndarray:
             c.dims[i] = d
             c.incs[i] = self.incs[i]
         c.offs = self.offs
+        if self.bids:
+            c.nbroadcastids = len(self.bids)
+            for i, v in enumerate(self.bids):
+                c.broadcastids[i] = v
         c.data = self._buf.ptr if self._buf is not None else None
         if self.badvalue_ is not None:
             c.has_badvalue = 1

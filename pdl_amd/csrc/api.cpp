@@ -500,18 +500,59 @@ static pdlhip_error dim_checks(const Sig &s, pdlhip_pdl **pdls, const bool *crea
   return ok();
 }
 
-// pdl_initbroadcaststruct (implicit broadcast dims, nobl = 2) + checks +
-// output creation + core inc sizes.  The CPU pthread split is not planned:
-// the whole broadcast is one device launch.
+// entry k of an ndarray's broadcast-id list: a plain ndarray has the one
+// entry ndims; entries past the list read as 0, as PDL's zeroed
+// def_broadcastids slots do (pdlapi.c:211-214)
+static int64_t bcast_id(const pdlhip_pdl *p, int k) {
+  const int n = p->nbroadcastids > 0 ? std::min(p->nbroadcastids, PDLHIP_MAXBIDS) : 1;
+  if (k >= n) return 0;
+  return p->nbroadcastids > 0 ? p->broadcastids[k] : p->ndims;
+}
+static int bcast_nids(const pdlhip_pdl *p) {
+  return p->nbroadcastids > 0 ? std::min(p->nbroadcastids, PDLHIP_MAXBIDS) : 1;
+}
+
+// pdl_initbroadcaststruct (implicit broadcast dims, then the dims of the
+// explicit broadcast ids, nobl = 2) + checks + output creation + core inc
+// sizes.  The CPU pthread split is not planned: the whole broadcast is one
+// device launch.
 static pdlhip_error redodims(const Sig &s, pdlhip_pdl **pdls, const bool *creating, Brc &b) {
   const int np = s.npdls;
   b.npdls = np;
-  int nimpl = 0;
-  for (int j = 0; j < np; j++)
-    if (!creating[j]) nimpl = std::max(nimpl, pdls[j]->ndims - s.realdims[j]);
+  int nimpl = 0, nids = 0;
+  for (int j = 0; j < np; j++) {
+    if (creating[j]) continue;
+    // an id list must ascend within [0, ndims] and end at ndims (as
+    // broadcastI's do), so that no dim serves two broadcast dims
+    const pdlhip_pdl *p = pdls[j];
+    if (p->nbroadcastids > PDLHIP_MAXBIDS)
+      return make_err(PDLHIP_EUSERERROR, "pdlhip: %s: parameter '%s' has more than %d broadcast ids", s.name.c_str(),
+                      s.par_names[j].c_str(), PDLHIP_MAXBIDS);
+    for (int k = 0; k < bcast_nids(p); k++)
+      if (bcast_id(p, k) < (k ? bcast_id(p, k - 1) : 0) || bcast_id(p, k) > p->ndims ||
+          (k == bcast_nids(p) - 1 && bcast_id(p, k) != p->ndims))
+        return make_err(PDLHIP_EUSERERROR, "pdlhip: %s: parameter '%s': broadcast ids not ascending to ndims %d",
+                        s.name.c_str(), s.par_names[j].c_str(), p->ndims);
+    nids = std::max(nids, bcast_nids(pdls[j]));
+    nimpl = std::max<int>(nimpl, (int)(bcast_id(pdls[j], 0) - s.realdims[j]));
+  }
   if (nimpl > PDLHIP_MAXDIMS) return make_err(PDLHIP_EFATAL, "pdlhip: too many broadcast dims");
+  // dims per explicit id (pdlbroadcast.c:389-392): PDL sums every ndarray's
+  // counts into ndims but loops over the last ndarray's counts only; the
+  // other dims it adds stay size 1 and are not planned here
+  int64_t nexpl_all = 0, nexpl[PDLHIP_MAXBIDS] = {0};
+  for (int j = 0; j < np; j++) {
+    if (creating[j]) continue;
+    for (int i = 0; i < nids; i++) {
+      nexpl[i] = std::max<int64_t>(0, bcast_id(pdls[j], i + 1) - bcast_id(pdls[j], i));
+      nexpl_all += nexpl[i];
+    }
+  }
+  int nplanned = nimpl;
+  for (int i = 0; i < nids; i++) nplanned += (int)nexpl[i];
+  if (nplanned > PDLHIP_MAXDIMS + 2) return make_err(PDLHIP_EFATAL, "pdlhip: too many broadcast dims");
   b.nimpl = nimpl;
-  b.ndims = nimpl + std::max(0, 2 - nimpl);
+  b.ndims = nplanned + std::max(0, 2 - nplanned);
   for (int d = 0; d < b.ndims; d++) b.dims[d] = 1;
   for (int k = 0; k < b.ndims * np; k++) b.incs[k] = 0;
   // pdl_broadcast_dim_checks (pdlbroadcast.c:275-327)
@@ -521,7 +562,7 @@ static pdlhip_error redodims(const Sig &s, pdlhip_pdl **pdls, const bool *creati
       const pdlhip_pdl *p = pdls[j];
       bool isout = j >= s.nparents;
       int rd = s.realdims[j];
-      if (nth >= p->ndims - rd) {
+      if (nth >= bcast_id(p, 0) - rd) {
         if (isout && b.dims[nth] != 1)
           return param_error(s, nth, pdls, nimpl, creating,
                              "implicit dim %d size %lld, can't broadcast over output ndarray with size > 1", nth,
@@ -548,6 +589,39 @@ static pdlhip_error redodims(const Sig &s, pdlhip_pdl **pdls, const bool *creati
       }
     }
   }
+  // the explicit ids' dims, after the implicit ones (pdlbroadcast.c:437-467)
+  int nth = nimpl;
+  for (int id = 0; id < nids; id++) {
+    for (int i = 0; i < nexpl[id]; i++, nth++) {
+      for (int j = 0; j < np; j++) {
+        if (creating[j]) continue;
+        const pdlhip_pdl *p = pdls[j];
+        if (bcast_nids(p) < id) continue;
+        if (bcast_id(p, id + 1) - bcast_id(p, id) <= i) continue;
+        const int64_t which = i + bcast_id(p, id);
+        if (which < 0 || which >= p->ndims) return make_err(PDLHIP_EUSERERROR, "pdlhip: broadcast id %d of "
+                                                            "parameter '%s' names dim %lld of %d", id,
+                                                            s.par_names[j].c_str(), (long long)which, p->ndims);
+        const int64_t mydim = p->dims[which];
+        if (mydim == 1) continue;
+        // a dim both active and broadcast would be walked twice, past the
+        // ndarray's end (PDL does not check this)
+        if (which < s.realdims[j])
+          return make_err(PDLHIP_EUSERERROR, "pdlhip: %s: parameter '%s': dim %lld is both an active dim and in "
+                          "broadcast id %d", s.name.c_str(), s.par_names[j].c_str(), (long long)which, id + 1);
+        if (b.dims[nth] == 1) {
+          b.dims[nth] = mydim;
+        } else if (b.dims[nth] != mydim) {
+          // PDL prints the dim at i + realdims as the size found
+          const int at = i + s.realdims[j];
+          return param_error(s, j, pdls, nimpl, creating,
+                             "Mismatched implicit broadcast dimension %d: should be %lld, is %lld", i,
+                             (long long)b.dims[nth], (long long)(at < p->ndims ? p->dims[at] : 0));
+        }
+        b.incs[nth * np + j] = p->incs[which];
+      }
+    }
+  }
   RETURN_IF(dim_checks(s, pdls, creating, false, nimpl, b.ind_sizes));
   for (int i = 0; i < np; i++) {
     if (!creating[i]) continue;
@@ -557,10 +631,16 @@ static pdlhip_error redodims(const Sig &s, pdlhip_pdl **pdls, const bool *creati
   // pdl_broadcast_create_parameter (pdlbroadcast.c:490-523)
   for (int i = 0; i < np; i++) {
     if (!creating[i]) continue;
+    if (nexpl_all > 0)
+      return make_err(PDLHIP_EUSERERROR,
+                      "%s: trying to create parameter '%s' while explicitly broadcasting.\n"
+                      "See the manual for why this is impossible",
+                      s.name.c_str(), s.par_names[i].c_str());
     pdlhip_pdl *p = pdls[i];
     int rd = s.realdims[i];
     if (rd + nimpl > PDLHIP_MAXDIMS) return make_err(PDLHIP_EFATAL, "pdlhip: too many dims");
     p->ndims = rd + nimpl;
+    p->nbroadcastids = 0;
     for (int j = 0; j < rd; j++) p->dims[j] = b.ind_sizes[s.ind[i][j]];
     for (int j = 0; j < nimpl; j++) p->dims[rd + j] = b.dims[j];
     int64_t inc = 1;
@@ -574,6 +654,40 @@ static pdlhip_error redodims(const Sig &s, pdlhip_pdl **pdls, const bool *creati
   for (int i = 0; i < np; i++)
     for (int j = 0; j < s.realdims[i]; j++)
       b.inc_sizes[i][j] = (pdls[i]->ndims <= j || pdls[i]->dims[j] <= 1) ? 0 : pdls[i]->incs[j];
+  return ok();
+}
+
+// PDL's loop writes an output once per step of a broadcast dim over which
+// the output does not move (inc 0): an output whose dim in an explicit
+// broadcast id is 1 or a dummy (pdlbroadcast.c:437-467 checks only the
+// sizes), or an output the implicit checks met before a later operand set
+// the size.  The last write stays, so the plan keeps that dim's last index
+// only.  Refused when outputs disagree on such a dim, or when an input
+// shares an output's buffer (PDL's later steps would read earlier writes).
+static pdlhip_error collapse_repeated_writes(const Sig &s, pdlhip_pdl **pdls, Brc &b) {
+  const int np = s.npdls;
+  for (int d = 0; d < b.ndims; d++) {
+    if (b.dims[d] <= 1) continue;
+    int nzero = 0;
+    for (int j = s.nparents; j < np; j++) nzero += b.incs[d * np + j] == 0;
+    if (nzero == 0) continue;
+    if (nzero != np - s.nparents)
+      return make_err(PDLHIP_EUSERERROR,
+                      "pdlhip: %s: broadcast dim %d of size %lld repeats some outputs' writes but not others': "
+                      "not supported on the device", s.name.c_str(), d, (long long)b.dims[d]);
+    for (int i = 0; i < s.nparents; i++)
+      for (int o = s.nparents; o < np; o++)
+        if (pdls[i]->data && pdls[i]->data == pdls[o]->data)
+          return make_err(PDLHIP_EUSERERROR,
+                          "pdlhip: %s: output '%s' is written repeatedly over broadcast dim %d and shares its "
+                          "buffer with input '%s': not supported on the device", s.name.c_str(),
+                          s.par_names[o].c_str(), d, s.par_names[i].c_str());
+    for (int i = 0; i < s.nparents; i++) {
+      b.offs[i] += (b.dims[d] - 1) * b.incs[d * np + i];
+      b.incs[d * np + i] = 0;
+    }
+    b.dims[d] = 1;
+  }
   return ok();
 }
 
@@ -1201,6 +1315,7 @@ static pdlhip_error run(const Sig &s, pdlhip_pdl **user, int npdls, const int64_
     }
   }
   if (!err.error) err = redodims(s, pdls, creating, b);
+  if (!err.error) err = collapse_repeated_writes(s, pdls, b);
   if (!err.error) {
     if (inputs_bad)
       for (int i = s.nparents; i < npdls; i++) pdls[i]->state |= PDLHIP_BADVAL;

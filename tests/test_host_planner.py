@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_init_without_gpu_fails_loudly_or_succeeds():
     e = L.lib().pdlhip_abi_version()
-    assert e == 1
+    assert e == 2
 
 
 @pytest.fixture
@@ -112,6 +112,37 @@ def test_planner_matches_oracle_random(plan_only):
         assert o == d, (op, specs, swap, o, d)
         n += o[0] == "ok"
     assert n > 300
+
+
+def test_planner_matches_oracle_explicit_ids(plan_only):
+    """operands (outputs included) made with broadcastI over random dims:
+    the explicit-id pass (pdlbroadcast.c:437-467), the last operand's
+    counts, the mismatch and create-while-broadcasting errors"""
+    types = [O.S, O.L, O.F, O.D]
+    n_ok = n_err = 0
+    for trial in range(3000):
+        op = RNG.choice(OPS3[:6] + ["sumover", "minimum", "assgn", "matmult"])
+        nin = 2 if op in OPS3 or op == "matmult" else 1
+        nout_given = RNG.random() < 0.85
+        specs = []
+        for k in range(nin + 1):
+            if k == nin and not nout_given:
+                specs.append(None)
+                continue
+            dims = [int(RNG.choice([1, 2, 3])) for _ in range(int(RNG.integers(1, 5)))]
+            view = []
+            if RNG.random() < 0.7:
+                nd = len(dims)
+                wd = [int(x) for x in RNG.permutation(nd)[:int(RNG.integers(1, nd + 1))]]
+                view.append(("broadcastI", int(RNG.choice([1, 1, 2])), wd))
+            specs.append((int(RNG.choice(types)), dims, 0, view))
+        o, d = both(op, specs)
+        assert o == d, (op, specs, o, d)
+        if o[0] == "ok":
+            n_ok += 1
+        else:
+            n_err += 1
+    assert n_ok > 200 and n_err > 200, (n_ok, n_err)
 
 
 def test_planner_golden_errors(plan_only):
