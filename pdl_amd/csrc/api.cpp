@@ -1651,8 +1651,10 @@ static const Fin *find_final(const char *name) {
 }
 
 // The fused program of a chain over input slots of the given types: PDL's
-// trans type of every node (each must equal the chain's), the left-deep form
-// when the program has one, and the type of the final op's output.
+// trans type of every node (the chain's type is the last node's; a chain
+// whose nodes differ runs each node in its own type), the left-deep form
+// when the program has one and one type, and the type of the final op's
+// output.
 static pdlhip_error chain_setup(const char *who, int nin, const int *stype, int nnodes, const pdlhip_node *nodes,
                                 const Fin &f, RaggedH &h, int *otype) {
   h.scan = f.scan;
@@ -1702,10 +1704,17 @@ static pdlhip_error chain_setup(const char *who, int nin, const int *stype, int 
       pg.lin = 0;
     }
   }
-  for (int j = 0; j < nnodes; j++)
-    if (ntype[j] != T)
-      return make_err(PDLHIP_EUSERERROR, "%s: node %d computes in %s, the chain in %s; run the ops separately", who, j,
-                      type_name(ntype[j]), type_name(T));
+  // nodes in other types than the last: the general form computes each in
+  // its own type
+  pg.mixed = 0;
+  for (int j = 0; j < nnodes; j++) {
+    pg.ntype[j] = ntype[j];
+    if (!type_on_device(ntype[j]))
+      return make_err(PDLHIP_EUSERERROR, "%s: node %d computes in %s, not supported on the device", who, j,
+                      type_name(ntype[j]));
+    if (ntype[j] != T) pg.mixed = 1;
+  }
+  if (pg.mixed) pg.lin = 0;
   if (!f.scan && T != PDLHIP_F && T != PDLHIP_D)
     return make_err(PDLHIP_EUSERERROR, "%s: %s of a %s chain: float or double only", who, f.name, type_name(T));
   if (f.scan && T >= PDLHIP_F)

@@ -107,6 +107,10 @@ struct RagProg {
   // (operands swapped when lswap[k])
   int32_t lin, first;
   int32_t lslot[RAG_MAXNODE], lswap[RAG_MAXNODE];
+  // nodes computing in different types (PDL's trans type of each op):
+  // the general form, each node in ntype[k] with its operands converted
+  int32_t mixed;
+  int32_t ntype[RAG_MAXNODE];
   int32_t store;                 // write every chain value to the item's `val`
   int32_t stream;                // streaming loads/stores (the list is larger than the infinity cache)
   pdlhip_value out_bv;

@@ -118,12 +118,67 @@ template <class T> __device__ inline T run_op(int op, T a, T b) {
   case OP_GE: return binop<OP_GE, T>(a, b);
   case OP_EQ: return binop<OP_EQ, T>(a, b);
   case OP_NE: return binop<OP_NE, T>(a, b);
+  case OP_SHL: return binop<OP_SHL, T>(a, b);
+  case OP_SHR: return binop<OP_SHR, T>(a, b);
+  case OP_OR: return binop<OP_OR, T>(a, b);
+  case OP_AND: return binop<OP_AND, T>(a, b);
+  case OP_XOR: return binop<OP_XOR, T>(a, b);
   case OP_POWER: return binop<OP_POWER, T>(a, b);
   case OP_ATAN2: return binop<OP_ATAN2, T>(a, b);
   case OP_MODULO: return binop<OP_MODULO, T>(a, b);
   case OP_SPACESHIP: return binop<OP_SPACESHIP, T>(a, b);
   default: return a;
   }
+}
+
+// a node's value in a mixed chain, kept as 64 bits: integers sign- or
+// zero-extended, float and double as their own bits
+template <class S> __device__ inline uint64_t to_raw64(S v) {
+  if constexpr (sizeof(S) == 4 && IsFloat<S>::v) return (uint64_t)__float_as_uint(v);
+  else if constexpr (IsFloat<S>::v) return (uint64_t)__double_as_longlong(v);
+  else if constexpr ((S)-1 > (S)0) return (uint64_t)v;
+  else return (uint64_t)(int64_t)v;
+}
+template <class S> __device__ inline S from_raw64(uint64_t r) {
+  if constexpr (sizeof(S) == 4 && IsFloat<S>::v) return __uint_as_float((uint32_t)r);
+  else if constexpr (IsFloat<S>::v) return __longlong_as_double((long long)r);
+  else return (S)r;
+}
+template <class T> __device__ inline T raw_as(uint64_t r, int type) {
+  switch (type) {
+#define X(TC) case TC: return conv_to<T>(from_raw64<CType<TC>::T>(r));
+    PDLHIP_FOR_EACH_TYPE(X)
+#undef X
+  default: return T(0);
+  }
+}
+
+// operand x of a node, converted to the node's type S (converttypei,
+// pdlconv.c:163-174): slot x >= 0, or node -x-1's value
+template <class S>
+__device__ inline S mixed_operand(const RagProg &pg, const RagItem &it, int64_t i, int x, const uint64_t *v) {
+  return x >= 0 ? load_as<S>(it.ptr[x], pg.type[x], i * it.inc[x]) : raw_as<S>(v[-x - 1], pg.ntype[-x - 1]);
+}
+
+// value of a mixed chain at element i: each node runs in its own type, as
+// PDL runs each op of the chain with its own trans type
+template <class T> __device__ inline T chain_value_mixed(const RagProg &pg, const RagItem &it, int64_t i) {
+  uint64_t v[RAG_MAXNODE];
+  for (int n = 0; n < pg.nnodes; n++) {
+    const int x = pg.node[n].x, y = pg.node[n].y, op = pg.node[n].op;
+    switch (pg.ntype[n]) {
+#define X(TC)                                                                                   \
+  case TC: {                                                                                    \
+    typedef CType<TC>::T S;                                                                     \
+    v[n] = to_raw64<S>(run_op<S>(op, mixed_operand<S>(pg, it, i, x, v), mixed_operand<S>(pg, it, i, y, v))); \
+    break;                                                                                      \
+  }
+      PDLHIP_FOR_EACH_TYPE(X)
+#undef X
+    default: v[n] = 0; break;
+    }
+  }
+  return raw_as<T>(v[pg.nnodes - 1], pg.ntype[pg.nnodes - 1]);
 }
 
 // general program: value of the chain at element i of item `it`
@@ -321,7 +376,7 @@ __global__ __launch_bounds__(256) void rag_reduce(RagProg pg, const RagItem *__r
       }
     } else {
       for (int64_t i = sg.start + lane; i < sg.end; i += 64) {
-        const T x = chain_value<T>(pg, it, i);
+        const T x = pg.mixed ? chain_value_mixed<T>(pg, it, i) : chain_value<T>(pg, it, i);
         if (pg.store) as_global((T *)it.val)[i * it.val_inc] = x;
         if constexpr (RED) part_push<OP, T, Tacc>(p, x, i);
       }

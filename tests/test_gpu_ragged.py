@@ -388,3 +388,75 @@ def test_general_chain_program_matches_eager_ops():
         x = v.gather().astype(np.longdouble)
         truth = float(x.sum())
         assert abs(float(rs.outputs[i].numpy()) - truth) <= sum_bound(truth, float(np.abs(x).sum()), O.F), i
+
+
+# ------------------------------------------------------------------ mixed-type chains
+MIXED = [
+    # (slot types, chain): each node computes in PDL's trans type of its op
+    ([O.L, O.L, O.F], [("mult", 0, 1), ("plus", ("node", 0), 2)]),                # long (wraps) -> float
+    ([O.S, O.US, O.F, O.D], [("minus", 0, 1), ("mult", ("node", 0), 2), ("divide", ("node", 1), 3)]),
+    ([O.F, O.F, O.D], [("or2", 0, 1), ("plus", ("node", 0), 2)]),                 # floats -> longlong -> double
+    ([O.L, O.L, O.D], [("gt", 0, 1), ("mult", ("node", 0), 2)]),                  # a comparison in long, then double
+    ([O.B, O.B, O.F, O.F], [("plus", 0, 1), ("minus", 2, 3), ("mult", ("node", 0), ("node", 1))]),  # byte wraps
+]
+
+
+def mixed_slots(rng, types, n):
+    out_o, out_d = [], []
+    for t in types:
+        dt = O.NP_TYPES[t]
+        if np.dtype(dt).kind == "f":
+            x = (rng.standard_normal(n) * 1000).astype(dt)
+        else:
+            info = np.iinfo(dt)
+            x = rng.integers(max(info.min, -2 ** 40), min(info.max, 2 ** 40), n, endpoint=True).astype(dt)
+        out_o.append(O.OPdl.from_numpy(x, t))
+        out_d.append(C.ndarray.from_numpy(x, t))
+    return out_o, out_d
+
+
+@pytest.mark.parametrize("k", range(len(MIXED)))
+def test_mixed_type_chains_match_the_eager_ops(k):
+    """every node in its own trans type with its operands converted
+    (pdlconv.c:163-174): the chain's values bit for bit with the oracle's
+    eager ops (integer wraps, float-to-integer truncation, comparisons),
+    minimum bit-exact, sums within the compensated-sum bound"""
+    types, chain = MIXED[k]
+    rng = np.random.default_rng(60 + k)
+    lens = lengths(rng, 30)
+    items = [mixed_slots(rng, types, n) for n in lens]
+    rv = C.Ragged([d for _, d in items], chain, "minimum", values=True)
+    rv.run()
+    rs = C.Ragged([d for _, d in items], chain, "sumover")
+    rs.run()
+    for i, (so, _) in enumerate(items):
+        v = eager_chain(oracle_run, so, chain, O.OPdl.null)
+        assert rv.values[i].dtype == v.dtype
+        np.testing.assert_array_equal(rv.values[i].numpy().view(np.uint8), v.gather().view(np.uint8),
+                                      err_msg="item %d" % i)
+        out = O.OPdl.null()
+        O.run("minimum", v, out)
+        assert out.gather().tobytes() == rv.outputs[i].numpy().tobytes(), i
+        x = v.gather().astype(np.longdouble)
+        truth = float(x.sum())
+        assert abs(float(rs.outputs[i].numpy()) - truth) <= sum_bound(truth, float(np.abs(x).sum()), v.dtype), i
+
+
+def test_mixed_type_dense_chain_broadcasts_like_the_eager_ops():
+    """short * short (wrapping in short), plus a broadcast float row, over
+    (300, 257): values bit for bit, maximum per row bit-exact"""
+    rng = np.random.default_rng(70)
+    a = rng.integers(-30000, 30000, (300, 257)).astype(np.int16)
+    b = rng.integers(-30000, 30000, (300, 257)).astype(np.int16)
+    c = rng.standard_normal((1, 257)).astype(np.float32)
+    chain = [("mult", 0, 1), ("plus", ("node", 0), 2)]
+    ch = C.Chain([C.ndarray.from_numpy(a, O.S), C.ndarray.from_numpy(b, O.S), C.ndarray.from_numpy(c, O.F)],
+                 chain, "maximum", value=True)
+    ch.run()
+    so = [O.OPdl.from_numpy(a, O.S), O.OPdl.from_numpy(b, O.S), O.OPdl.from_numpy(c, O.F)]
+    ov = eager_chain(oracle_run, so, chain, O.OPdl.null)
+    assert ch.value.dtype == ov.dtype == O.F
+    np.testing.assert_array_equal(ch.value.numpy(), ov.gather())
+    out = O.OPdl.null()
+    O.run("maximum", ov, out)
+    np.testing.assert_array_equal(ch.result.numpy(), out.gather())

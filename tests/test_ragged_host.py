@@ -50,6 +50,16 @@ def test_double_scalar_promotes_the_chain(plan_only):
     assert r.outputs[0].dtype == O.D
 
 
+def test_mixed_chains_take_each_nodes_type(plan_only):
+    """a node in long feeding a float node: each node keeps PDL's trans
+    type, the chain's (and the reduction's) is the last node's"""
+    r = C.Ragged([[vec(O.L, 4), vec(O.L, 4), vec(O.F, 4)]], [("plus", 0, 1), ("mult", ("node", 0), 2)], "sumover")
+    assert r.outputs[0].dtype == O.F
+    r = C.Ragged([[vec(O.F, 4), vec(O.F, 4), vec(O.D, 4)]],
+                 [("or2", 0, 1), ("plus", ("node", 0), 2)], "minimum", values=True)
+    assert r.outputs[0].dtype == O.D and r.values[0].dtype == O.D
+
+
 def test_refusals(plan_only):
     f4, f5 = vec(O.F, 4), vec(O.F, 5)
     expect("lengths 4 and 5 differ", [[f4, f5]], [("plus", 0, 1)], "sumover")
@@ -58,9 +68,9 @@ def test_refusals(plan_only):
     expect("takes one input and no chain", [[vec(O.L, 4)]], [("plus", 0, 0)], "cumusumover")
     expect("integer types only", [[f4]], [], "cumusumover")
     expect("float or double only", [[vec(O.L, 4)]], [], "sumover")
-    # a node in long feeding a float node: the chain has no single type
-    expect("computes in long", [[vec(O.L, 4), vec(O.L, 4), vec(O.F, 4)]],
-           [("plus", 0, 1), ("mult", ("node", 0), 2)], "sumover")
+    # a mixed chain whose last node is an integer type: no float reduction
+    expect("float or double only", [[vec(O.F, 4), vec(O.F, 4)]], [("plus", 0, 1), ("or2", ("node", 0), 0)],
+           "sumover")
     expect("differs in type or rank", [[vec(O.F, 4)], [vec(O.D, 4)]], [], "sumover")
     expect("is not 1-D", [[C.shell(O.F, [2, 2])]], [], "sumover")
     expect("must exist with type", [[f4]], [], "sumover", outputs=[C.shell(O.D, [])])
