@@ -37,6 +37,7 @@
 #include <float.h>
 #include <limits.h>
 #include <math.h>
+#include <complex.h>
 #include <pthread.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -78,6 +79,9 @@ void pdlc_default_badval(int t, pdlc_anyval *out) {
 }
 
 static int type_supported(int t) { return t >= PDLC_SB && t <= PDLC_D; }
+/* float and double complex (C99 _Complex, as GCC compiles PDL's PP code):
+ * restated for plus, minus, mult, eq, ne, assgn, conversion and matmult */
+static int type_cplx(int t) { return t == PDLC_CF || t == PDLC_CD; }
 
 /* X-macro over the types this restatement computes in:
  * X(code, ctype, ppsym, is_integer, is_unsigned) */
@@ -338,8 +342,28 @@ static inline double tg_atan2_d(double a, double b) { return atan2(a, b); }
 #define UNSIGNED_CASES(X) \
   X(PDLC_B, uint8_t, B) X(PDLC_US, uint16_t, U) X(PDLC_UL, uint32_t, K) X(PDLC_ULL, uint64_t, P)
 
+#define CPLX_BIOP_LOOP(ctype, EXPR) do { \
+  ctype *a = tr->pdls[0]->data, *b = tr->pdls[1]->data, *c = tr->pdls[2]->data; \
+  BLOOP(tr, thr, { ctype av = a[O0], bv = b[O1]; c[O2] = (ctype)(EXPR); }); \
+} while (0)
+
 static void rd_biop(trans *tr, int thr) {
   int op = tr->vt->opcode;
+  if (type_cplx(tr->datatype)) {
+#define CPLX_CASE(OPC, EXPR) \
+    if (op == OPC) { \
+      if (tr->datatype == PDLC_CF) { CPLX_BIOP_LOOP(float complex, EXPR); } \
+      else { CPLX_BIOP_LOOP(double complex, EXPR); } \
+      return; \
+    }
+    CPLX_CASE(OP_PLUS, av + bv)
+    CPLX_CASE(OP_MULT, av * bv)
+    CPLX_CASE(OP_MINUS, av - bv)
+    CPLX_CASE(OP_EQ, av == bv)
+    CPLX_CASE(OP_NE, av != bv)
+#undef CPLX_CASE
+    return;
+  }
 #define CASE_ARITH(OPC, EXPR) \
   if (op == OPC) { switch (tr->datatype) { \
     case PDLC_SB: BIOP_LOOP(int8_t, A, EXPR); break; case PDLC_B: BIOP_LOOP(uint8_t, B, EXPR); break; \
@@ -432,6 +456,12 @@ static void rd_bifunc(trans *tr, int thr) {
 } while (0)
 
 static void rd_assgn(trans *tr, int thr) {
+  if (tr->datatype == PDLC_CF || tr->datatype == PDLC_CD) {
+    const size_t sz = type_sizes[tr->datatype];
+    char *a = tr->pdls[0]->data, *b = tr->pdls[1]->data;
+    BLOOP(tr, thr, { memcpy(b + O1 * sz, a + O0 * sz, sz); });
+    return;
+  }
   switch (tr->datatype) {
 #define X(code, ctype, ppsym) case code: ASSGN_LOOP(ctype, ppsym); break;
     ARITH_CASES(X)
@@ -786,7 +816,38 @@ static void rd_ufunc(trans *tr, int thr) {
     } }); \
 } while (0)
 
+#define CPLX_MATMULT_LOOP(ctype) do { \
+  ctype *a = tr->pdls[0]->data, *b = tr->pdls[1]->data, *c = tr->pdls[2]->data; \
+  const int64_t tsiz = 8 * sizeof(double) / sizeof(ctype); \
+  const int64_t atdi = tr->pdls[0]->incs[0]; \
+  const int64_t btdi = tr->pdls[1]->incs[1]; \
+  const int64_t T_ = tr->ind_sizes[0], H_ = tr->ind_sizes[1], W_ = tr->ind_sizes[2]; \
+  const int64_t a_t = tr->inc_sizes[0][0], a_h = tr->inc_sizes[0][1]; \
+  const int64_t b_w = tr->inc_sizes[1][0], b_t = tr->inc_sizes[1][1]; \
+  const int64_t c_w = tr->inc_sizes[2][0], c_h = tr->inc_sizes[2][1]; \
+  (void)a_t; (void)b_t; \
+  BLOOP(tr, thr, { \
+    for (int64_t ho = 0; ho < H_; ho += tsiz) for (int64_t wo = 0; wo < W_; wo += tsiz) { \
+      int64_t he = ho + tsiz < H_ ? ho + tsiz : H_, we = wo + tsiz < W_ ? wo + tsiz : W_; \
+      for (int64_t h = ho; h < he; h++) for (int64_t w = wo; w < we; w++) \
+        c[O2 + w * c_w + h * c_h] = 0; \
+      for (int64_t to = 0; to < T_; to += tsiz) { \
+        int64_t te = to + tsiz < T_ ? to + tsiz : T_; \
+        for (int64_t h = ho; h < he; h++) for (int64_t w = wo; w < we; w++) { \
+          ctype *cp = c + O2 + w * c_w + h * c_h; \
+          ctype cc = *cp; \
+          const ctype *ad = a + O0 + to * a_t + h * a_h; \
+          const ctype *bd = b + O1 + w * b_w + to * b_t; \
+          for (int64_t t = to; t < te; t++) { cc += *ad * *bd; ad += atdi; bd += btdi; } \
+          *cp = cc; \
+        } \
+      } \
+    } }); \
+} while (0)
+
 static void rd_matmult(trans *tr, int thr) {
+  if (tr->datatype == PDLC_CF) { CPLX_MATMULT_LOOP(float complex); return; }
+  if (tr->datatype == PDLC_CD) { CPLX_MATMULT_LOOP(double complex); return; }
   switch (tr->datatype) {
 #define X(code, ctype, ppsym) case code: MATMULT_LOOP(ctype, ppsym); break;
     ARITH_CASES(X)
@@ -1390,7 +1451,23 @@ static void get_elem_as(const void *base, int t, int64_t off, long double *ld, i
 #define CONV_CASE_SRC(scode, sctype, sppsym, sisint, sisuns) case scode: { \
     sctype sv = ((const sctype *)src)[soff]; \
     switch (dt) { TYPES_REAL_(CONV_CASE_DST) default: break; } break; }
+/* with a complex end: C's conversions (a complex converted to a real type
+ * keeps its real part; through intmax_t for an unsigned target, as
+ * COPYCONVERT writes it) */
+#define TYPES_CPLX_DST(X) TYPES_REAL_(X) X(PDLC_CF, float complex, G, 0, 0) X(PDLC_CD, double complex, C, 0, 0)
+#define CONV_CASE_SRC_C(scode, sctype, sppsym, sisint, sisuns) case scode: { \
+    sctype sv = ((const sctype *)src)[soff]; \
+    switch (dt) { TYPES_CPLX_DST(CONV_CASE_DST) default: break; } break; }
 static void convert_one(const void *src, int st, int64_t soff, void *dst, int dt, int64_t doff) {
+  if (type_cplx(st) || type_cplx(dt)) {
+    switch (st) {
+      TYPES_REAL(CONV_CASE_SRC_C)
+      CONV_CASE_SRC_C(PDLC_CF, float complex, G, 0, 0)
+      CONV_CASE_SRC_C(PDLC_CD, double complex, C, 0, 0)
+    default: break;
+    }
+    return;
+  }
   switch (st) {
     TYPES_REAL(CONV_CASE_SRC)
   default: break;
@@ -1544,8 +1621,15 @@ int pdlc_run(const char *op, pdlc_pdl **user_pdls, int npdls, const int64_t *oth
   /* type coercion */
   int ttype = transtype_select(vt, tr.pdls);
   tr.datatype = ttype;
-  if (!type_supported(ttype))
+  const int cplx_vt = vt->readdata == rd_matmult || vt->readdata == rd_assgn ||
+                      (vt->readdata == rd_biop && (vt->opcode == OP_PLUS || vt->opcode == OP_MINUS ||
+                                                   vt->opcode == OP_MULT || vt->opcode == OP_EQ ||
+                                                   vt->opcode == OP_NE));
+  if (!type_supported(ttype) && !(type_cplx(ttype) && cplx_vt))
     return make_error(err, 1, "pdlcpu: %s: type %s is not supported by this restatement", vt->name,
+                      pdlc_type_name(ttype));
+  if (type_cplx(ttype) && inputs_bad)
+    return make_error(err, 1, "pdlcpu: %s: bad values in %s ndarrays are not restated", vt->name,
                       pdlc_type_name(ttype));
   int creating[MAXP] = {0};
   int req_type[MAXP];
@@ -1566,7 +1650,7 @@ int pdlc_run(const char *op, pdlc_pdl **user_pdls, int npdls, const int64_t *oth
     pdlc_pdl *p = tr.pdls[i];
     if (creating[i]) continue;
     if (p->datatype == req_type[i]) continue;
-    if (!type_supported(p->datatype))
+    if (!type_supported(p->datatype) && !(type_cplx(p->datatype) && cplx_vt))
       return make_error(err, 1, "pdlcpu: type %s is not supported", pdlc_type_name(p->datatype));
     /* inplace: the same pdl as input and output converts once */
     int dup = -1;

@@ -20,7 +20,8 @@ SB, B, S, US, L, UL, IND, ULL, LL, F, D, LD, CF, CD, CLD = range(15)
 TYPE_NAMES = ["sbyte", "byte", "short", "ushort", "long", "ulong", "indx", "ulonglong",
               "longlong", "float", "double", "ldouble", "cfloat", "cdouble", "cldouble"]
 NP_TYPES = {SB: np.int8, B: np.uint8, S: np.int16, US: np.uint16, L: np.int32, UL: np.uint32,
-            IND: np.int64, ULL: np.uint64, LL: np.int64, F: np.float32, D: np.float64}
+            IND: np.int64, ULL: np.uint64, LL: np.int64, F: np.float32, D: np.float64,
+            CF: np.complex64, CD: np.complex128}
 NOMYDIMS, BADVAL, VAFFINE = 1, 2, 4
 MAXDIMS = 16
 

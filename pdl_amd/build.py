@@ -39,6 +39,7 @@ def units():
     u.append(("k_badops.hip", "badops", []))
     u.append(("k_rng.hip", "rng", []))
     u.append(("k_index.hip", "index", []))
+    u.append(("k_complex.hip", "complex", []))
     u.append(("k_mfma.hip", "mfma", []))
     u.append(("k_batch.hip", "batch", []))
     u.append(("k_ragged.hip", "ragged", []))

@@ -28,11 +28,13 @@ TYPE_NAMES = ["sbyte", "byte", "short", "ushort", "long", "ulong", "indx", "ulon
 SHORT_CTYPES = ["SByte", "Byte", "Short", "Ushort", "Long", "ULong", "Indx", "ULongLong", "LongLong",
                 "Float", "Double", "LDouble", "CFloat", "CDouble", "CLDouble"]
 NP_TYPES = {L.SB: np.int8, L.B: np.uint8, L.S: np.int16, L.US: np.uint16, L.L: np.int32, L.UL: np.uint32,
-            L.IND: np.int64, L.ULL: np.uint64, L.LL: np.int64, L.F: np.float32, L.D: np.float64}
+            L.IND: np.int64, L.ULL: np.uint64, L.LL: np.int64, L.F: np.float32, L.D: np.float64,
+            L.CF: np.complex64, L.CD: np.complex128}
 _FROM_NP = {np.dtype(np.int8): L.SB, np.dtype(np.uint8): L.B, np.dtype(np.int16): L.S,
             np.dtype(np.uint16): L.US, np.dtype(np.int32): L.L, np.dtype(np.uint32): L.UL,
             np.dtype(np.int64): L.LL, np.dtype(np.uint64): L.ULL, np.dtype(np.float32): L.F,
-            np.dtype(np.float64): L.D, np.dtype(np.bool_): L.B}
+            np.dtype(np.float64): L.D, np.dtype(np.bool_): L.B, np.dtype(np.complex64): L.CF,
+            np.dtype(np.complex128): L.CD}
 _VALFIELD = {L.SB: "A", L.B: "B", L.S: "S", L.US: "U", L.L: "L", L.UL: "K", L.IND: "N", L.ULL: "P",
              L.LL: "Q", L.F: "F", L.D: "D"}
 

@@ -812,9 +812,54 @@ __global__ void zero_i32(int *p, int n) {
 
 static pdlhip_error readdata_index(const Sig &s, int T, bool bad, const Brc &b, pdlhip_pdl **pdls);
 
+// the ops that take float and double complex on the device
+static bool cplx_op(const Sig &s) {
+  if (s.family == FAM_MM || s.family == FAM_ASSGN) return true;
+  return s.family == FAM_BIOP && (s.op == OP_PLUS || s.op == OP_MINUS || s.op == OP_MULT || s.op == OP_EQ ||
+                                  s.op == OP_NE);
+}
+
+// readdata of a complex op (k_complex.hip); every operand has type T here
+static pdlhip_error readdata_cplx(const Sig &s, int T, const Brc &b, pdlhip_pdl **pdls) {
+  hipError_t e = hipSuccess;
+  if (s.family == FAM_BIOP || s.family == FAM_ASSGN) {
+    const bool as = s.family == FAM_ASSGN;
+    int which[3] = {0, as ? 0 : 1, as ? 1 : 2};
+    Dimset d = brc_dimset(b, which, 3);
+    if (!canonicalise(d, 2)) return ok();
+    EwPlan p;
+    if (!fill_ew(p, d)) return make_err(PDLHIP_EFATAL, "pdlhip: %s: too many distinct dims", s.name.c_str());
+    for (int q = 0; q < 3; q++) p.off[q] = b.offs[which[q]];
+    if (as) e = launch_cplx_convert(T, T, p, pdls[0]->data, pdls[1]->data, g.stream);
+    else e = launch_cplx_binop(T, s.op, p, pdls[0]->data, pdls[1]->data, pdls[2]->data, g.stream);
+  } else {
+    int which[3] = {0, 1, 2};
+    Dimset d = brc_dimset(b, which, 3);
+    MmPlan p;
+    std::memset(&p, 0, sizeof p);
+    p.T = b.ind_sizes[0]; p.H = b.ind_sizes[1]; p.W = b.ind_sizes[2];
+    p.a_t = b.inc_sizes[0][0]; p.a_h = b.inc_sizes[0][1];
+    p.b_w = b.inc_sizes[1][0]; p.b_t = b.inc_sizes[1][1];
+    p.c_w = b.inc_sizes[2][0]; p.c_h = b.inc_sizes[2][1];
+    const bool nonempty = canonicalise(d, 2);
+    if (d.n > MAXK) return make_err(PDLHIP_EFATAL, "pdlhip: matmult: too many distinct dims");
+    p.nd = nonempty ? d.n : 0;
+    p.nbatch = nonempty ? 1 : 0;
+    for (int k = 0; k < p.nd; k++) {
+      p.dims[k] = d.dims[k];
+      p.nbatch *= d.dims[k];
+      for (int q = 0; q < 3; q++) p.inc[q][k] = d.inc[q][k];
+    }
+    for (int q = 0; q < 3; q++) p.off[q] = b.offs[q];
+    e = launch_cplx_matmult(T, p, pdls[0]->data, pdls[1]->data, pdls[2]->data, g.stream);
+  }
+  return e == hipSuccess ? ok() : hip_err(e, s.name.c_str());
+}
+
 // the readdata of one op over a planned broadcast (the pdlhip_readdata body)
 static pdlhip_error readdata(const Sig &s, int T, bool bad, const Brc &b, pdlhip_pdl **pdls, const int64_t *other) {
   if (g.plan_only) return ok();
+  if (type_is_cplx(T) && cplx_op(s) && !bad) return readdata_cplx(s, T, b, pdls);
   if (!type_on_device(T)) return make_err(PDLHIP_EUSERERROR, "pdlhip: %s: type %s is not supported on the device",
                                           s.name.c_str(), type_name(T));
   if (s.family == FAM_INDEX) return readdata_index(s, T, bad, b, pdls);
@@ -1257,9 +1302,12 @@ static pdlhip_error run(const Sig &s, pdlhip_pdl **user, int npdls, const int64_
       return make_err(PDLHIP_EUSERERROR, "Error in %s: io parameter '%s' is null", s.name.c_str(),
                       s.par_names[i].c_str());
   const int T = select_transtype(s, pdls);
-  if (!type_on_device(T))
+  if (!type_on_device(T) && !(type_is_cplx(T) && cplx_op(s)))
     return make_err(PDLHIP_EUSERERROR, "pdlhip: %s: type %s is not supported on the device", s.name.c_str(),
                     type_name(T));
+  if (type_is_cplx(T) && inputs_bad)
+    return make_err(PDLHIP_EUSERERROR, "pdlhip: %s: bad values in %s ndarrays are not supported on the device",
+                    s.name.c_str(), type_name(T));
   bool creating[MAXP] = {false};
   int req[MAXP];
   for (int i = 0; i < npdls; i++) {
@@ -1284,12 +1332,12 @@ static pdlhip_error run(const Sig &s, pdlhip_pdl **user, int npdls, const int64_
   const bool ew = s.family == FAM_BIOP || s.family == FAM_BIFUNC || s.family == FAM_UFUNC;
   for (int i = npdls - 1; i >= 0 && !err.error; i--) {
     if (creating[i] || pdls[i]->datatype == req[i]) continue;
-    if (!type_on_device(pdls[i]->datatype)) {
+    if (!type_on_device(pdls[i]->datatype) && !(type_is_cplx(pdls[i]->datatype) && cplx_op(s))) {
       err = make_err(PDLHIP_EUSERERROR, "pdlhip: %s: type %s is not supported on the device", s.name.c_str(),
                      type_name(pdls[i]->datatype));
       break;
     }
-    if (ew && i < s.nparents) {
+    if (ew && i < s.nparents && !type_is_cplx(T) && !type_is_cplx(pdls[i]->datatype)) {
       bool inout = false;
       for (int j = s.nparents; j < npdls; j++) inout = inout || orig[j] == orig[i];
       int64_t nel = 1;

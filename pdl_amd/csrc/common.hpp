@@ -161,6 +161,9 @@ inline int type_size(int t) {
   return (t >= 0 && t < PDLHIP_NTYPES) ? s[t] : 0;
 }
 inline bool type_on_device(int t) { return t >= PDLHIP_SB && t <= PDLHIP_D; }
+// float and double complex: plus, minus, mult, eq, ne, assgn, conversion and
+// matmult on the device (k_complex.hip); other ops refuse them
+inline bool type_is_cplx(int t) { return t == PDLHIP_CF || t == PDLHIP_CD; }
 
 // Streaming (nontemporal) loads and stores raise the rate of a pure stream by
 // about a tenth on this part, but leave nothing in the 256 MiB infinity cache

@@ -90,6 +90,8 @@ static hipError_t convert_D(int stype, const EwPlan &p, const void *src, void *d
 
 hipError_t launch_convert(int stype, int dtype, const EwPlan &p, const void *src, void *dst, bool bad,
                           const BadInfo &sbi, const pdlhip_value &dbv, hipStream_t s) {
+  if (type_is_cplx(stype) || type_is_cplx(dtype))
+    return bad ? hipErrorInvalidValue : launch_cplx_convert(stype, dtype, p, src, dst, s);
   switch (dtype) {
 #define X(TC) case TC: return convert_D<CType<TC>::T>(stype, p, src, dst, bad, sbi, dbv, s);
     PDLHIP_FOR_EACH_TYPE(X)

@@ -1,0 +1,118 @@
+"""Pins the oracle's float/double complex ops (C99 complex arithmetic as
+GCC compiles PDL's PP code) on the CPU: the reference's known answer for
+complex matmult and its mismatch text (t/primitive-matmult.t:29-38), the
+product's definition (ac - bd) + i(ad + bc) evaluated with separate numpy
+real ops (each step rounded once, no FMA: a contracted product fails it) and
+numpy's complex sum for finite values, infinities the C99 Annex G recovery
+gives (worked by hand from G.5.1), and the conversions between complex and
+real types (the real part survives, through intmax_t for an unsigned
+target)."""
+import numpy as np
+import pytest
+
+from oracle import pdlcpu as O
+
+
+def run(op, *arrs, types=None):
+    types = types or [None] * len(arrs)
+    ps = [O.OPdl.from_numpy(a, t) for a, t in zip(arrs, types)]
+    out = O.OPdl.null()
+    O.run(op, *ps, out)
+    return out
+
+
+def test_complex_matmult_known_answer():
+    """t/primitive-matmult.t:32-34: pdl('1 1+i 1') x pdl('2 3 i')->transpose
+    = [[5+4i]] (the PMCode dummies a 1-d left operand to (3,1))"""
+    a = O.OPdl.from_numpy(np.array([1, 1 + 1j, 1]), O.CD).dummy(1)      # dims (3,1)
+    b = O.OPdl.from_numpy(np.array([[2], [3], [1j]]), O.CD)             # dims (1,3)
+    c = O.OPdl.null()
+    O.run("matmult", a, b, c)
+    assert c.dims == [1, 1]
+    assert c.gather().tolist() == [[5 + 4j]]
+
+
+def test_complex_matmult_dim_mismatch_text():
+    """t/primitive-matmult.t:35-37: $cm1->transpose x $cm2 barfs in the
+    PMCode (the signature alone would broadcast the size-1 t); the package's
+    matmult gives the same text before any device work (plan-only here)"""
+    from pdl_amd import core as C
+    C.set_plan_only(True)
+    try:
+        a = C.shell(O.CD, [1, 3])
+        b = C.shell(O.CD, [1, 3])
+        with pytest.raises(C.PDLError) as e:
+            C.matmult(a, b)
+        assert str(e.value) == "Dim mismatch in matmult of [1x3] x [1x3]: 1 != 3"
+    finally:
+        C.set_plan_only(False)
+
+
+@pytest.mark.parametrize("t,npt", [(O.CF, np.complex64), (O.CD, np.complex128)], ids=["cfloat", "cdouble"])
+def test_elementwise_equals_numpy_on_finite_values(t, npt):
+    rng = np.random.default_rng(3)
+    n = 20000
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(npt)
+    y = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(npt)
+    y[:50] = x[:50]                                                      # some equal pairs
+    # the product from its definition, each real step a separate numpy op
+    # (numpy's own complex multiply may fuse a*c - b*d into an FMA)
+    a, b, c, d = x.real, x.imag, y.real, y.imag
+    prod = np.empty_like(x)
+    prod.real = a * c - b * d
+    prod.imag = a * d + b * c
+    for op, want in (("plus", x + y), ("minus", x - y), ("mult", prod),
+                     ("eq", (x == y).astype(npt)), ("ne", (x != y).astype(npt))):
+        got = run(op, x, y, types=[t, t]).gather()
+        assert got.dtype == npt, op
+        np.testing.assert_array_equal(got.view(np.uint8), want.view(np.uint8), err_msg=op)
+
+
+def test_matmult_is_the_sequential_complex_sum():
+    rng = np.random.default_rng(4)
+    a = (rng.standard_normal((5, 7)) + 1j * rng.standard_normal((5, 7))).astype(np.complex64)   # a(t=7,h=5)
+    b = (rng.standard_normal((7, 3)) + 1j * rng.standard_normal((7, 3))).astype(np.complex64)   # b(w=3,t=7)
+    got = run("matmult", a, b, types=[O.CF, O.CF]).gather()
+    want = np.zeros((5, 3), np.complex64)
+    for h in range(5):
+        for w in range(3):
+            cc = np.complex64(0)
+            for t in range(7):
+                x, y = a[h, t], b[t, w]
+                re = np.float32(x.real * y.real) - np.float32(x.imag * y.imag)
+                im = np.float32(x.real * y.imag) + np.float32(x.imag * y.real)
+                cc = np.complex64(complex(np.float32(cc.real + re), np.float32(cc.imag + im)))
+            want[h, w] = cc
+    np.testing.assert_array_equal(got.view(np.uint8), want.view(np.uint8))
+
+
+def test_annex_g_infinities():
+    """products whose plain formula gives NaN + iNaN: (inf + i inf)(nan + i)
+    boxes the infinite factor to (1 + i) and the NaN of the other to 0:
+    inf * ((1*0 - 1*1) + i(1*1 + 1*0)) = -inf + i inf; (inf + 0i)(nan + 0i)
+    stays NaN + iNaN (inf * 0); finite products that overflow: (1e300 +
+    1e300 i)^2 = NaN + i inf (only one part NaN, no recovery)"""
+    inf, nan = np.inf, np.nan
+    x = np.array([complex(inf, inf), complex(inf, 0), complex(1e300, 1e300)])
+    y = np.array([complex(nan, 1), complex(nan, 0), complex(1e300, 1e300)])
+    got = run("mult", x, y, types=[O.CD, O.CD]).gather()
+    assert got[0].real == -inf and got[0].imag == inf
+    assert np.isnan(got[1].real) and np.isnan(got[1].imag)
+    assert np.isnan(got[2].real) and got[2].imag == inf
+
+
+def test_conversions_with_a_complex_end():
+    z = np.array([1.75 - 2j, -3.5 + 0.25j, 1e10 + 1j])
+    # complex to real: the real part (long truncates; ulong through intmax_t)
+    c = run("assgn", z, types=[O.CD])
+    np.testing.assert_array_equal(c.gather(), z)
+    out = O.OPdl.from_numpy(np.zeros(3, np.int64), O.LL)
+    O.run("assgn", O.OPdl.from_numpy(z, O.CD), out)
+    np.testing.assert_array_equal(out.gather(), [1, -3, 10000000000])
+    out = O.OPdl.from_numpy(np.zeros(3, np.float32), O.F)
+    O.run("assgn", O.OPdl.from_numpy(z, O.CD), out)
+    np.testing.assert_array_equal(out.gather(), z.real.astype(np.float32))
+    # real to complex: (x, 0); a long plus a cfloat computes in cfloat
+    s = run("plus", np.array([1, 2, 3], np.int32), np.array([0.5j, 1, 2 + 2j], np.complex64), types=[O.L, O.CF])
+    assert s.dtype == O.CF
+    np.testing.assert_array_equal(s.gather(), np.array([1 + 0.5j, 3, 5 + 2j], np.complex64))
