@@ -39,13 +39,14 @@ def algorithmic_bytes(nx, ny, nk, ni, nj, mi, mj):
 
 
 def measured_traffic(op):
-    """HBM bytes per launch of this op's kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes (scaled by a 1 GiB copy), or None"""
+    """(HBM bytes per launch, kernel) of this op from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (scaled by a 1 GiB copy), or (None, None)"""
     try:
         with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
-            return json.load(f)["bytes_per_launch"].get(op)
+            d = json.load(f)
+        return d["bytes_per_launch"].get(op), d.get("kernels", {}).get(op)
     except (OSError, ValueError, KeyError):
-        return None
+        return None, None
 
 
 # ---------------------------------------------------------------- inputs
@@ -665,7 +666,8 @@ def main():
                                "i16 (8192,8192) 1% BAD sumover",
                    "bytes_per_step": r["total_bytes"], "frac_of_hbm_peak": round(r["value"] / HBM_PEAK_GBS, 3)},
         "roofline": {"bound": "hbm", "achieved": per["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": per["frac"], "traffic": measured_traffic(r["dominant"]), "kernel": r["dominant"]},
+                     "frac": per["frac"], "traffic": measured_traffic(r["dominant"])[0],
+                     "traffic_kernel": measured_traffic(r["dominant"])[1], "kernel": r["dominant"]},
         "per_op": r["per_op"], "checks": r["check"],
     }
     out["small"] = run_small()

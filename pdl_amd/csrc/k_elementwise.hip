@@ -108,15 +108,17 @@ __global__ __launch_bounds__(EW_THREADS) void ew_elems(const T *__restrict__ a, 
 // ---------------------------------------------------------------- tile
 // Tiles of TI (dim 0, output-contiguous) x TJ (dim q, where a transposed
 // input is contiguous).  TMASK bit k: input k is read through LDS.  For
-// 4-byte types each thread moves 16-byte vectors on both sides: the
-// transposed input is read 4-wide along j, the output written 4-wide along i.
-// 4-byte types use 32 (i) x 128 (j) tiles: 512-byte runs on the transposed
-// read, 128-byte runs on the streaming write; others 64 x 64.
-// tile of the transposing kernel: TI along the output's unit stride, TJ along
-// the transposed input's; about 33 KB of LDS per block, so four blocks share a CU
+// 4- and 8-byte types each thread moves 16-byte vectors on both sides: the
+// transposed input is read along j, the output written along i.
+// Shapes measured at 1 GiB with the reductions that follow (streaming):
+// 4-byte types 256 (i) x 32 (j) with one transposed input (1 KB runs on
+// the write, 128-byte runs on the read: 6.37 TB/s, against 5.95 for 64 x
+// 128 and 6.18 for 128 x 32) and 128 x 32 with two (6.41 against 6.02 for
+// 32 x 128); 8-byte types 64 x 64 (6.3 TB/s, against 5.8-5.9 for 128 x 32,
+// 128 x 16 and 64 x 32); others 64 x 64.  About 33 KB of LDS per block.
 template <class T, int TMASK> struct TileShape {
-  static constexpr int I = (sizeof(T) == 4 && TMASK == 3) ? 32 : 64;
-  static constexpr int J = sizeof(T) == 4 ? 128 : 64;
+  static constexpr int I = sizeof(T) == 4 ? (TMASK == 3 ? 128 : 256) : 64;
+  static constexpr int J = sizeof(T) == 4 ? 32 : 64;
 };
 constexpr int TILE_GROUP = 8;   // tile-rows (along i) visited together, column after column
 
