@@ -694,9 +694,31 @@ enum { R_SUMOVER, R_PRODOVER, R_DSUMOVER, R_DPRODOVER, R_CUMUSUMOVER, R_CUMUPROD
     else { cmin[O1] = curmin; cmax[O2] = curmax; cmin_ind[O3] = mi; cmax_ind[O4] = xi; } }); \
 } while (0)
 
+/* complex sums and products (int+ of a complex type is itself): C99
+ * complex += and *= in PDL's loop, prodover's stop at a zero product */
+#define CPLX_OVER_LOOP(ctype, INIT, OPEQ, LEFTZERO, CUMU) do { \
+  ctype *a = tr->pdls[0]->data, *b = tr->pdls[1]->data; \
+  const int64_t N_ = tr->ind_sizes[0], ia = tr->inc_sizes[0][0], ib = CUMU ? tr->inc_sizes[1][0] : 0; \
+  BLOOP(tr, thr, { ctype tmp = INIT; const ctype *ap = a + O0; ctype *bp = b + O1; \
+    for (int64_t n = 0; n < N_; n++) { tmp OPEQ ap[n * ia]; if (CUMU) bp[n * ib] = tmp; if (LEFTZERO) break; } \
+    if (!CUMU) *bp = tmp; }); \
+} while (0)
+
 static void rd_ufunc(trans *tr, int thr) {
   int op = tr->vt->opcode;
   int dt = tr->datatype;
+  if (dt == PDLC_CF || dt == PDLC_CD) {
+#define CPLX_RED(ctype) \
+    switch (op) { \
+    case R_SUMOVER: CPLX_OVER_LOOP(ctype, 0, +=, 0, 0); break; \
+    case R_PRODOVER: CPLX_OVER_LOOP(ctype, 1, *=, tmp == 0, 0); break; \
+    case R_CUMUSUMOVER: CPLX_OVER_LOOP(ctype, 0, +=, 0, 1); break; \
+    case R_CUMUPRODOVER: CPLX_OVER_LOOP(ctype, 1, *=, 0, 1); break; \
+    default: break; }
+    if (dt == PDLC_CF) { CPLX_RED(float complex) } else { CPLX_RED(double complex) }
+#undef CPLX_RED
+    return;
+  }
   switch (op) {
   case R_SUMOVER: case R_PRODOVER:
     switch (dt) {
@@ -1622,6 +1644,8 @@ int pdlc_run(const char *op, pdlc_pdl **user_pdls, int npdls, const int64_t *oth
   int ttype = transtype_select(vt, tr.pdls);
   tr.datatype = ttype;
   const int cplx_vt = vt->readdata == rd_matmult || vt->readdata == rd_assgn ||
+                      (vt->readdata == rd_ufunc && (vt->opcode == R_SUMOVER || vt->opcode == R_PRODOVER ||
+                                                    vt->opcode == R_CUMUSUMOVER || vt->opcode == R_CUMUPRODOVER)) ||
                       (vt->readdata == rd_biop && (vt->opcode == OP_PLUS || vt->opcode == OP_MINUS ||
                                                    vt->opcode == OP_MULT || vt->opcode == OP_EQ ||
                                                    vt->opcode == OP_NE));

@@ -815,6 +815,8 @@ static pdlhip_error readdata_index(const Sig &s, int T, bool bad, const Brc &b, 
 // the ops that take float and double complex on the device
 static bool cplx_op(const Sig &s) {
   if (s.family == FAM_MM || s.family == FAM_ASSGN) return true;
+  if (s.family == FAM_RED)
+    return s.op == R_SUM || s.op == R_PROD || s.op == R_CUMUSUM || s.op == R_CUMUPROD;
   return s.family == FAM_BIOP && (s.op == OP_PLUS || s.op == OP_MINUS || s.op == OP_MULT || s.op == OP_EQ ||
                                   s.op == OP_NE);
 }
@@ -832,6 +834,26 @@ static pdlhip_error readdata_cplx(const Sig &s, int T, const Brc &b, pdlhip_pdl 
     for (int q = 0; q < 3; q++) p.off[q] = b.offs[which[q]];
     if (as) e = launch_cplx_convert(T, T, p, pdls[0]->data, pdls[1]->data, g.stream);
     else e = launch_cplx_binop(T, s.op, p, pdls[0]->data, pdls[1]->data, pdls[2]->data, g.stream);
+  } else if (s.family == FAM_RED) {
+    int which[2] = {0, 1};
+    Dimset d = brc_dimset(b, which, 2);
+    RedPlan p;
+    std::memset(&p, 0, sizeof p);
+    p.npdl = 2;
+    p.n = b.ind_sizes[0];
+    p.inc_n[0] = b.inc_sizes[0][0];
+    p.inc_n[1] = (s.realdims[1] == 1) ? b.inc_sizes[1][0] : 0;
+    const bool nonempty = canonicalise(d, 1);
+    if (d.n > MAXK) return make_err(PDLHIP_EFATAL, "pdlhip: %s: too many distinct dims", s.name.c_str());
+    p.nd = nonempty ? d.n : 0;
+    p.nout = nonempty ? 1 : 0;
+    for (int k = 0; k < p.nd; k++) {
+      p.dims[k] = d.dims[k];
+      p.nout *= d.dims[k];
+      for (int q = 0; q < 2; q++) p.inc[q][k] = d.inc[q][k];
+    }
+    for (int q = 0; q < 2; q++) p.off[q] = b.offs[q];
+    e = launch_cplx_reduce(T, s.op, p, pdls[0]->data, pdls[1]->data, g.stream);
   } else {
     int which[3] = {0, 1, 2};
     Dimset d = brc_dimset(b, which, 3);

@@ -1,7 +1,9 @@
 // Complex ndarrays on the device: PDL's CF (float complex) and CD (double
 // complex) types (Types.pm), for the Ops.pd biops plus, minus, mult, eq and
 // ne (GenericTypes $A, Ops.pd:288-301), assgn and converttypei
-// (pdlconv.c:163-174), and matmult (Primitive.pd:191-245, ppdefs_all).
+// (pdlconv.c:163-174), the Ufunc.pd sums and products sumover, prodover,
+// cumusumover, cumuprodover (Ufunc.pd:88-140, int+ of a complex type is
+// itself), and matmult (Primitive.pd:191-245, ppdefs_all).
 //
 // The arithmetic is C99 complex arithmetic as GCC compiles PDL's
 // `$c() = $a() * $b()` and `cc += *ad * *bd` (no FMA contraction on the
@@ -214,6 +216,43 @@ hipError_t launch_cplx_convert(int stype, int dtype, const EwPlan &p, const void
 #undef X
   default: return hipErrorInvalidValue;
   }
+}
+
+// sumover / prodover (int+ = the complex type) and their cumulative forms
+// (Ufunc.pd:88-140): one thread per output row, PDL's sequential loop;
+// prodover stops once the running product is zero (both parts)
+template <class R, int OP>
+__global__ __launch_bounds__(256) void k_cplx_reduce(const Cplx<R> *a, Cplx<R> *b, RedPlan p) {
+  for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < p.nout; r += (int64_t)gridDim.x * 256) {
+    int64_t rest = r, oa = p.off[0], ob = p.off[1];
+    for (int d = 0; d < p.nd; d++) {
+      const int64_t i = rest % p.dims[d];
+      rest /= p.dims[d];
+      oa += i * p.inc[0][d];
+      ob += i * p.inc[1][d];
+    }
+    const bool prod = OP == R_PROD || OP == R_CUMUPROD;
+    Cplx<R> tmp = {prod ? R(1) : R(0), R(0)};
+    for (int64_t t = 0; t < p.n; t++) {
+      const Cplx<R> v = a[oa + t * p.inc_n[0]];
+      if (prod) tmp = cmul(tmp, v);
+      else tmp = {tmp.re + v.re, tmp.im + v.im};
+      if (OP == R_CUMUSUM || OP == R_CUMUPROD) b[ob + t * p.inc_n[1]] = tmp;
+      if (OP == R_PROD && tmp.re == R(0) && tmp.im == R(0)) break;
+    }
+    if (OP == R_SUM || OP == R_PROD) b[ob] = tmp;
+  }
+}
+
+hipError_t launch_cplx_reduce(int T, int op, const RedPlan &p, const void *a, void *b, hipStream_t s) {
+  if (p.nout == 0) return hipSuccess;
+#define CR(R, OP) hipLaunchKernelGGL((k_cplx_reduce<R, OP>), dim3(grid_for(p.nout)), dim3(256), 0, s, (const Cplx<R> *)a, (Cplx<R> *)b, p)
+#define CRT(R) switch (op) { case R_SUM: CR(R, R_SUM); break; case R_PROD: CR(R, R_PROD); break; \
+    case R_CUMUSUM: CR(R, R_CUMUSUM); break; case R_CUMUPROD: CR(R, R_CUMUPROD); break; default: return hipErrorInvalidValue; }
+  if (T == PDLHIP_CF) { CRT(float) } else if (T == PDLHIP_CD) { CRT(double) } else return hipErrorInvalidValue;
+#undef CRT
+#undef CR
+  return hipGetLastError();
 }
 
 hipError_t launch_cplx_matmult(int T, const MmPlan &p, const void *a, const void *b, void *c, hipStream_t s) {

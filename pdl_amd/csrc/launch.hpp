@@ -74,6 +74,7 @@ hipError_t launch_ragged_scan(int T, int op, const RagLaunch &L, hipStream_t s);
 hipError_t launch_cplx_binop(int T, int op, const EwPlan &p, const void *a, const void *b, void *c, hipStream_t s);
 hipError_t launch_cplx_convert(int stype, int dtype, const EwPlan &p, const void *src, void *dst, hipStream_t s);
 hipError_t launch_cplx_matmult(int T, const MmPlan &p, const void *a, const void *b, void *c, hipStream_t s);
+hipError_t launch_cplx_reduce(int T, int op, const RedPlan &p, const void *a, void *b, hipStream_t s);
 hipError_t launch_convert(int stype, int dtype, const EwPlan &p, const void *src, void *dst, bool bad,
                           const BadInfo &sbi, const pdlhip_value &dbv, hipStream_t s);
 // PDL's random / randsym / grandom stream (k_rng.hip).  kind 0 random, 1

@@ -5,8 +5,10 @@ operands converted to complex; infinities and NaNs through the Annex G
 product; assgn and conversions to and from real types; matmult (the
 reference's known answer, t/primitive-matmult.t:32-34, batches, views,
 the PMCode's scalar shortcut and mismatch text).  Bar: bit-exact, every NaN
-counted equal.  Refusals: bad values in complex ndarrays and the ops the
-device does not take complex for (divide, reductions)."""
+counted equal.  Sums and products (sumover, prodover and their cumulative
+forms) against the oracle in PDL's sequential order.  Refusals: bad values
+in complex ndarrays and the ops the device does not take complex for
+(divide, average)."""
 import numpy as np
 import pytest
 
@@ -146,13 +148,35 @@ def test_matmult_bit_exact(t):
     same(want, got)
 
 
+@pytest.mark.parametrize("t", [O.CF, O.CD], ids=["cfloat", "cdouble"])
+@pytest.mark.parametrize("op", ["sumover", "prodover", "cumusumover", "cumuprodover"])
+def test_sums_and_products_bit_exact(t, op):
+    """Ufunc.pd:88-140 in PDL's sequential order: rows of a transposed view,
+    a broadcast over three dims, zeros (prodover stops at a zero product),
+    infinities and NaNs"""
+    rng = np.random.default_rng(40 + t)
+    with np.errstate(invalid="ignore"):
+        x = (crand(rng, (3, 70, 45), t, special=True) * 0.9).astype(NPT[t])
+    x[1, 5, 7] = 0
+    x[2, 9, :] = 0
+    ox, dx = O.OPdl.from_numpy(x, t), C.ndarray.from_numpy(x, t)
+    for v in ((), (("xchg", 0, 1),)):
+        o, d = ox, dx
+        for step in v:
+            o, d = getattr(o, step[0])(*step[1:]), getattr(d, step[0])(*step[1:])
+        oc, dc = O.OPdl.null(), C.ndarray()
+        O.run(op, o, oc)
+        C._run(op, [d, dc])
+        same(oc, dc)
+
+
 def test_refusals():
     x = C.ndarray.from_numpy(np.ones(4, np.complex128), O.CD)
     with pytest.raises(pdl_amd.PDLError) as e:
         C._run("divide", [x, x, C.ndarray()])
     assert "not supported on the device" in str(e.value)
     with pytest.raises(pdl_amd.PDLError) as e:
-        C._run("sumover", [x, C.ndarray()])
+        C._run("average", [x, C.ndarray()])
     assert "not supported on the device" in str(e.value)
     xb = C.ndarray.from_numpy(np.ones(4, np.complex128), O.CD, badflag=True)
     with pytest.raises(pdl_amd.PDLError) as e:

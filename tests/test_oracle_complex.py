@@ -116,3 +116,36 @@ def test_conversions_with_a_complex_end():
     s = run("plus", np.array([1, 2, 3], np.int32), np.array([0.5j, 1, 2 + 2j], np.complex64), types=[O.L, O.CF])
     assert s.dtype == O.CF
     np.testing.assert_array_equal(s.gather(), np.array([1 + 0.5j, 3, 5 + 2j], np.complex64))
+
+
+@pytest.mark.parametrize("t,npt", [(O.CF, np.complex64), (O.CD, np.complex128)], ids=["cfloat", "cdouble"])
+def test_complex_sums_and_products(t, npt):
+    """Ufunc.pd:88-140 for a complex type (int+ is the type itself): the sum
+    and the cumulative sum are numpy's sequential cumsum; the product is the
+    sequential product of the definition; prodover stops at a zero running
+    product, so a NaN after a zero does not reach the result"""
+    rng = np.random.default_rng(5)
+    x = (rng.standard_normal((6, 33)) + 1j * rng.standard_normal((6, 33))).astype(npt)
+    cs = run("cumusumover", x, types=[t]).gather()
+    np.testing.assert_array_equal(cs.view(np.uint8), np.cumsum(x, axis=1).view(np.uint8))
+    s = run("sumover", x, types=[t]).gather()
+    np.testing.assert_array_equal(s.view(np.uint8), np.cumsum(x, axis=1)[:, -1].copy().view(np.uint8))
+    rt = x.real.dtype.type
+
+    def cmul(p, q):
+        return complex(rt(rt(p.real * q.real) - rt(p.imag * q.imag)), rt(rt(p.real * q.imag) + rt(p.imag * q.real)))
+
+    y = x * npt(0.9)
+    want = np.empty_like(y)
+    for r in range(y.shape[0]):
+        acc = npt(1)
+        for k in range(y.shape[1]):
+            acc = npt(cmul(acc, y[r, k]))
+            want[r, k] = acc
+    cp = run("cumuprodover", y, types=[t]).gather()
+    np.testing.assert_array_equal(cp.view(np.uint8), want.view(np.uint8))
+    p = run("prodover", y, types=[t]).gather()
+    np.testing.assert_array_equal(p.view(np.uint8), want[:, -1].copy().view(np.uint8))
+    z = np.array([[2 + 1j, 0, complex(np.nan, 0), 3]], npt)
+    assert run("prodover", z, types=[t]).gather().tolist() == [0j]
+    assert np.isnan(run("cumuprodover", z, types=[t]).gather()[0, 2].real)
