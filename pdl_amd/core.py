@@ -635,12 +635,58 @@ class ndarray:
     def make_physical(self):
         return self
 
-    def flat(self):
-        """clump(-1): a dense 1-D copy (Slices.pd:1373-1400)"""
-        c = self.copy()
-        c.dims = [c.nelem]
-        c.incs = [1]
+    def clump(self, n):
+        """Slices.pd:1373-1400 (_clump_int): the first n dims as one (n = -1:
+        up to broadcastids[0]; n < -1 counts from the end; n past ndims
+        clumps them all), a child with dataflow.  Dims that sit contiguously
+        in memory clump into a view of the same data; others into a dense
+        copy that is refreshed from the parent and written back into it."""
+        nd = self.ndims
+        if n > nd:
+            n = nd
+        if n < -1:
+            n = nd + n + 1
+        nrem = self.broadcastids()[0] if n == -1 else n
+        d1 = 1
+        for d in self.dims[:nrem]:
+            d1 *= d
+        dims = [d1] + list(self.dims[nrem:])
+        # broadcast ids move with the dims ($SETDELTABROADCASTIDS(1-nrem))
+        bids = [b + 1 - nrem for b in self.bids] if self.bids else None
+        # the clumped dims merge into one stride when each non-unit dim
+        # starts where the one before it ends
+        nz = [(self.dims[i], self.incs[i]) for i in range(nrem) if self.dims[i] != 1]
+        if d1 == 0 or all(nz[k + 1][1] == nz[k][1] * nz[k][0] for k in range(len(nz) - 1)):
+            v = self._view(dims, [nz[0][1] if nz else 1] + list(self.incs[nrem:]), self.offs)
+            v.bids = bids
+            return v
+        # element i of the child is element i of the parent in PDL's order
+        # (EquivCPOffsCode): the copies run over plain views, without the
+        # parent's broadcast ids
+        c = ndarray.new_dense(self.dtype, self.dims)
+        c.state |= self.state & L.BADVAL
+        c.badvalue_ = self.badvalue_
+        _run("assgn", [self._view(self.dims, self.incs, self.offs), c])
+        c.dims, c.incs, c.bids = dims, _dense_incs(dims), bids
+        pdims = list(self.dims)
+
+        def as_parent(child):
+            return child._view(pdims, _dense_incs(pdims), 0)
+
+        def plain(p):
+            return p._view(p.dims, p.incs, p.offs)
+
+        def fwd(f):
+            _call("assgn", [plain(f.parents[0]), as_parent(f.child)])
+
+        def back(f):
+            _call("assgn", [as_parent(f.child), plain(f.parents[0])])
+        c._buf.flow = _Flow([self], c._alias(), fwd, back)
         return c
+
+    def flat(self):
+        """clump(-1) (Core.pm flat): all dims as one, with dataflow"""
+        return self.clump(-1)
 
     def clump_all(self):
         return self.flat()

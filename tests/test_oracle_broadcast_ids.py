@@ -150,3 +150,15 @@ def test_mismatched_explicit_dims_croak():
     with pytest.raises(O.OracleError) as e:
         O.run("assgn", a.broadcastI(1, [1]), c.broadcastI(1, [1]))
     assert "Mismatched implicit broadcast dimension 0: should be 4, is 3" in str(e.value)
+
+
+def test_clump_dims_of_views():
+    """_clump_int's dims (Slices.pd:1376-1392): n past ndims clumps all,
+    n < -1 counts from the end, -1 stops at broadcastids[0], whose ids move
+    by 1 - n; contiguous dims clump into a view"""
+    x = C.shell(C.L.D, [4, 5, 6])
+    assert x.clump(2).dims == [20, 6] and x.clump(2).incs == [1, 20]
+    assert x.clump(-2).dims == [20, 6] and x.clump(0).dims == [1, 4, 5, 6] and x.clump(7).dims == [120]
+    b = x.broadcast(2)                                               # dims (4,5,6), ids [2,3]
+    v = b.clump(-1)
+    assert v.dims == [20, 6] and v.broadcastids() == [1, 2]
