@@ -213,8 +213,20 @@ __device__ __forceinline__ void list_mfma_tile(const MmItem &it, int64_t tm, int
       rb[q] = b[(bw < W ? bw : W - 1) * b_w + (bk < K ? bk : K - 1) * b_t];
     }
   };
+  // interior chunks of the common layout (a's k and b's w unit-stride):
+  // every staged element lands at a slot a compile-time distance from the
+  // thread's first, with no bounds tests
+  const bool plain = a_t == 1 && b_w == 1;
   auto stage = [&](int buf, int k0) {
     const bool inside = inside_hw && k0 + KS <= K;
+    if (plain && inside) {
+#pragma unroll
+      for (int q = 0; q < NE; q++) {
+        As[buf][fa][sa + SK * q] = ra[q];
+        Bs[buf][sb + 4 * q][fb] = rb[q];
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < NE; q++) {
       const int s1 = sa + da * q, s2 = sb + db * q;
