@@ -182,3 +182,23 @@ def test_refusals():
     with pytest.raises(pdl_amd.PDLError) as e:
         C._run("plus", [xb, x, C.ndarray()])
     assert "bad values" in str(e.value)
+
+
+def test_readdata_entry_with_pdl_plans():
+    """pdlhip_readdata (the vtable hook of the XS shim) over the plans PDL's
+    redodims builds for complex ops: a broadcast cfloat plus with a
+    transposed operand and a batched cdouble matmult"""
+    from test_gpu_parity import readdata_with_oracle_plan
+    rng = np.random.default_rng(77)
+    x = crand(rng, (5, 30, 40), O.CF)
+    y = crand(rng, (30, 1), O.CF)
+    ins = [(O.OPdl.from_numpy(x, O.CF).xchg(0, 1), C.ndarray.from_numpy(x, O.CF).xchg(0, 1)),
+           (O.OPdl.from_numpy(y, O.CF).xchg(0, 1), C.ndarray.from_numpy(y, O.CF).xchg(0, 1))]
+    outs_o, outs_d = readdata_with_oracle_plan("plus", [i[0] for i in ins], [i[1] for i in ins], 1)
+    same(outs_o[0], outs_d[0])
+    a = crand(rng, (4, 9, 11), O.CD)
+    b = crand(rng, (4, 11, 7), O.CD)
+    ins = [(O.OPdl.from_numpy(a, O.CD), C.ndarray.from_numpy(a, O.CD)),
+           (O.OPdl.from_numpy(b, O.CD), C.ndarray.from_numpy(b, O.CD))]
+    outs_o, outs_d = readdata_with_oracle_plan("matmult", [i[0] for i in ins], [i[1] for i in ins], 1)
+    same(outs_o[0], outs_d[0])
