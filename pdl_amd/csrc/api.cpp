@@ -511,6 +511,19 @@ static int64_t bcast_id(const pdlhip_pdl *p, int k) {
 static int bcast_nids(const pdlhip_pdl *p) {
   return p->nbroadcastids > 0 ? std::min(p->nbroadcastids, PDLHIP_MAXBIDS) : 1;
 }
+// an ndarray carrying explicit broadcast ids (a broadcastI view)
+static bool explicit_ids(const pdlhip_pdl *p) {
+  return p && (p->nbroadcastids > 1 || (p->nbroadcastids == 1 && p->broadcastids[0] != p->ndims));
+}
+// the entry points with planners of their own (ranges, ragged lists, fused
+// chains) take implicit broadcasting only
+static pdlhip_error refuse_explicit_ids(const char *who, pdlhip_pdl *const *ps, int n) {
+  for (int i = 0; i < n; i++)
+    if (explicit_ids(ps[i]))
+      return make_err(PDLHIP_EUSERERROR, "%s: ndarray %d has explicit broadcast ids, which this entry point does "
+                      "not take", who, i);
+  return ok();
+}
 
 // pdl_initbroadcaststruct (implicit broadcast dims, then the dims of the
 // explicit broadcast ids, nobl = 2) + checks + output creation + core inc
@@ -2352,6 +2365,10 @@ static pdlhip_error rangeb(pdlhip_pdl *parent, pdlhip_pdl *ind, const int64_t *s
                            const char *boundary, pdlhip_pdl *child, bool back) {
   RETURN_IF(ensure_init());
   if (!parent || !ind || !child) return make_err(PDLHIP_EUSERERROR, "pdlhip: rangeb: NULL ndarray");
+  {
+    pdlhip_pdl *ps[3] = {parent, ind, child};
+    RETURN_IF(refuse_explicit_ids("pdlhip: rangeb", ps, 3));
+  }
   if ((parent->state & PDLHIP_NOMYDIMS) || (ind->state & PDLHIP_NOMYDIMS))
     return make_err(PDLHIP_EUSERERROR, "pdlhip: rangeb: the source and the index must exist");
   if (!type_on_device(parent->datatype) || !type_on_device(ind->datatype))
@@ -2584,6 +2601,10 @@ pdlhip_error pdlhip_ragged_prepare(int nitems, int ninputs, pdlhip_pdl **inputs,
                                    pdlhip_ragged **handle) {
   if (!handle) return make_err(PDLHIP_EUSERERROR, "pdlhip_ragged_prepare: no handle");
   RETURN_IF(ensure_init());
+  if (inputs && nitems > 0 && ninputs > 0)
+    RETURN_IF(refuse_explicit_ids("pdlhip_ragged_prepare", inputs, nitems * ninputs));
+  if (outputs && nitems > 0) RETURN_IF(refuse_explicit_ids("pdlhip_ragged_prepare", outputs, nitems));
+  if (values && nitems > 0) RETURN_IF(refuse_explicit_ids("pdlhip_ragged_prepare", values, nitems));
   RaggedH *h = nullptr;
   pdlhip_error e = ragged_prepare(nitems, ninputs, inputs, nnodes, nodes, final_op, outputs, values, &h);
   if (e.error) return e;
@@ -2597,6 +2618,11 @@ pdlhip_error pdlhip_chain_prepare(int ninputs, pdlhip_pdl **inputs, int nnodes, 
                                   pdlhip_ragged **handle) {
   if (!handle) return make_err(PDLHIP_EUSERERROR, "pdlhip_chain_prepare: no handle");
   RETURN_IF(ensure_init());
+  if (inputs && ninputs > 0) RETURN_IF(refuse_explicit_ids("pdlhip_chain_prepare", inputs, ninputs));
+  {
+    pdlhip_pdl *ps[2] = {value, result};
+    RETURN_IF(refuse_explicit_ids("pdlhip_chain_prepare", ps, 2));
+  }
   RaggedH *h = nullptr;
   pdlhip_error e = chain_prepare(ninputs, inputs, nnodes, nodes, final_op, value, result, &h);
   if (e.error) return e;

@@ -102,3 +102,14 @@ def test_prepared_matmult_list_refuses_entries_that_cannot_join(plan_only):
     with pytest.raises(PDLError) as e:
         C.MatmultList([C.shell(O.F, [3, 4]), C.shell(O.D, [5, 2])], [C.shell(O.F, [6, 3]), C.shell(O.D, [7, 5])])
     assert "another type than the first" in str(e.value)
+
+
+def test_explicit_broadcast_ids_are_refused(plan_only):
+    """the ragged and chain planners take implicit broadcasting only"""
+    a = C.shell(O.F, [4, 3]).broadcast(1)
+    with pytest.raises(PDLError) as e:
+        C.Chain([a, C.shell(O.F, [4, 3])], [("plus", 0, 1)], "sumover")
+    assert "explicit broadcast ids" in str(e.value)
+    with pytest.raises(PDLError) as e:
+        C.Ragged([[C.shell(O.F, [4]).broadcast(0)]], [], "sumover")
+    assert "explicit broadcast ids" in str(e.value)
