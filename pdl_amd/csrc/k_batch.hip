@@ -240,7 +240,9 @@ __device__ __forceinline__ void list_mfma_tile(const MmItem &it, int64_t tm, int
     lf32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; r++) acc[r] = 0.0f;
-    load(0);
+    // an empty inner dimension loads nothing (its clamp would reach index
+    // -1) and stores the zero sums
+    if (K > 0) load(0);
     int buf = 0;
     for (int k0 = 0; k0 < K; k0 += KS) {
       stage(buf, k0);
@@ -272,7 +274,9 @@ __device__ __forceinline__ void list_mfma_tile(const MmItem &it, int64_t tm, int
     for (int i = 0; i < 2; i++)
 #pragma unroll
       for (int j = 0; j < 2; j++) acc[i][j] = ld4{0.0, 0.0, 0.0, 0.0};
-    load(0);
+    // an empty inner dimension loads nothing (its clamp would reach index
+    // -1) and stores the zero sums
+    if (K > 0) load(0);
     int buf = 0;
     for (int k0 = 0; k0 < K; k0 += KS) {
       stage(buf, k0);
@@ -431,7 +435,7 @@ __device__ __forceinline__ void list_bf16x3_tile(const MmItem &it, int64_t tm, i
   lf32x16 acc;
 #pragma unroll
   for (int r = 0; r < 16; r++) acc[r] = 0.0f;
-  load(0);
+  if (K > 0) load(0);                  // an empty inner dimension loads nothing
   int buf = 0;
   const int fr = lane & 31, fseg = lane >> 5;
   const int ar_ = 32 * wr + fr, br_ = 32 * wc + fr;
@@ -499,7 +503,7 @@ static hipError_t list_bf16x3(const MmItem *d_items, const MmItem *h_items, int 
   }
   for (int i = 0; i < nitems; i++) {
     const int64_t ntm = (h_items[i].H + 63) / 64, ntn = (h_items[i].W + 63) / 64;
-    if (ntm * ntn == 0 || h_items[i].T == 0) continue;
+    if (ntm * ntn == 0) continue;             // an empty k stores zeros
     hipLaunchKernelGGL(list_bf16x3_one, dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, d_items, i, ntn);
   }
   return hipGetLastError();
@@ -516,7 +520,7 @@ static hipError_t list_mfma_T(const MmItem *d_items, const MmItem *h_items, int 
   }
   for (int i = 0; i < nitems; i++) {
     const int64_t ntm = (h_items[i].H + 63) / 64, ntn = (h_items[i].W + 63) / 64;
-    if (ntm * ntn == 0 || h_items[i].T == 0) continue;
+    if (ntm * ntn == 0) continue;             // an empty k stores zeros
     hipLaunchKernelGGL((list_mfma_one<T, KS>), dim3((unsigned)(ntm * ntn)), dim3(256), 0, s, d_items, i, ntn);
   }
   return hipGetLastError();

@@ -5,7 +5,8 @@ through the SIMT 32/64 tiles, the FP32 matrix-core kernel with its default
 and its half k stage, and the BF16 three-way split; double lists through the
 SIMT tiles and both FP64 matrix-core variants.  Each forced run must reuse
 the cached choice (no timing) and give the same result in its queue and
-per-product launches."""
+per-product launches, and products over an empty inner dimension must come
+out as zeros in both."""
 import json
 import os
 import subprocess
@@ -43,8 +44,20 @@ for mode in ("queue", "each"):
                np.linalg.norm(truth.astype(np.float64), axis=1))
         worst = max(worst, float(err.max()))
 ml.free()
+# products over an empty inner dimension (h x 0 times 0 x w) are zeros in
+# both launch modes, per-product first
+eh = [np.zeros((h, 0), dt) for h in (5, 70)]
+eb = [np.zeros((0, w), dt) for w in (7, 65)]
+ml = C.MatmultList([C.ndarray.from_numpy(a, t) for a in eh], [C.ndarray.from_numpy(b, t) for b in eb])
+empty_zero = True
+for mode in ("each", "queue"):
+    ml.outputs[1].assign(C.ndarray.from_numpy(np.ones((70, 65), dt), t))
+    ml.launch(mode)
+    outs = [o.numpy() for o in ml.outputs]
+    empty_zero = empty_zero and [o.shape for o in outs] == [(5, 7), (70, 65)] and all(not o.any() for o in outs)
+ml.free()
 out = C.tune_stats()
-out.update({"worst": worst, "queue_equals_each": same})
+out.update({"worst": worst, "queue_equals_each": same, "empty_zero": empty_zero})
 print(json.dumps(out))
 """ % ROOT
 
@@ -64,10 +77,10 @@ def test_every_list_kernel_candidate(tmp_path, t, cands, tol):
     first = run(env, t)
     assert first["measured"] >= 1 and first["worst"] < tol, first
     keys = [line.split()[0] for line in path.read_text().splitlines() if line.startswith("list:")]
-    assert len(keys) == 1, keys
+    assert len(keys) == 2, keys                     # the random list, the empty-k list
     for c in cands:
-        path.write_text("%s %d\n" % (keys[0], c))
+        path.write_text("".join("%s %d\n" % (k, c) for k in keys))
         got = run(env, t)
         assert got["measured"] == 0 and got["reused"] >= 1, (c, got)
         assert got["worst"] < tol, (c, got)
-        assert got["queue_equals_each"], (c, got)
+        assert got["queue_equals_each"] and got["empty_zero"], (c, got)
