@@ -80,7 +80,9 @@ void pdlc_default_badval(int t, pdlc_anyval *out) {
 
 static int type_supported(int t) { return t >= PDLC_SB && t <= PDLC_D; }
 /* float and double complex (C99 _Complex, as GCC compiles PDL's PP code):
- * restated for plus, minus, mult, eq, ne, assgn, conversion and matmult */
+ * restated for plus, minus, mult, divide (libgcc's __divsc3/__divdc3, which
+ * GCC calls for `/`), eq, ne, assgn, conversion, matmult and the sums and
+ * products over a dim */
 static int type_cplx(int t) { return t == PDLC_CF || t == PDLC_CD; }
 
 /* X-macro over the types this restatement computes in:
@@ -359,6 +361,7 @@ static void rd_biop(trans *tr, int thr) {
     CPLX_CASE(OP_PLUS, av + bv)
     CPLX_CASE(OP_MULT, av * bv)
     CPLX_CASE(OP_MINUS, av - bv)
+    CPLX_CASE(OP_DIVIDE, av / bv)
     CPLX_CASE(OP_EQ, av == bv)
     CPLX_CASE(OP_NE, av != bv)
 #undef CPLX_CASE
@@ -1647,8 +1650,8 @@ int pdlc_run(const char *op, pdlc_pdl **user_pdls, int npdls, const int64_t *oth
                       (vt->readdata == rd_ufunc && (vt->opcode == R_SUMOVER || vt->opcode == R_PRODOVER ||
                                                     vt->opcode == R_CUMUSUMOVER || vt->opcode == R_CUMUPRODOVER)) ||
                       (vt->readdata == rd_biop && (vt->opcode == OP_PLUS || vt->opcode == OP_MINUS ||
-                                                   vt->opcode == OP_MULT || vt->opcode == OP_EQ ||
-                                                   vt->opcode == OP_NE));
+                                                   vt->opcode == OP_MULT || vt->opcode == OP_DIVIDE ||
+                                                   vt->opcode == OP_EQ || vt->opcode == OP_NE));
   if (!type_supported(ttype) && !(type_cplx(ttype) && cplx_vt))
     return make_error(err, 1, "pdlcpu: %s: type %s is not supported by this restatement", vt->name,
                       pdlc_type_name(ttype));

@@ -830,8 +830,8 @@ static bool cplx_op(const Sig &s) {
   if (s.family == FAM_MM || s.family == FAM_ASSGN) return true;
   if (s.family == FAM_RED)
     return s.op == R_SUM || s.op == R_PROD || s.op == R_CUMUSUM || s.op == R_CUMUPROD;
-  return s.family == FAM_BIOP && (s.op == OP_PLUS || s.op == OP_MINUS || s.op == OP_MULT || s.op == OP_EQ ||
-                                  s.op == OP_NE);
+  return s.family == FAM_BIOP && (s.op == OP_PLUS || s.op == OP_MINUS || s.op == OP_MULT || s.op == OP_DIVIDE ||
+                                  s.op == OP_EQ || s.op == OP_NE);
 }
 
 // readdata of a complex op (k_complex.hip); every operand has type T here

@@ -63,11 +63,49 @@ template <class R> __device__ inline Cplx<R> cmul(Cplx<R> x, Cplx<R> y) {
   return {re, im};
 }
 
+template <class R> __device__ inline R cfabs(R v) { return v < R(0) ? -v : (v == R(0) ? R(0) : v); }
+
+// x / y as the C99 `/` of two complex operands compiles under GCC 11: a call
+// to libgcc's __divsc3 / __divdc3, Smith's method in the operands' own type
+// (the ratio of y's smaller part to its larger), then C99 Annex G's recovery
+// of infinities and zeros when both parts came out NaN (G.5.1)
+template <class R> __device__ inline Cplx<R> cdiv(Cplx<R> x, Cplx<R> y) {
+  R a = x.re, b = x.im, c = y.re, d = y.im, re, im;
+  if (cfabs(c) < cfabs(d)) {
+    const R ratio = c / d, denom = c * ratio + d;
+    re = (a * ratio + b) / denom;
+    im = (b * ratio - a) / denom;
+  } else {
+    const R ratio = d / c, denom = d * ratio + c;
+    re = (b * ratio + a) / denom;
+    im = (b - a * ratio) / denom;
+  }
+  if (__builtin_isnan(re) && __builtin_isnan(im)) {
+    const R inf = __builtin_huge_val();
+    if (c == R(0) && d == R(0) && (!__builtin_isnan(a) || !__builtin_isnan(b))) {
+      re = csign<R>(inf, c) * a;                                // a nonzero x over zero
+      im = csign<R>(inf, c) * b;
+    } else if ((__builtin_isinf(a) || __builtin_isinf(b)) && __builtin_isfinite(c) && __builtin_isfinite(d)) {
+      a = csign<R>(__builtin_isinf(a) ? R(1) : R(0), a);        // infinite x over finite y
+      b = csign<R>(__builtin_isinf(b) ? R(1) : R(0), b);
+      re = inf * (a * c + b * d);
+      im = inf * (b * c - a * d);
+    } else if ((__builtin_isinf(c) || __builtin_isinf(d)) && __builtin_isfinite(a) && __builtin_isfinite(b)) {
+      c = csign<R>(__builtin_isinf(c) ? R(1) : R(0), c);        // finite x over infinite y
+      d = csign<R>(__builtin_isinf(d) ? R(1) : R(0), d);
+      re = R(0) * (a * c + b * d);
+      im = R(0) * (b * c - a * d);
+    }
+  }
+  return {re, im};
+}
+
 template <class R> __device__ inline Cplx<R> cbinop(int op, Cplx<R> x, Cplx<R> y) {
   switch (op) {
   case OP_PLUS: return {x.re + y.re, x.im + y.im};
   case OP_MINUS: return {x.re - y.re, x.im - y.im};
   case OP_MULT: return cmul(x, y);
+  case OP_DIVIDE: return cdiv(x, y);
   case OP_EQ: return {(x.re == y.re && x.im == y.im) ? R(1) : R(0), R(0)};
   case OP_NE: return {(x.re != y.re || x.im != y.im) ? R(1) : R(0), R(0)};
   default: return x;

@@ -2,13 +2,14 @@
 the oracle's C99 complex arithmetic, through the C ABI: plus, minus, mult,
 eq, ne with broadcasting, views, swapped and inplace operands and real
 operands converted to complex; infinities and NaNs through the Annex G
-product; assgn and conversions to and from real types; matmult (the
+product; divide through libgcc's Smith's method and its recovery; assgn
+and conversions to and from real types; matmult (the
 reference's known answer, t/primitive-matmult.t:32-34, batches, views,
 the PMCode's scalar shortcut and mismatch text).  Bar: bit-exact, every NaN
 counted equal.  Sums and products (sumover, prodover and their cumulative
 forms) against the oracle in PDL's sequential order.  Refusals: bad values
 in complex ndarrays and the ops the device does not take complex for
-(divide, average)."""
+(power; average computes in long double for a complex input)."""
 import numpy as np
 import pytest
 
@@ -55,9 +56,9 @@ def same(o, d):
 
 
 @pytest.mark.parametrize("t", [O.CF, O.CD], ids=["cfloat", "cdouble"])
-@pytest.mark.parametrize("op", ["plus", "minus", "mult", "eq", "ne"])
+@pytest.mark.parametrize("op", ["plus", "minus", "mult", "divide", "eq", "ne"])
 def test_elementwise_bit_exact(t, op):
-    rng = np.random.default_rng(10 * t + ["plus", "minus", "mult", "eq", "ne"].index(op))
+    rng = np.random.default_rng(10 * t + ["plus", "minus", "mult", "divide", "eq", "ne"].index(op))
     x = crand(rng, (6, 50, 33), t, special=True)
     y = crand(rng, (50, 1), t, special=True)              # broadcast along dim 0, size-1 dim
     y[3, 0] = x[0, 3, 0]
@@ -74,6 +75,28 @@ def test_elementwise_bit_exact(t, op):
     O.run(op, ox.xchg(0, 2), O.OPdl.from_numpy(r, O.F), oc)
     C._run(op, [dx.xchg(0, 2), C.ndarray.from_numpy(r, O.F), dc])
     same(oc, dc)
+
+
+@pytest.mark.parametrize("t", [O.CF, O.CD], ids=["cfloat", "cdouble"])
+def test_divide_special_and_wide_operands(t):
+    """every pair of a grid of special parts (signed zeros, infinities, NaN,
+    the type's extremes) and values of wide exponent range through libgcc's
+    Smith's method and its Annex G recovery"""
+    rt = np.float32 if t == O.CF else np.float64
+    fi = np.finfo(rt)
+    parts = np.array([0.0, -0.0, 1.0, -2.5, np.inf, -np.inf, np.nan, fi.max, fi.tiny, fi.tiny * fi.eps], rt)
+    vals = (parts[:, None] + 1j * parts[None, :]).reshape(-1).astype(NPT[t])
+    x = np.repeat(vals, vals.size)
+    y = np.tile(vals, vals.size)
+    rng = np.random.default_rng(12 + t)
+    span = 60 if t == O.CF else 400
+    w = lambda: (rng.standard_normal(x.size) * 2.0 ** rng.integers(-span, span, x.size)).astype(rt)
+    x2, y2 = (w() + 1j * w()).astype(NPT[t]), (w() + 1j * w()).astype(NPT[t])
+    for xa, ya in ((x, y), (x2, y2)):
+        oc, dc = O.OPdl.null(), C.ndarray()
+        O.run("divide", O.OPdl.from_numpy(xa, t), O.OPdl.from_numpy(ya, t), oc)
+        C._run("divide", [C.ndarray.from_numpy(xa, t), C.ndarray.from_numpy(ya, t), dc])
+        same(oc, dc)
 
 
 def test_inplace_and_integer_operands():
@@ -173,7 +196,7 @@ def test_sums_and_products_bit_exact(t, op):
 def test_refusals():
     x = C.ndarray.from_numpy(np.ones(4, np.complex128), O.CD)
     with pytest.raises(pdl_amd.PDLError) as e:
-        C._run("divide", [x, x, C.ndarray()])
+        C._run("power", [x, x, C.ndarray()])
     assert "not supported on the device" in str(e.value)
     with pytest.raises(pdl_amd.PDLError) as e:
         C._run("average", [x, C.ndarray()])

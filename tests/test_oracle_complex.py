@@ -6,7 +6,9 @@ real ops (each step rounded once, no FMA: a contracted product fails it) and
 numpy's complex sum for finite values, infinities the C99 Annex G recovery
 gives (worked by hand from G.5.1), and the conversions between complex and
 real types (the real part survives, through intmax_t for an unsigned
-target)."""
+target).  Division: libgcc's Smith's method (GCC 11 calls __divsc3 /
+__divdc3 for `/`) restated with separate numpy real ops on values of wide
+range, and the Annex G recovery cases worked by hand."""
 import numpy as np
 import pytest
 
@@ -149,3 +151,56 @@ def test_complex_sums_and_products(t, npt):
     z = np.array([[2 + 1j, 0, complex(np.nan, 0), 3]], npt)
     assert run("prodover", z, types=[t]).gather().tolist() == [0j]
     assert np.isnan(run("cumuprodover", z, types=[t]).gather()[0, 2].real)
+
+
+def smith(x, y):
+    """Smith's method, each real step a separate numpy op in the type: the
+    ratio of y's smaller part to its larger"""
+    a, b, c, d = x.real, x.imag, y.real, y.imag
+    with np.errstate(all="ignore"):
+        first = np.abs(c) < np.abs(d)
+        r1 = c / d
+        den1 = c * r1 + d
+        re1, im1 = (a * r1 + b) / den1, (b * r1 - a) / den1
+        r2 = d / c
+        den2 = d * r2 + c
+        re2, im2 = (b * r2 + a) / den2, (b - a * r2) / den2
+    out = np.empty_like(x)
+    out.real = np.where(first, re1, re2)
+    out.imag = np.where(first, im1, im2)
+    return out
+
+
+@pytest.mark.parametrize("t,npt", [(O.CF, np.complex64), (O.CD, np.complex128)], ids=["cfloat", "cdouble"])
+def test_divide_is_smiths_method(t, npt):
+    rng = np.random.default_rng(6)
+    n = 20000
+    rt = np.float32 if t == O.CF else np.float64
+    span = 60 if t == O.CF else 400
+    def wide():
+        return (rng.standard_normal(n) * 2.0 ** rng.integers(-span, span, n)).astype(rt)
+    x = (wide() + 1j * wide()).astype(npt)
+    y = (wide() + 1j * wide()).astype(npt)
+    y[:40] = y[:40].real.astype(npt)                                     # real divisors
+    y[40:80] = (1j * y[40:80].imag).astype(npt)                          # imaginary divisors
+    got = run("divide", x, y, types=[t, t]).gather()
+    want = smith(x, y)
+    ok = ~(np.isnan(want.real) & np.isnan(want.imag))
+    np.testing.assert_array_equal(got[ok].view(rt), want[ok].view(rt))
+
+
+def test_divide_annex_g_cases():
+    """(1+i)/(±0+0i) = ±inf(1+i); (inf+i inf)/2 boxes x to (1+i): inf(2) + i
+    inf(2); (1+i)/(inf+i inf) boxes y: 0(1+1) + i 0(1-1) = 0 + 0i; 0/0 stays
+    NaN + iNaN; (inf+i inf)/(1+i) has one NaN part only (no recovery)"""
+    inf, nan = np.inf, np.nan
+    x = np.array([1 + 1j, 1 + 1j, complex(inf, inf), 1 + 1j, 0j, complex(inf, inf), 3 - 4j])
+    y = np.array([0j, complex(-0.0, 0.0), 2 + 0j, complex(inf, inf), 0j, 1 + 1j, 1 + 2j])
+    for t in (O.CF, O.CD):
+        got = run("divide", x, y, types=[t, t]).gather()
+        assert got[0] == complex(inf, inf) and got[1] == complex(-inf, -inf)
+        assert got[2] == complex(inf, inf)
+        assert got[3] == 0j and not np.signbit(got[3].real) and not np.signbit(got[3].imag)
+        assert np.isnan(got[4].real) and np.isnan(got[4].imag)
+        assert got[5].real == inf and np.isnan(got[5].imag)
+        assert got[6] == -1 - 2j                                         # (3-4i)/(1+2i), exact
